@@ -1,0 +1,214 @@
+/*
+ * fastmc_amd.h -- C ABI of the MI355X (gfx950) trial-move energy path.
+ *
+ * This is the drop-in boundary between a GCMC driver (the reference's
+ * Fortran move loop, or this repository's own Fortran driver) and the HIP
+ * kernels.  Everything is plain C: opaque handles, plain pointers and sizes,
+ * scalars by value, int status returns (0 = OK, negative = FMC_ERR_*).
+ *
+ * Units are the reference's internal DL_POLY units (energy 10 J/mol, length
+ * Angstrom, charge e); conversion to kcal/mol (engunit 418.4) stays in the
+ * driver, exactly as in the reference.  Atom-type indices are 1-based
+ * (ltpsit/ltype); guest-type and molecule indices are 0-based.
+ *
+ * Citations below are to the reference checkout (fastmc/...).
+ */
+#ifndef FASTMC_AMD_H
+#define FASTMC_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FMC_OK 0
+#define FMC_ERR_ARG -1         /* bad argument (index, size, NULL) */
+#define FMC_ERR_STATE -2       /* call out of order (e.g. commit w/o trial) */
+#define FMC_ERR_UNSUPPORTED -3 /* imcon / potential not on this path */
+#define FMC_ERR_CAPACITY -4    /* guest capacity exceeded (reference err 75) */
+#define FMC_ERR_HIP -5         /* HIP runtime failure */
+#define FMC_ERR_NODEVICE -6    /* no gfx950 device visible */
+#define FMC_ERR_CELL -7        /* cutoff exceeds half cell width (err 95) */
+
+/* trial kinds (gcmc.f insert / delete / displace blocks) */
+#define FMC_INSERT 1
+#define FMC_DELETE 2
+#define FMC_DISPLACE 3
+
+/* vdW potential keys (vdw_terms.f:48-83): 1 12-6, 2 lj, 3 nm, 4 buck,
+ * 5 bhm, 6 hbnd, 7 snm/hcnm, 8 morse, 9 wca */
+
+typedef struct fmc_ctx fmc_ctx;
+
+/* ---------------------------------------------------------------------
+ * Single-chain context: set-up.  Replaces the module-global state the
+ * reference fills in initscan/readfield/readconfig/alloc_config_arrays
+ * (utility_pack.f:8-56,176-457,706-905; readinputs.f:11-515) and the table
+ * generators erfcgen (ewald.f:1-30) and forgen (vdw_terms.f:189-396).
+ * ------------------------------------------------------------------- */
+int fmc_ctx_create(fmc_ctx **out);
+int fmc_ctx_destroy(fmc_ctx *ctx);
+/* imcon 1 cubic, 2 orthorhombic, 3 parallelepiped (images,
+ * utility_pack.f:1138-1349); cell = DL_POLY cell(1:9) */
+int fmc_set_cell(fmc_ctx *ctx, int imcon, const double *cell);
+/* "ewald sum alpha k1 k2 k3" (utility_pack.f:395-398) */
+int fmc_set_ewald(fmc_ctx *ctx, double alpha, int kmax1, int kmax2,
+                  int kmax3, double epsq);
+/* "ewald precision eps" -> alpha, kmax (utility_pack.f:380-392); needs the
+ * cell and cutoff set first */
+int fmc_set_ewald_precision(fmc_ctx *ctx, double eps, double epsq);
+/* cutoff, delr (neighbour skin, utility_pack.f:3130) and rvdw (table range,
+ * utility_pack.f:725; pass 0 for rvdw = rcut) */
+int fmc_set_cutoffs(fmc_ctx *ctx, double rcut, double delr, double rvdw);
+/* VDW block of FIELD (vdw_terms.f:1-187): ntpatm unique atom types;
+ * potential t acts between types atom_a[t], atom_b[t] (1-based) with key
+ * keys[t] and parameters prm[5*t .. 5*t+4], energies already multiplied by
+ * engunit (vdw_terms.f:119-139) */
+int fmc_set_vdw(fmc_ctx *ctx, int ntpatm, int ntpvdw, const int *atom_a,
+                const int *atom_b, const int *keys, const double *prm);
+/* framework atoms (condensefram, utility_pack.f:908-947); frozen may be
+ * NULL (all frozen, the FIELD convention for rigid frameworks) */
+int fmc_add_framework(fmc_ctx *ctx, int n, const double *x, const double *y,
+                      const double *z, const double *q, const int *type,
+                      const int *frozen);
+/* rigid guest template, COM-centred as readfield leaves it
+ * (readinputs.f:226-253); *gtype_out gets the 0-based guest type */
+int fmc_add_guest_type(fmc_ctx *ctx, int nsites, const double *x,
+                       const double *y, const double *z, const double *q,
+                       const int *type, const double *mass, int *gtype_out);
+/* a guest molecule already present (CONFIG with NUMMOLS > 0 / REVIVE) */
+int fmc_add_guest_molecule(fmc_ctx *ctx, int gtype, const double *x,
+                           const double *y, const double *z);
+/* maximum guest atoms ("max guest atoms", utility_pack.f:208) */
+int fmc_set_capacity(fmc_ctx *ctx, int max_guest_atoms);
+/* build tables, k-vectors and device state; computes S(k) of all atoms
+ * like the ewald1 call at gcmc.f:522 */
+int fmc_ctx_finalize(fmc_ctx *ctx);
+
+/* info: nk = k-vectors kept (ewald.f:81-84), volm, mxgrid, ewald3en of
+ * each guest type (gcmc.f:529-552) */
+int fmc_get_info(const fmc_ctx *ctx, int *nk, double *volm, int *mxgrid,
+                 double *drewd, double *dlrpot);
+int fmc_guest_ewald3(const fmc_ctx *ctx, int gtype, double *e3);
+int fmc_guest_count(const fmc_ctx *ctx, int gtype, int *nmol);
+int fmc_get_molecule(const fmc_ctx *ctx, int gtype, int imol, double *x,
+                     double *y, double *z);
+/* copies of the tables as built: which 0 = erc(0..mxgrid), 1 = vvv table t
+ * (0..mxgrid); entry 0 duplicates entry 1 (guard for r < one grid step) */
+int fmc_get_table(const fmc_ctx *ctx, int which, int t, double *out);
+/* running structure factor, Nk entries each (ckcsum/ckssum) */
+int fmc_get_sk(const fmc_ctx *ctx, double *re, double *im);
+
+/* ---------------------------------------------------------------------
+ * Hot path.
+ * ------------------------------------------------------------------- */
+/* ewald1 (ewald.f:368-630): all_atoms 0 = framework only (no state change),
+ * 1 = every atom; the all-atom call resets the running S(k) */
+int fmc_recip_full(fmc_ctx *ctx, int all_atoms, double *energy);
+
+/* One trial move: the fused replacement of guestlistgen
+ * (utility_pack.f:3113-3177), ewald1_guest (ewald.f:32-365) for pass 1 and
+ * pass 2, and the per-site ewald2 (ewald.f:632-694) + srfrce
+ * (vdw_terms.f:398-498) loops of gcmc.f:906-936, 1076-1104, 1254-1324.
+ *   kind FMC_INSERT:   x,y,z = new site coordinates; imol ignored
+ *   kind FMC_DELETE:   molecule imol of gtype (resident); x,y,z ignored
+ *   kind FMC_DISPLACE: molecule imol of gtype moves to x,y,z
+ * out[0] = ewld1eng (reciprocal delta incl. the +/- self term of
+ *          ewald.f:335-362), out[1] = ewld2sum, out[2] = vdwsum, with the
+ * reference's signs, so the driver forms estep exactly as gcmc.f:946,1112,
+ * 1328.  Blocking; the move is pending until fmc_commit or the next trial.
+ */
+int fmc_trial(fmc_ctx *ctx, int kind, int gtype, int imol, const double *x,
+              const double *y, const double *z, double *out);
+/* accept the pending trial (gcmc.f:965-981, 1134-1151, 1355-1368 and
+ * condense, utility_pack.f:3325-3423): S(k) <- S_new, coordinates updated;
+ * a deleted molecule's successors shift down by one, as in gcmc.f:1127 */
+int fmc_commit(fmc_ctx *ctx);
+/* gstlrcorrect (vdw_terms.f:684-745) for N(gtype)+sign molecules, minus
+ * elrc (gcmc.f:891-901, 1056-1069) */
+int fmc_tail_delta(fmc_ctx *ctx, int gtype, int sign, double elrc,
+                   double *delrc);
+/* single_point (gcmc.f:2518-2710): out = {framework recip+self,
+ * recip(all)-recip(framework), real-space coulomb, vdW, ewald3 exclusions,
+ * elrc}; internal units.  Also resets the running S(k) like gcmc.f:522. */
+int fmc_energy_full(fmc_ctx *ctx, double *out);
+
+/* ---------------------------------------------------------------------
+ * Batched device-resident chains.  No reference counterpart call site: it
+ * runs many independent copies of the gcmc.f:589-1960 move loop (insert,
+ * delete, displace incl. translate/rotate-only moves, duni RNG, energy_eval
+ * acceptance, adaptive steps) entirely on the GPU, one chain per work-group,
+ * replacing the reference's one-chain-per-MPI-rank replica mode
+ * (gcmc.f:366-368).
+ * ------------------------------------------------------------------- */
+#define FMC_PREC_FP64 0  /* everything FP64: same accept/reject stream */
+#define FMC_PREC_MIXED 1 /* FP32 pair terms and phases, FP64 S(k)/totals */
+
+typedef struct fmc_batch fmc_batch;
+
+typedef struct fmc_chain_params {
+  double temp;          /* K */
+  int ntpguest;         /* guest types in the chain's context (<= 4) */
+  int seed[4];          /* duni (i,j,k,l) (gcmc.f:2753-2758) */
+  double press[4];      /* Pa per guest type (CONTROL bar * 1e5) */
+  /* raw move frequencies per guest (readinputs.f:665-688): insert, delete,
+   * displace, jump, flex, swap, translate, rotate, multi-jump; all zero
+   * means 1/3 insert/delete/displace (gcmc.f:285-298) */
+  double freq[4][9];
+  double delr[4];       /* initial displacement length (CONTROL delr) */
+  double disp_ratio[4]; /* target acceptance ratios (default 0.5) */
+  double tran_ratio[4];
+  double rota_ratio;
+  double engunit;       /* 418.4 for kcal */
+  long long eq_steps;   /* moves before production averaging starts */
+} fmc_chain_params;
+
+typedef struct fmc_chain_stats {
+  long long steps;            /* moves done */
+  long long prod_steps;       /* production samples */
+  long long accepted[6];      /* by kind: 0 none,1 ins,2 del,3 disp,4 tran,5 rota */
+  long long attempted[6];
+  int nmol[4];                /* current molecules per guest type */
+  double energy[4];           /* accumulated energy (kcal/mol), gcmc.f:981 */
+  double sum_n[4], sum_n2[4]; /* production sums of N, N^2 */
+  double sum_e[4];            /* production sum of energy */
+  double delrdisp[4], rotangle, tran_delr[4], rota_rotangle;
+} fmc_chain_stats;
+
+/* max_chains chains; precision FMC_PREC_*.  */
+int fmc_batch_create(fmc_batch **out, int max_chains, int precision);
+int fmc_batch_destroy(fmc_batch *b);
+/* add a chain whose system is the finalized context ctx (copied; ctx may be
+ * destroyed afterwards).  *chain_id gets the slot. */
+int fmc_batch_add_chain(fmc_batch *b, const fmc_ctx *ctx,
+                        const fmc_chain_params *p, int *chain_id);
+/* remove a chain and free its slot (serving) */
+int fmc_batch_remove_chain(fmc_batch *b, int chain_id);
+/* run n_moves moves on every active chain (one persistent launch) */
+int fmc_batch_run(fmc_batch *b, long long n_moves);
+/* enqueue without waiting / wait for completion */
+int fmc_batch_launch(fmc_batch *b, long long n_moves);
+int fmc_batch_sync(fmc_batch *b);
+int fmc_batch_stats(fmc_batch *b, int chain_id, fmc_chain_stats *st);
+/* per-move log of the last run for one chain (FP64 parity checks):
+ * rec[5*i..5*i+4] = {kind, iguest, accepted, nmol_after, estep}; enable
+ * before running with capacity moves */
+int fmc_batch_enable_log(fmc_batch *b, int chain_id, long long capacity);
+int fmc_batch_get_log(fmc_batch *b, int chain_id, long long n, double *rec);
+/* FP64 full recompute of every chain's S(k) (periodic resync); returns the
+ * largest |S_running - S_exact| seen */
+int fmc_batch_resync(fmc_batch *b, double *max_dev);
+/* tile shape: threads per chain work-group (64..1024); 0 = autotuned */
+int fmc_batch_set_tile(fmc_batch *b, int threads);
+int fmc_batch_get_tile(const fmc_batch *b, int *threads);
+/* bytes of device memory one chain of this context needs (admission) */
+int fmc_chain_footprint(const fmc_ctx *ctx, int precision,
+                        long long *bytes);
+
+/* library version / device check */
+int fmc_version(int *major, int *minor);
+int fmc_device_ok(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
