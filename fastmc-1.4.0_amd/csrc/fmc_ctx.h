@@ -1,0 +1,47 @@
+// The single-chain context behind the opaque fmc_ctx handle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "fmc_host.h"
+
+struct fmc_ctx {
+  fmc::SystemDesc sys;
+  // host mirror of the resident guest coordinates, per guest type
+  std::vector<double> hx[fmc::kMaxGuestTypes], hy[fmc::kMaxGuestTypes],
+      hz[fmc::kMaxGuestTypes];
+  std::vector<int> nmol;
+  int cap_atoms = 0;
+  int nk = 0;
+  // device state
+  double* d_erc = nullptr;
+  double* d_vvv = nullptr;
+  int* d_pairtab = nullptr;
+  double *d_fx = nullptr, *d_fy = nullptr, *d_fz = nullptr, *d_fq = nullptr;
+  int* d_ftype = nullptr;
+  int* d_kv = nullptr;
+  double* d_ak = nullptr;
+  double* d_sre[2] = {nullptr, nullptr};
+  double* d_sim[2] = {nullptr, nullptr};
+  int cur = 0;
+  double* d_gx[fmc::kMaxGuestTypes] = {nullptr};
+  double* d_gy[fmc::kMaxGuestTypes] = {nullptr};
+  double* d_gz[fmc::kMaxGuestTypes] = {nullptr};
+  double2* d_rpart = nullptr;
+  int recip_chunk = 1024, recip_nchunk_cap = 0;
+  double* d_epart = nullptr;
+  double* d_partial = nullptr;
+  unsigned* d_counter = nullptr;
+  double* d_out = nullptr;
+  double* h_out = nullptr;
+  hipStream_t stream = nullptr;
+  int nb_recip_max = 64, nb_real_max = 64;
+  // pending trial
+  int pending = 0, pend_gt = 0, pend_mol = 0;
+  double pend_x[fmc::kMaxSites], pend_y[fmc::kMaxSites], pend_z[fmc::kMaxSites];
+
+  int upload_state();
+  void release();
+};

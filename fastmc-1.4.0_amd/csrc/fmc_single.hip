@@ -1,0 +1,971 @@
+// Single-chain drop-in context: device-resident framework, guests, tables
+// and running structure factor S(k); one fused kernel launch per trial move.
+//
+// Layout in HBM (one context):
+//   framework  fx, fy, fz, fq (FP64 SoA), ftype (int, 0-based)
+//   guests     per guest type a segment of capacity max_guest_atoms sites,
+//              gx, gy, gz (FP64 SoA), molecule m site s at m*ns + s
+//   tables     erc[mxgrid+1], vvv[ntable][mxgrid+1], pairtab[ntpatm^2]
+//   k-vectors  kv[Nk] packed (l, m, n) int8 triples, ak[Nk] FP64
+//   S(k)       two FP64 (re, im) buffers; a trial writes S_new into the
+//              spare one and commit swaps them (ewald.f:260-293 ckcsnew and
+//              gcmc.f:971-974 ckcsum <- ckcsnew)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fastmc_amd.h"
+#include "fmc_ctx.h"
+#include "fmc_device.h"
+#include "fmc_host.h"
+
+namespace fmc {
+
+constexpr int kBS = 256;       // threads per block, 4 waves
+constexpr int kTabMax = 33;    // phase table entries per dimension (kmax<=32)
+constexpr int kMaxBlocks = 512;
+
+struct GuestSeg {
+  const double* x;
+  const double* y;
+  const double* z;
+  int ns;
+  int natoms;  // nmol * ns
+  double q[kMaxSites];
+  int t[kMaxSites];  // 0-based
+};
+
+struct TrialParams {
+  DevSys S;
+  int nk;
+  const int* kv;
+  const double* ak;
+  const double* sre;
+  const double* sim;
+  double* nre;
+  double* nim;
+  int ngt;
+  GuestSeg g[kMaxGuestTypes];
+  int kind, ns, own_t, own_begin;
+  int kmax[3];
+  double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
+  double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
+  double q[kMaxSites];
+  int t[kMaxSites];
+  double rvolm2;   // 2*rvolm
+  double engsic;   // self term of the moving molecule
+  int nb_recip, nb_real;
+  double* partial;  // [3][nb_recip+nb_real]
+  unsigned* counter;
+  double* out;
+};
+
+__device__ __forceinline__ int unpack_l(int v) { return static_cast<signed char>(v & 0xff); }
+__device__ __forceinline__ int unpack_m(int v) { return static_cast<signed char>((v >> 8) & 0xff); }
+__device__ __forceinline__ int unpack_n(int v) { return static_cast<signed char>((v >> 16) & 0xff); }
+
+// Which atom is global index a: framework first, then guest segments.
+__device__ __forceinline__ bool load_atom(const DevSys& S, const GuestSeg* g,
+                                          int ngt, int a, double& x, double& y,
+                                          double& z, double& q, int& t,
+                                          int& gt, int& gi) {
+  if (a < S.nfw) {
+    x = S.fx[a]; y = S.fy[a]; z = S.fz[a]; q = S.fq[a]; t = S.ftype[a];
+    gt = -1; gi = a;
+    return true;
+  }
+  a -= S.nfw;
+  for (int k = 0; k < ngt; ++k) {
+    if (a < g[k].natoms) {
+      const int s = a % g[k].ns;
+      x = g[k].x[a]; y = g[k].y[a]; z = g[k].z[a];
+      q = g[k].q[s]; t = g[k].t[s];
+      gt = k; gi = a;
+      return true;
+    }
+    a -= g[k].natoms;
+  }
+  return false;
+}
+
+// Last-block-done reduction of [3][nb] partials into out (fixed order).
+__device__ void finish_trial(const TrialParams& P, double* red) {
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  const int nb = P.nb_recip + P.nb_real;
+  if (threadIdx.x == 0) last = (atomicAdd(P.counter, 1u) == static_cast<unsigned>(nb - 1));
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double v[3];
+  for (int c = 0; c < 3; ++c) {
+    double a = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      a += __builtin_nontemporal_load(&P.partial[c * nb + b]);
+    v[c] = block_sum(a, red);
+  }
+  if (threadIdx.x == 0) {
+    double e1 = P.rvolm2 * kR4pie0 * v[0] / P.S.epsq;
+    if (P.kind == FMC_INSERT) e1 = e1 + P.engsic;
+    if (P.kind == FMC_DELETE) e1 = e1 - P.engsic;
+    P.out[0] = e1;
+    P.out[1] = v[1];
+    P.out[2] = v[2];
+    *P.counter = 0u;
+  }
+}
+
+__global__ void __launch_bounds__(kBS) trial_kernel(TrialParams P) {
+  __shared__ double2 tab[2][kMaxSites][3][kTabMax];
+  __shared__ double red[kBS / kWave];
+  const int tid = threadIdx.x;
+  const int nb = P.nb_recip + P.nb_real;
+  const bool use_old = (P.kind != FMC_INSERT);
+  const bool use_new = (P.kind != FMC_DELETE);
+  double e1 = 0.0, e2 = 0.0, ev = 0.0;
+  if (static_cast<int>(blockIdx.x) < P.nb_recip) {
+    // ---- reciprocal part (ewald1_guest, ewald.f:97-331) ----
+    const int ntab = 2 * P.ns * 3;
+    if (tid < ntab) {
+      const int set = tid / (P.ns * 3);
+      const int s = (tid / 3) % P.ns;
+      const int d = tid % 3;
+      const double x = set ? P.nx[s] : P.ox[s];
+      const double y = set ? P.ny[s] : P.oy[s];
+      const double z = set ? P.nz[s] : P.oz[s];
+      const double f = P.S.rcell[d] * x + P.S.rcell[3 + d] * y + P.S.rcell[6 + d] * z;
+      phase_table(f, P.kmax[d], &tab[set][s][d][0]);
+    }
+    __syncthreads();
+    for (int k = blockIdx.x * kBS + tid; k < P.nk; k += P.nb_recip * kBS) {
+      const int v = P.kv[k];
+      const int l = unpack_l(v), m = unpack_m(v), n = unpack_n(v);
+      const int am = m < 0 ? -m : m, an = n < 0 ? -n : n;
+      double2 so = make_double2(0.0, 0.0), sn = make_double2(0.0, 0.0);
+      if (use_old)
+        for (int s = 0; s < P.ns; ++s) {
+          const double2 t = structure_term(tab[0][s][0][l], tab[0][s][1][am], m,
+                                           tab[0][s][2][an], n, P.q[s]);
+          so.x = so.x + t.x;
+          so.y = so.y + t.y;
+        }
+      if (use_new)
+        for (int s = 0; s < P.ns; ++s) {
+          const double2 t = structure_term(tab[1][s][0][l], tab[1][s][1][am], m,
+                                           tab[1][s][2][an], n, P.q[s]);
+          sn.x = sn.x + t.x;
+          sn.y = sn.y + t.y;
+        }
+      const double cold = P.sre[k], sold = P.sim[k];
+      double c, s;
+      if (P.kind == FMC_INSERT) {
+        c = sn.x + cold;
+        s = sn.y + sold;
+      } else if (P.kind == FMC_DELETE) {
+        c = cold - so.x;
+        s = sold - so.y;
+      } else {
+        c = (cold - so.x) + sn.x;
+        s = (sold - so.y) + sn.y;
+      }
+      P.nre[k] = c;
+      P.nim[k] = s;
+      e1 += P.ak[k] * (c * c + s * s - cold * cold - sold * sold);
+    }
+  } else {
+    // ---- real space over every atom (ewald2 + srfrce, ewald.f:632-694,
+    //      vdw_terms.f:398-498), pass 1 with minus sign for displace ----
+    const int rb = blockIdx.x - P.nb_recip;
+    int ntot = P.S.nfw;
+    for (int k = 0; k < P.ngt; ++k) ntot += P.g[k].natoms;
+    for (int a = rb * kBS + tid; a < ntot; a += P.nb_real * kBS) {
+      double x, y, z, q;
+      int t, gt, gi;
+      load_atom(P.S, P.g, P.ngt, a, x, y, z, q, t, gt, gi);
+      if (gt == P.own_t && gi >= P.own_begin && gi < P.own_begin + P.ns &&
+          P.kind != FMC_INSERT)
+        continue;
+      for (int s = 0; s < P.ns; ++s) {
+        const double ce = P.q[s] / P.S.epsq * kR4pie0;
+        if (use_old) {
+          double a2 = 0.0, av = 0.0;
+          pair_fp64(P.S, ce, P.t[s], P.ox[s] - x, P.oy[s] - y, P.oz[s] - z, q,
+                    t, a2, av);
+          if (P.kind == FMC_DISPLACE) { e2 -= a2; ev -= av; }
+          else { e2 += a2; ev += av; }
+        }
+        if (use_new) {
+          double a2 = 0.0, av = 0.0;
+          pair_fp64(P.S, ce, P.t[s], P.nx[s] - x, P.ny[s] - y, P.nz[s] - z, q,
+                    t, a2, av);
+          e2 += a2;
+          ev += av;
+        }
+      }
+    }
+  }
+  e1 = block_sum(e1, red);
+  e2 = block_sum(e2, red);
+  ev = block_sum(ev, red);
+  if (tid == 0) {
+    P.partial[0 * nb + blockIdx.x] = e1;
+    P.partial[1 * nb + blockIdx.x] = e2;
+    P.partial[2 * nb + blockIdx.x] = ev;
+  }
+  finish_trial(P, red);
+}
+
+// ---- full structure factor (ewald1, ewald.f:368-630) ----
+struct RecipFullParams {
+  DevSys S;
+  int ngt;
+  GuestSeg g[kMaxGuestTypes];
+  int natoms;   // atoms included (framework only or all)
+  int nk;
+  const int* kv;
+  const double* ak;
+  int chunk;
+  double2* part;   // [nchunk][nk]
+};
+
+__global__ void __launch_bounds__(kBS) recip_partial_kernel(RecipFullParams P) {
+  __shared__ double4 at[kBS];
+  const int c = blockIdx.y;
+  const int a0 = c * P.chunk;
+  const int na = min(P.chunk, P.natoms - a0);
+  double2 acc = make_double2(0.0, 0.0);
+  const int k = blockIdx.x * kBS + threadIdx.x;
+  int l = 0, m = 0, n = 0;
+  if (k < P.nk) {
+    const int v = P.kv[k];
+    l = unpack_l(v); m = unpack_m(v); n = unpack_n(v);
+  }
+  for (int base = 0; base < na; base += kBS) {
+    __syncthreads();
+    const int i = base + threadIdx.x;
+    if (i < na) {
+      double x, y, z, q;
+      int t, gt, gi;
+      load_atom(P.S, P.g, P.ngt, a0 + i, x, y, z, q, t, gt, gi);
+      at[threadIdx.x] = make_double4(
+          P.S.rcell[0] * x + P.S.rcell[3] * y + P.S.rcell[6] * z,
+          P.S.rcell[1] * x + P.S.rcell[4] * y + P.S.rcell[7] * z,
+          P.S.rcell[2] * x + P.S.rcell[5] * y + P.S.rcell[8] * z, q);
+    }
+    __syncthreads();
+    const int cnt = min(kBS, na - base);
+    if (k < P.nk)
+      for (int j = 0; j < cnt; ++j) {
+        const double4 a = at[j];
+        double ph = l * a.x + m * a.y + n * a.z;
+        ph -= rint(ph);
+        double sv, cv;
+        sincospi(2.0 * ph, &sv, &cv);
+        acc.x += a.w * cv;
+        acc.y += a.w * sv;
+      }
+  }
+  if (k < P.nk) P.part[static_cast<size_t>(c) * P.nk + k] = acc;
+}
+
+__global__ void __launch_bounds__(kBS) recip_reduce_kernel(
+    const double2* part, int nchunk, int nk, const double* ak, double* sre,
+    double* sim, double* epart) {
+  __shared__ double red[kBS / kWave];
+  const int k = blockIdx.x * kBS + threadIdx.x;
+  double e = 0.0;
+  if (k < nk) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int c = 0; c < nchunk; ++c) {
+      const double2 p = part[static_cast<size_t>(c) * nk + k];
+      s.x += p.x;
+      s.y += p.y;
+    }
+    if (sre) {
+      sre[k] = s.x;
+      sim[k] = s.y;
+    }
+    e = ak[k] * (s.x * s.x + s.y * s.y);
+  }
+  e = block_sum(e, red);
+  if (threadIdx.x == 0) epart[blockIdx.x] = e;
+}
+
+// Sum n partials in order (single block).
+__global__ void __launch_bounds__(kBS) sum_kernel(const double* part, int n,
+                                                  double* out) {
+  __shared__ double red[kBS / kWave];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBS) a += part[i];
+  a = block_sum(a, red);
+  if (threadIdx.x == 0) *out = a;
+}
+
+// ---- single_point real space: every guest atom against the framework and
+//      the later guest atoms (each pair once, gcmc.f:2644-2680) ----
+struct PairAllParams {
+  DevSys S;
+  int ngt;
+  GuestSeg g[kMaxGuestTypes];
+  int nguest;   // guest atoms
+  double* part; // [2][gridDim.x * gridDim.y]
+};
+
+__global__ void __launch_bounds__(kBS) pair_all_kernel(PairAllParams P) {
+  __shared__ double red[kBS / kWave];
+  const int i = blockIdx.y;  // guest atom (global guest index)
+  double xi, yi, zi, qi;
+  int ti, gti, gii;
+  load_atom(P.S, P.g, P.ngt, P.S.nfw + i, xi, yi, zi, qi, ti, gti, gii);
+  const int nsi = P.g[gti].ns;
+  const int moli = gii / nsi;
+  const double ce = qi / P.S.epsq * kR4pie0;
+  const int ntot = P.S.nfw + P.nguest;
+  double e2 = 0.0, ev = 0.0;
+  for (int a = blockIdx.x * kBS + threadIdx.x; a < ntot; a += gridDim.x * kBS) {
+    // framework atoms, or guest atoms after i in condensed order
+    if (a >= P.S.nfw && a - P.S.nfw <= i) continue;
+    double x, y, z, q;
+    int t, gt, gi;
+    load_atom(P.S, P.g, P.ngt, a, x, y, z, q, t, gt, gi);
+    if (gt == gti && gi / P.g[gt].ns == moli) continue;
+    pair_fp64(P.S, ce, ti, xi - x, yi - y, zi - z, q, t, e2, ev);
+  }
+  e2 = block_sum(e2, red);
+  ev = block_sum(ev, red);
+  if (threadIdx.x == 0) {
+    const int b = blockIdx.y * gridDim.x + blockIdx.x;
+    const int nbt = gridDim.x * gridDim.y;
+    P.part[b] = e2;
+    P.part[nbt + b] = ev;
+  }
+}
+
+}  // namespace fmc
+
+using namespace fmc;
+
+#define HIPCHECK(x)                                                     \
+  do {                                                                  \
+    hipError_t e_ = (x);                                                \
+    if (e_ != hipSuccess) {                                             \
+      std::fprintf(stderr, "fastmc_amd: HIP error %s at %s:%d\n",       \
+                   hipGetErrorString(e_), __FILE__, __LINE__);          \
+      return FMC_ERR_HIP;                                               \
+    }                                                                   \
+  } while (0)
+
+namespace {
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+  if (n == 0) n = 1;
+  HIPCHECK(hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
+  return FMC_OK;
+}
+
+template <typename T>
+int upload(T* d, const T* h, size_t n) {
+  if (n) HIPCHECK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
+void fill_devsys(fmc_ctx* c, DevSys* S) {
+  const SystemDesc& s = c->sys;
+  S->imcon = s.imcon;
+  std::memcpy(S->cell, s.cell, sizeof(S->cell));
+  std::memcpy(S->rcell, s.rcell, sizeof(S->rcell));
+  S->rcut2 = s.rcut * s.rcut;
+  S->rdrewd = 1.0 / s.drewd;
+  S->rdlrpot = 1.0 / s.dlrpot;
+  S->mxgrid = s.mxgrid;
+  S->ntpatm = s.ntpatm;
+  S->epsq = s.epsq;
+  S->erc = c->d_erc;
+  S->vvv = c->d_vvv;
+  S->pairtab = c->d_pairtab;
+  S->nfw = static_cast<int>(s.fx.size());
+  S->fx = c->d_fx;
+  S->fy = c->d_fy;
+  S->fz = c->d_fz;
+  S->fq = c->d_fq;
+  S->ftype = c->d_ftype;
+}
+
+void fill_segs(fmc_ctx* c, GuestSeg* g, int* ngt) {
+  *ngt = static_cast<int>(c->sys.guests.size());
+  for (int t = 0; t < *ngt; ++t) {
+    const GuestTemplate& tp = c->sys.guests[t];
+    g[t].x = c->d_gx[t];
+    g[t].y = c->d_gy[t];
+    g[t].z = c->d_gz[t];
+    g[t].ns = tp.ns;
+    g[t].natoms = c->nmol[t] * tp.ns;
+    for (int s = 0; s < kMaxSites; ++s) {
+      g[t].q[s] = s < tp.ns ? tp.q[s] : 0.0;
+      g[t].t[s] = s < tp.ns ? tp.type[s] - 1 : 0;
+    }
+  }
+}
+
+int recip_full_impl(fmc_ctx* c, bool all_atoms, bool set_running,
+                    double* energy) {
+  RecipFullParams P;
+  fill_devsys(c, &P.S);
+  fill_segs(c, P.g, &P.ngt);
+  int nguest = 0;
+  for (int t = 0; t < P.ngt; ++t) nguest += P.g[t].natoms;
+  P.natoms = P.S.nfw + (all_atoms ? nguest : 0);
+  P.nk = c->nk;
+  P.kv = c->d_kv;
+  P.ak = c->d_ak;
+  P.chunk = c->recip_chunk;
+  const int nchunk = std::max(1, (P.natoms + P.chunk - 1) / P.chunk);
+  if (nchunk > c->recip_nchunk_cap) return FMC_ERR_CAPACITY;
+  P.part = c->d_rpart;
+  const int kb = (c->nk + kBS - 1) / kBS;
+  hipLaunchKernelGGL(recip_partial_kernel, dim3(kb, nchunk), dim3(kBS), 0,
+                     c->stream, P);
+  HIPCHECK(hipGetLastError());
+  double* sre = set_running ? c->d_sre[c->cur] : nullptr;
+  double* sim = set_running ? c->d_sim[c->cur] : nullptr;
+  hipLaunchKernelGGL(recip_reduce_kernel, dim3(kb), dim3(kBS), 0, c->stream,
+                     c->d_rpart, nchunk, c->nk, c->d_ak, sre, sim, c->d_epart);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, c->d_epart,
+                     kb, c->d_out);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  double sq = 0.0;
+  for (double q : c->sys.fq) sq += q * q;
+  if (all_atoms)
+    for (size_t t = 0; t < c->sys.guests.size(); ++t)
+      for (int s = 0; s < c->sys.guests[t].ns; ++s)
+        sq += c->nmol[t] * c->sys.guests[t].q[s] * c->sys.guests[t].q[s];
+  const SystemDesc& s = c->sys;
+  const double engsic = -kR4pie0 / s.epsq * s.alpha * sq / kSqrpi;
+  const double rvolm = 2.0 * kPi / s.volm;
+  *energy = 2.0 * rvolm * kR4pie0 * c->h_out[0] / s.epsq + engsic;
+  return FMC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI: set-up
+// ---------------------------------------------------------------------------
+
+extern "C" int fmc_ctx_create(fmc_ctx** out) {
+  if (!out) return FMC_ERR_ARG;
+  *out = new fmc_ctx();
+  return FMC_OK;
+}
+
+extern "C" int fmc_ctx_destroy(fmc_ctx* c) {
+  if (!c) return FMC_ERR_ARG;
+  c->release();
+  delete c;
+  return FMC_OK;
+}
+
+extern "C" int fmc_set_cell(fmc_ctx* c, int imcon, const double* cell) {
+  if (!c || !cell) return FMC_ERR_ARG;
+  if (imcon < 1 || imcon > 3) return FMC_ERR_UNSUPPORTED;
+  c->sys.imcon = imcon;
+  std::memcpy(c->sys.cell, cell, 9 * sizeof(double));
+  return FMC_OK;
+}
+
+extern "C" int fmc_set_ewald(fmc_ctx* c, double alpha, int k1, int k2, int k3,
+                             double epsq) {
+  if (!c || alpha <= 0 || k1 < 0 || k2 < 0 || k3 < 0) return FMC_ERR_ARG;
+  if (k1 >= kTabMax || k2 >= kTabMax || k3 >= kTabMax) return FMC_ERR_UNSUPPORTED;
+  c->sys.alpha = alpha;
+  c->sys.kmax[0] = k1;
+  c->sys.kmax[1] = k2;
+  c->sys.kmax[2] = k3;
+  c->sys.epsq = epsq;
+  c->sys.have_ewald = true;
+  return FMC_OK;
+}
+
+extern "C" int fmc_set_ewald_precision(fmc_ctx* c, double eps, double epsq) {
+  if (!c || c->sys.imcon == 0 || c->sys.rcut <= 0) return FMC_ERR_STATE;
+  double alpha;
+  int km[3];
+  ewald_precision(eps, c->sys.rcut, c->sys.imcon, c->sys.cell, &alpha, km);
+  return fmc_set_ewald(c, alpha, km[0], km[1], km[2], epsq);
+}
+
+extern "C" int fmc_set_cutoffs(fmc_ctx* c, double rcut, double delr,
+                               double rvdw) {
+  if (!c || rcut <= 0) return FMC_ERR_ARG;
+  c->sys.rcut = rcut;
+  c->sys.delr = delr;
+  c->sys.rvdw = rvdw;
+  return FMC_OK;
+}
+
+extern "C" int fmc_set_vdw(fmc_ctx* c, int ntpatm, int ntpvdw,
+                           const int* a, const int* b, const int* keys,
+                           const double* prm) {
+  if (!c || ntpatm <= 0 || ntpvdw < 0) return FMC_ERR_ARG;
+  SystemDesc& s = c->sys;
+  s.ntpatm = ntpatm;
+  s.ntpvdw = ntpvdw;
+  s.vdw_a.assign(a, a + ntpvdw);
+  s.vdw_b.assign(b, b + ntpvdw);
+  s.vdw_key.assign(keys, keys + ntpvdw);
+  s.vdw_prm.assign(prm, prm + 5 * ntpvdw);
+  for (int t = 0; t < ntpvdw; ++t)
+    if (keys[t] < 0 || keys[t] > 9) return FMC_ERR_UNSUPPORTED;
+  return FMC_OK;
+}
+
+extern "C" int fmc_add_framework(fmc_ctx* c, int n, const double* x,
+                                 const double* y, const double* z,
+                                 const double* q, const int* type,
+                                 const int* frozen) {
+  if (!c || n < 0 || (n && (!x || !y || !z || !q || !type))) return FMC_ERR_ARG;
+  if (c->sys.finalized) return FMC_ERR_STATE;
+  SystemDesc& s = c->sys;
+  for (int i = 0; i < n; ++i) {
+    s.fx.push_back(x[i]);
+    s.fy.push_back(y[i]);
+    s.fz.push_back(z[i]);
+    s.fq.push_back(q[i]);
+    s.ftype.push_back(type[i]);
+    s.ffrz.push_back(frozen ? frozen[i] : 1);
+  }
+  return FMC_OK;
+}
+
+extern "C" int fmc_add_guest_type(fmc_ctx* c, int ns, const double* x,
+                                  const double* y, const double* z,
+                                  const double* q, const int* type,
+                                  const double* mass, int* gt) {
+  if (!c || ns <= 0 || ns > kMaxSites || !x || !y || !z || !q || !type)
+    return FMC_ERR_ARG;
+  if (c->sys.finalized) return FMC_ERR_STATE;
+  if (static_cast<int>(c->sys.guests.size()) >= kMaxGuestTypes)
+    return FMC_ERR_CAPACITY;
+  GuestTemplate g;
+  g.ns = ns;
+  for (int i = 0; i < ns; ++i) {
+    g.x[i] = x[i];
+    g.y[i] = y[i];
+    g.z[i] = z[i];
+    g.q[i] = q[i];
+    g.type[i] = type[i];
+    g.mass[i] = mass ? mass[i] : 1.0;
+  }
+  c->sys.guests.push_back(g);
+  c->sys.gx0.emplace_back();
+  c->sys.gy0.emplace_back();
+  c->sys.gz0.emplace_back();
+  if (gt) *gt = static_cast<int>(c->sys.guests.size()) - 1;
+  return FMC_OK;
+}
+
+extern "C" int fmc_add_guest_molecule(fmc_ctx* c, int gt, const double* x,
+                                      const double* y, const double* z) {
+  if (!c || gt < 0 || gt >= static_cast<int>(c->sys.guests.size()))
+    return FMC_ERR_ARG;
+  if (c->sys.finalized) return FMC_ERR_STATE;
+  for (int i = 0; i < c->sys.guests[gt].ns; ++i) {
+    c->sys.gx0[gt].push_back(x[i]);
+    c->sys.gy0[gt].push_back(y[i]);
+    c->sys.gz0[gt].push_back(z[i]);
+  }
+  return FMC_OK;
+}
+
+extern "C" int fmc_set_capacity(fmc_ctx* c, int max_guest_atoms) {
+  if (!c || max_guest_atoms <= 0) return FMC_ERR_ARG;
+  if (c->sys.finalized) return FMC_ERR_STATE;
+  c->sys.max_guest_atoms = max_guest_atoms;
+  return FMC_OK;
+}
+
+int fmc_ctx::upload_state() {
+  const SystemDesc& s = sys;
+  const int ng = static_cast<int>(s.guests.size());
+  const size_t nfw = s.fx.size();
+  int r;
+  if ((r = dalloc(&d_erc, s.erc.size())) || (r = upload(d_erc, s.erc.data(), s.erc.size())))
+    return r;
+  if ((r = dalloc(&d_vvv, s.vvv.size())) || (r = upload(d_vvv, s.vvv.data(), s.vvv.size())))
+    return r;
+  if ((r = dalloc(&d_pairtab, s.pairtab.size())) ||
+      (r = upload(d_pairtab, s.pairtab.data(), s.pairtab.size())))
+    return r;
+  std::vector<int> ft0(nfw);
+  for (size_t i = 0; i < nfw; ++i) ft0[i] = s.ftype[i] - 1;
+  if ((r = dalloc(&d_fx, nfw)) || (r = upload(d_fx, s.fx.data(), nfw))) return r;
+  if ((r = dalloc(&d_fy, nfw)) || (r = upload(d_fy, s.fy.data(), nfw))) return r;
+  if ((r = dalloc(&d_fz, nfw)) || (r = upload(d_fz, s.fz.data(), nfw))) return r;
+  if ((r = dalloc(&d_fq, nfw)) || (r = upload(d_fq, s.fq.data(), nfw))) return r;
+  if ((r = dalloc(&d_ftype, nfw)) || (r = upload(d_ftype, ft0.data(), nfw))) return r;
+  nk = static_cast<int>(s.kvecs.size());
+  std::vector<int> kv(nk);
+  std::vector<double> ak(nk);
+  for (int k = 0; k < nk; ++k) {
+    kv[k] = (s.kvecs[k].l & 0xff) | ((s.kvecs[k].m & 0xff) << 8) |
+            ((s.kvecs[k].n & 0xff) << 16);
+    ak[k] = s.kvecs[k].ak;
+  }
+  if ((r = dalloc(&d_kv, nk)) || (r = upload(d_kv, kv.data(), nk))) return r;
+  if ((r = dalloc(&d_ak, nk)) || (r = upload(d_ak, ak.data(), nk))) return r;
+  for (int b = 0; b < 2; ++b) {
+    if ((r = dalloc(&d_sre[b], nk)) || (r = dalloc(&d_sim[b], nk))) return r;
+  }
+  cap_atoms = s.max_guest_atoms;
+  nmol.assign(ng, 0);
+  for (int t = 0; t < ng; ++t) {
+    const int ns = s.guests[t].ns;
+    if ((r = dalloc(&d_gx[t], cap_atoms)) || (r = dalloc(&d_gy[t], cap_atoms)) ||
+        (r = dalloc(&d_gz[t], cap_atoms)))
+      return r;
+    hx[t].assign(cap_atoms, 0.0);
+    hy[t].assign(cap_atoms, 0.0);
+    hz[t].assign(cap_atoms, 0.0);
+    const int n0 = static_cast<int>(s.gx0[t].size());
+    for (int i = 0; i < n0; ++i) {
+      hx[t][i] = s.gx0[t][i];
+      hy[t][i] = s.gy0[t][i];
+      hz[t][i] = s.gz0[t][i];
+    }
+    nmol[t] = n0 / ns;
+    if ((r = upload(d_gx[t], hx[t].data(), n0)) || (r = upload(d_gy[t], hy[t].data(), n0)) ||
+        (r = upload(d_gz[t], hz[t].data(), n0)))
+      return r;
+  }
+  int guest_atoms = 0;
+  for (int t = 0; t < ng; ++t) guest_atoms += nmol[t] * s.guests[t].ns;
+  if (guest_atoms > cap_atoms) return FMC_ERR_CAPACITY;
+  recip_chunk = 1024;
+  recip_nchunk_cap = static_cast<int>((nfw + cap_atoms) / recip_chunk) + 2;
+  if ((r = dalloc(&d_rpart, static_cast<size_t>(recip_nchunk_cap) * nk))) return r;
+  if ((r = dalloc(&d_epart, static_cast<size_t>(nk / 64 + 64)))) return r;
+  if ((r = dalloc(&d_partial, 3 * kMaxBlocks))) return r;
+  if ((r = dalloc(&d_counter, 1))) return r;
+  HIPCHECK(hipMemset(d_counter, 0, sizeof(unsigned)));
+  if ((r = dalloc(&d_out, 8))) return r;
+  HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_out), 8 * sizeof(double)));
+  HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  return FMC_OK;
+}
+
+void fmc_ctx::release() {
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(d_erc); f(d_vvv); f(d_pairtab); f(d_fx); f(d_fy); f(d_fz); f(d_fq);
+  f(d_ftype); f(d_kv); f(d_ak);
+  for (int b = 0; b < 2; ++b) { f(d_sre[b]); f(d_sim[b]); }
+  for (int t = 0; t < kMaxGuestTypes; ++t) { f(d_gx[t]); f(d_gy[t]); f(d_gz[t]); }
+  f(d_rpart); f(d_epart); f(d_partial); f(d_counter); f(d_out);
+  if (h_out) (void)hipHostFree(h_out);
+  if (stream) (void)hipStreamDestroy(stream);
+  d_erc = d_vvv = d_fx = d_fy = d_fz = d_fq = d_ak = nullptr;
+  d_pairtab = d_ftype = d_kv = nullptr;
+  h_out = nullptr;
+  stream = nullptr;
+}
+
+extern "C" int fmc_ctx_finalize(fmc_ctx* c) {
+  if (!c) return FMC_ERR_ARG;
+  if (c->sys.finalized) return FMC_ERR_STATE;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FMC_ERR_NODEVICE;
+  std::string err;
+  int r = c->sys.finalize(&err);
+  if (r) {
+    std::fprintf(stderr, "fastmc_amd: %s\n", err.c_str());
+    return r;
+  }
+  if ((r = c->upload_state())) return r;
+  double e;
+  return recip_full_impl(c, true, true, &e);
+}
+
+extern "C" int fmc_get_info(const fmc_ctx* c, int* nk, double* volm,
+                            int* mxgrid, double* drewd, double* dlrpot) {
+  if (!c || !c->sys.finalized) return FMC_ERR_STATE;
+  if (nk) *nk = c->nk;
+  if (volm) *volm = c->sys.volm;
+  if (mxgrid) *mxgrid = c->sys.mxgrid;
+  if (drewd) *drewd = c->sys.drewd;
+  if (dlrpot) *dlrpot = c->sys.dlrpot;
+  return FMC_OK;
+}
+
+extern "C" int fmc_guest_ewald3(const fmc_ctx* c, int gt, double* e3) {
+  if (!c || !c->sys.finalized || gt < 0 ||
+      gt >= static_cast<int>(c->sys.guests.size()) || !e3)
+    return FMC_ERR_ARG;
+  *e3 = c->sys.guests[gt].e3;
+  return FMC_OK;
+}
+
+extern "C" int fmc_guest_count(const fmc_ctx* c, int gt, int* n) {
+  if (!c || !c->sys.finalized || gt < 0 ||
+      gt >= static_cast<int>(c->sys.guests.size()) || !n)
+    return FMC_ERR_ARG;
+  *n = c->nmol[gt];
+  return FMC_OK;
+}
+
+extern "C" int fmc_get_molecule(const fmc_ctx* c, int gt, int m, double* x,
+                                double* y, double* z) {
+  if (!c || !c->sys.finalized || gt < 0 ||
+      gt >= static_cast<int>(c->sys.guests.size()) || m < 0 || m >= c->nmol[gt])
+    return FMC_ERR_ARG;
+  const int ns = c->sys.guests[gt].ns;
+  for (int s = 0; s < ns; ++s) {
+    x[s] = c->hx[gt][m * ns + s];
+    y[s] = c->hy[gt][m * ns + s];
+    z[s] = c->hz[gt][m * ns + s];
+  }
+  return FMC_OK;
+}
+
+extern "C" int fmc_get_table(const fmc_ctx* c, int which, int t, double* out) {
+  if (!c || !c->sys.finalized || !out) return FMC_ERR_ARG;
+  const int W = c->sys.mxgrid + 1;
+  if (which == 0) {
+    std::memcpy(out, c->sys.erc.data(), W * sizeof(double));
+    return FMC_OK;
+  }
+  if (t < 0 || t >= c->sys.ntable) return FMC_ERR_ARG;
+  std::memcpy(out, c->sys.vvv.data() + static_cast<size_t>(t) * W, W * sizeof(double));
+  return FMC_OK;
+}
+
+extern "C" int fmc_get_sk(const fmc_ctx* c, double* re, double* im) {
+  if (!c || !c->sys.finalized) return FMC_ERR_STATE;
+  HIPCHECK(hipMemcpy(re, c->d_sre[c->cur], c->nk * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(im, c->d_sim[c->cur], c->nk * sizeof(double), hipMemcpyDeviceToHost));
+  return FMC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI: hot path
+// ---------------------------------------------------------------------------
+
+extern "C" int fmc_recip_full(fmc_ctx* c, int all_atoms, double* energy) {
+  if (!c || !c->sys.finalized || !energy) return FMC_ERR_STATE;
+  c->pending = 0;
+  return recip_full_impl(c, all_atoms != 0, all_atoms != 0, energy);
+}
+
+extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
+                         const double* x, const double* y, const double* z,
+                         double* out) {
+  if (!c || !c->sys.finalized) return FMC_ERR_STATE;
+  if (gt < 0 || gt >= static_cast<int>(c->sys.guests.size()) || !out)
+    return FMC_ERR_ARG;
+  if (kind != FMC_INSERT && kind != FMC_DELETE && kind != FMC_DISPLACE)
+    return FMC_ERR_ARG;
+  const GuestTemplate& tp = c->sys.guests[gt];
+  const int ns = tp.ns;
+  if (kind != FMC_DELETE && (!x || !y || !z)) return FMC_ERR_ARG;
+  if (kind != FMC_INSERT && (imol < 0 || imol >= c->nmol[gt])) return FMC_ERR_ARG;
+  if (kind == FMC_INSERT) {
+    int used = 0;
+    for (size_t t = 0; t < c->sys.guests.size(); ++t)
+      used += c->nmol[t] * c->sys.guests[t].ns;
+    if (used + ns >= c->cap_atoms) return FMC_ERR_CAPACITY;  // error 75
+  }
+  TrialParams P;
+  fill_devsys(c, &P.S);
+  fill_segs(c, P.g, &P.ngt);
+  P.nk = c->nk;
+  P.kv = c->d_kv;
+  P.ak = c->d_ak;
+  P.sre = c->d_sre[c->cur];
+  P.sim = c->d_sim[c->cur];
+  P.nre = c->d_sre[1 - c->cur];
+  P.nim = c->d_sim[1 - c->cur];
+  P.kind = kind;
+  P.ns = ns;
+  P.own_t = gt;
+  P.own_begin = imol * ns;
+  for (int d = 0; d < 3; ++d) P.kmax[d] = c->sys.kmax[d];
+  for (int s = 0; s < kMaxSites; ++s) {
+    const bool in = s < ns;
+    P.q[s] = in ? tp.q[s] : 0.0;
+    P.t[s] = in ? tp.type[s] - 1 : 0;
+    P.nx[s] = (in && kind != FMC_DELETE) ? x[s] : 0.0;
+    P.ny[s] = (in && kind != FMC_DELETE) ? y[s] : 0.0;
+    P.nz[s] = (in && kind != FMC_DELETE) ? z[s] : 0.0;
+    P.ox[s] = (in && kind != FMC_INSERT) ? c->hx[gt][imol * ns + s] : 0.0;
+    P.oy[s] = (in && kind != FMC_INSERT) ? c->hy[gt][imol * ns + s] : 0.0;
+    P.oz[s] = (in && kind != FMC_INSERT) ? c->hz[gt][imol * ns + s] : 0.0;
+  }
+  P.rvolm2 = 2.0 * (2.0 * kPi / c->sys.volm);
+  P.engsic = tp.engsic;
+  P.nb_recip = std::max(1, std::min(c->nb_recip_max, (c->nk + kBS - 1) / kBS));
+  int ntot = P.S.nfw;
+  for (int t = 0; t < P.ngt; ++t) ntot += P.g[t].natoms;
+  P.nb_real = std::max(1, std::min(c->nb_real_max, (ntot + kBS - 1) / kBS));
+  P.partial = c->d_partial;
+  P.counter = c->d_counter;
+  P.out = c->d_out;
+  hipLaunchKernelGGL(trial_kernel, dim3(P.nb_recip + P.nb_real), dim3(kBS), 0,
+                     c->stream, P);
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, 3 * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  out[0] = c->h_out[0];
+  out[1] = c->h_out[1];
+  out[2] = c->h_out[2];
+  c->pending = kind;
+  c->pend_gt = gt;
+  c->pend_mol = imol;
+  for (int s = 0; s < ns; ++s) {
+    c->pend_x[s] = kind != FMC_DELETE ? x[s] : 0.0;
+    c->pend_y[s] = kind != FMC_DELETE ? y[s] : 0.0;
+    c->pend_z[s] = kind != FMC_DELETE ? z[s] : 0.0;
+  }
+  return FMC_OK;
+}
+
+extern "C" int fmc_commit(fmc_ctx* c) {
+  if (!c || !c->pending) return FMC_ERR_STATE;
+  const int gt = c->pend_gt;
+  const int ns = c->sys.guests[gt].ns;
+  int lo = 0, hi = 0;  // host-mirror range to upload
+  if (c->pending == FMC_INSERT) {
+    const int m = c->nmol[gt];
+    for (int s = 0; s < ns; ++s) {
+      c->hx[gt][m * ns + s] = c->pend_x[s];
+      c->hy[gt][m * ns + s] = c->pend_y[s];
+      c->hz[gt][m * ns + s] = c->pend_z[s];
+    }
+    c->nmol[gt] = m + 1;
+    lo = m * ns;
+    hi = (m + 1) * ns;
+  } else if (c->pending == FMC_DELETE) {
+    const int m = c->pend_mol;
+    const int n = c->nmol[gt];
+    for (int i = (m + 1) * ns; i < n * ns; ++i) {
+      c->hx[gt][i - ns] = c->hx[gt][i];
+      c->hy[gt][i - ns] = c->hy[gt][i];
+      c->hz[gt][i - ns] = c->hz[gt][i];
+    }
+    c->nmol[gt] = n - 1;
+    lo = m * ns;
+    hi = (n - 1) * ns;
+  } else {
+    const int m = c->pend_mol;
+    for (int s = 0; s < ns; ++s) {
+      c->hx[gt][m * ns + s] = c->pend_x[s];
+      c->hy[gt][m * ns + s] = c->pend_y[s];
+      c->hz[gt][m * ns + s] = c->pend_z[s];
+    }
+    lo = m * ns;
+    hi = (m + 1) * ns;
+  }
+  if (hi > lo) {
+    const size_t nb = (hi - lo) * sizeof(double);
+    HIPCHECK(hipMemcpyAsync(c->d_gx[gt] + lo, c->hx[gt].data() + lo, nb,
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_gy[gt] + lo, c->hy[gt].data() + lo, nb,
+                            hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->d_gz[gt] + lo, c->hz[gt].data() + lo, nb,
+                            hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  c->cur = 1 - c->cur;  // S(k) <- S_new
+  c->pending = 0;
+  return FMC_OK;
+}
+
+extern "C" int fmc_tail_delta(fmc_ctx* c, int gt, int sign, double elrc,
+                              double* delrc) {
+  if (!c || !c->sys.finalized || gt < 0 ||
+      gt >= static_cast<int>(c->sys.guests.size()) || !delrc)
+    return FMC_ERR_ARG;
+  std::vector<int> nt, nf;
+  c->sys.counts(c->nmol, &nt, &nf);
+  for (int s = 0; s < c->sys.guests[gt].ns; ++s)
+    nt[c->sys.guests[gt].type[s] - 1] += sign;
+  *delrc = c->sys.tail(nt, nf) - elrc;
+  return FMC_OK;
+}
+
+extern "C" int fmc_energy_full(fmc_ctx* c, double* out) {
+  if (!c || !c->sys.finalized || !out) return FMC_ERR_STATE;
+  c->pending = 0;
+  double efw = 0.0, eall = 0.0;
+  int r;
+  if ((r = recip_full_impl(c, false, false, &efw))) return r;
+  if ((r = recip_full_impl(c, true, true, &eall))) return r;
+  PairAllParams P;
+  fill_devsys(c, &P.S);
+  fill_segs(c, P.g, &P.ngt);
+  P.nguest = 0;
+  for (int t = 0; t < P.ngt; ++t) P.nguest += P.g[t].natoms;
+  double e2 = 0.0, ev = 0.0, e3 = 0.0;
+  if (P.nguest > 0) {
+    const int ntot = P.S.nfw + P.nguest;
+    const int gx = std::min(8, (ntot + kBS - 1) / kBS);
+    const size_t nbt = static_cast<size_t>(gx) * P.nguest;
+    double* part = nullptr;
+    HIPCHECK(hipMalloc(&part, 2 * nbt * sizeof(double)));
+    P.part = part;
+    hipLaunchKernelGGL(pair_all_kernel, dim3(gx, P.nguest), dim3(kBS), 0,
+                       c->stream, P);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, part,
+                       static_cast<int>(nbt), c->d_out);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, part + nbt,
+                       static_cast<int>(nbt), c->d_out + 1);
+    HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, 2 * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipFree(part));
+    e2 = c->h_out[0];
+    ev = c->h_out[1];
+    for (size_t t = 0; t < c->sys.guests.size(); ++t) {
+      const int ns = c->sys.guests[t].ns;
+      for (int m = 0; m < c->nmol[t]; ++m)
+        e3 += ewald3_molecule(c->sys, c->sys.guests[t], &c->hx[t][m * ns],
+                              &c->hy[t][m * ns], &c->hz[t][m * ns]);
+    }
+  }
+  std::vector<int> nt, nf;
+  c->sys.counts(c->nmol, &nt, &nf);
+  out[0] = efw;
+  out[1] = eall - efw;
+  out[2] = e2;
+  out[3] = ev;
+  out[4] = e3;
+  out[5] = c->sys.tail(nt, nf);
+  return FMC_OK;
+}
+
+extern "C" int fmc_version(int* major, int* minor) {
+  if (major) *major = 1;
+  if (minor) *minor = 0;
+  return FMC_OK;
+}
+
+extern "C" int fmc_device_ok(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return 0;
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, 0) != hipSuccess) return 0;
+  return std::strstr(p.gcnArchName, "gfx950") != nullptr ? 1 : 0;
+}
