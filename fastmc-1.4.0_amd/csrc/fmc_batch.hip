@@ -1,19 +1,1701 @@
-// Batched device-resident chains (persistent kernel filled in next).
+// Batched device-resident chains: many independent copies of the reference's
+// move loop (gcmc.f:589-1960) advance inside one persistent launch, one
+// chain per work-group.  Nothing returns to the host per move: duni, the
+// insert/delete/displace proposals, the energy of the trial, the acceptance
+// test and the commit all run on the GPU.
+//
+// Per chain, for the whole launch:
+//   * S(k) (FP64) and a_k sit in registers, KPT k-vectors per thread
+//     (k = tid + j*BT), so the reciprocal update reads no memory;
+//   * the trial's structure-factor change dS(k) stays in registers and is
+//     added to S(k) when the move is accepted;
+//   * per-site phase tables e^{i 2 pi l s} (ewald.f:97-137) sit in LDS;
+//   * real space walks a per-framework candidate list: the framework is cut
+//     into ~1.5 A cells and each cell lists every atom that can lie within
+//     rcut of a point in it (a superset, so the pair set -- and therefore the
+//     energy -- is exactly that of the reference's all-atom scan), plus all
+//     guest atoms;
+//   * one lane runs the serial part (RNG draws, proposal, acceptance).
+// Precision FP64 reproduces the oracle's accept/reject stream; MIXED does
+// the per-site phase products and the pair interpolation in FP32 with FP64
+// coordinates, S(k), energies and acceptance.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
 #include "fastmc_amd.h"
-extern "C" {
-int fmc_batch_create(fmc_batch**, int, int) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_destroy(fmc_batch*) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_add_chain(fmc_batch*, const fmc_ctx*, const fmc_chain_params*, int*) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_remove_chain(fmc_batch*, int) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_run(fmc_batch*, long long) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_launch(fmc_batch*, long long) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_sync(fmc_batch*) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_stats(fmc_batch*, int, fmc_chain_stats*) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_enable_log(fmc_batch*, int, long long) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_get_log(fmc_batch*, int, long long, double*) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_resync(fmc_batch*, double*) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_set_tile(fmc_batch*, int) { return FMC_ERR_UNSUPPORTED; }
-int fmc_batch_get_tile(const fmc_batch*, int*) { return FMC_ERR_UNSUPPORTED; }
-int fmc_chain_footprint(const fmc_ctx*, int, long long*) { return FMC_ERR_UNSUPPORTED; }
+#include "fmc_ctx.h"
+#include "fmc_device.h"
+#include "fmc_host.h"
+
+namespace fmc {
+
+constexpr int kBTab = 17;   // phase-table entries per dimension (kmax <= 16)
+constexpr int kLogFields = 8;
+
+struct BSys {
+  DevSys S;
+  int nk;
+  const int* kv;
+  const double* ak;
+  const double* sfw_re;   // framework S(k) (constant)
+  const double* sfw_im;
+  int gdim[3];
+  const int* goff;        // [ncell+1]
+  const unsigned short* gidx;
+  const float* erc32;
+  const float* vvv32;
+  int kmax[3];
+  double rvolm2;          // 2 * 2 pi / V
+  double volm;
+  int ngt;
+  int ns[kMaxGuestTypes];
+  double gq[kMaxGuestTypes][kMaxSites];
+  int gtype[kMaxGuestTypes][kMaxSites];  // 0-based
+  double gmass[kMaxGuestTypes][kMaxSites];
+  double tx[kMaxGuestTypes][kMaxSites], ty[kMaxGuestTypes][kMaxSites],
+      tz[kMaxGuestTypes][kMaxSites];
+  double e3[kMaxGuestTypes];
+  double engsic[kMaxGuestTypes];
+  // tail(n) = t0 + sum_g tb[g] n_g + sum_gh tA[g][h] n_g n_h
+  double t0, tb[kMaxGuestTypes], tA[kMaxGuestTypes][kMaxGuestTypes];
+};
+
+struct BChain {
+  int sys;
+  int ntpguest;
+  int cap_atoms;
+  double* sre;
+  double* sim;
+  double* dsre;   // dS scratch for the streaming (large-Nk) kernel
+  double* dsim;
+  double* gx[kMaxGuestTypes];
+  double* gy[kMaxGuestTypes];
+  double* gz[kMaxGuestTypes];
+  double temp, beta, statvolm, engunit;
+  double press[kMaxGuestTypes];
+  double cum[kMaxGuestTypes][9];
+  double disp_ratio[kMaxGuestTypes], tran_ratio[kMaxGuestTypes], rota_ratio;
+  long long eq_steps;
+  // RNG (gcmc.f:2712-2805)
+  float u[97];
+  float c, cd, cm;
+  int ir, jr;
+  // state
+  int nmol[kMaxGuestTypes];
+  double energy[kMaxGuestTypes];
+  double elrc;
+  double delrdisp[kMaxGuestTypes], tran_delr[kMaxGuestTypes], rotangle,
+      rota_rotangle;
+  int disp_count[kMaxGuestTypes], accept_disp[kMaxGuestTypes];
+  int tran_count[kMaxGuestTypes], accept_tran[kMaxGuestTypes];
+  int rota_count, accept_rota;
+  long long steps, prod_steps;
+  long long accepted[6], attempted[6];
+  double sum_n[kMaxGuestTypes], sum_n2[kMaxGuestTypes], sum_e[kMaxGuestTypes];
+  double* log;
+  long long log_cap, log_n;
+  int fault;   // 1: guest capacity reached (reference error 75)
+};
+
+// ---- RNG on one lane (gcmc.f:2786-2799) ----
+struct Rng {
+  float* u;
+  float c, cd, cm;
+  int ir, jr;
+  __device__ double next() {
+    float uni = u[ir - 1] - u[jr - 1];
+    if (uni < 0.0f) uni = uni + 1.0f;
+    u[ir - 1] = uni;
+    ir = ir - 1;
+    if (ir == 0) ir = 97;
+    jr = jr - 1;
+    if (jr == 0) jr = 97;
+    c = c - cd;
+    if (c < 0.0f) c = c + cm;
+    uni = uni - c;
+    if (uni < 0.0f) uni = uni + 1.0f;
+    return static_cast<double>(uni);
+  }
+};
+
+// quaternion rotation of rotation (utility_pack.f:2870-2906) with the
+// quaternion built as random_ins / random_rot do
+__device__ void rotate_sites(int n, double theta, double r2, double r3,
+                             double r4, double* x, double* y, double* z) {
+  const double b1 = 2.0 * r2 - 1.0, b2 = 2.0 * r3 - 1.0, b3 = 2.0 * r4 - 1.0;
+  double st, q1;
+  sincos(theta / 2.0, &st, &q1);
+  const double nrm = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+  const double q2 = st * (b1 / nrm), q3 = st * (b2 / nrm), q4 = st * (b3 / nrm);
+  double Q[9];
+  Q[0] = 2.0 * q2 * q2 + 2.0 * q1 * q1 - 1.0;
+  Q[1] = 2.0 * q2 * q3 - 2.0 * q1 * q4;
+  Q[2] = 2.0 * q2 * q4 + 2.0 * q1 * q3;
+  Q[3] = 2.0 * q2 * q3 + 2.0 * q1 * q4;
+  Q[4] = 2.0 * q1 * q1 + 2.0 * q3 * q3 - 1.0;
+  Q[5] = 2.0 * q3 * q4 - 2.0 * q1 * q2;
+  Q[6] = 2.0 * q2 * q4 - 2.0 * q1 * q3;
+  Q[7] = 2.0 * q3 * q4 + 2.0 * q1 * q2;
+  Q[8] = 2.0 * q4 * q4 + 2.0 * q1 * q1 - 1.0;
+  for (int i = 0; i < n; ++i) {
+    const double tx = Q[0] * x[i] + Q[1] * y[i] + Q[2] * z[i];
+    const double ty = Q[3] * x[i] + Q[4] * y[i] + Q[5] * z[i];
+    const double tz = Q[6] * x[i] + Q[7] * y[i] + Q[8] * z[i];
+    x[i] = tx; y[i] = ty; z[i] = tz;
+  }
+}
+
+// energy_eval (utility_pack.f:3049-3095)
+__device__ bool accept_test(double eng, double rande, double volm,
+                            double press, int ngsts, double temp, double beta,
+                            int kind) {
+  double test = 0.0;
+  if (-1.0 * beta * eng > 700.0) {
+    test = exp(700.0);
+  } else if (kind == FMC_DISPLACE) {
+    test = exp(-1.0 * beta * eng);
+  } else if (kind == FMC_INSERT) {
+    test = volm * press / (static_cast<double>(ngsts + 1) * kBoltz * temp) *
+           exp(-1.0 * beta * eng);
+  } else if (kind == FMC_DELETE) {
+    if (fabs(press) < 1e-7)
+      test = 1.0;
+    else
+      test = static_cast<double>(ngsts) * kBoltz * temp / (volm * press) *
+             exp(-1.0 * beta * eng);
+  }
+  return rande < test;
+}
+
+__device__ double tail_of(const BSys& Y, const int* n) {
+  double t = Y.t0;
+  for (int g = 0; g < Y.ngt; ++g) t += Y.tb[g] * n[g];
+  for (int g = 0; g < Y.ngt; ++g)
+    for (int h = 0; h < Y.ngt; ++h) t += Y.tA[g][h] * n[g] * n[h];
+  return t;
+}
+
+// The move as the serial lane hands it to the work-group.
+struct MoveDesc {
+  int kind;     // 0 no-op, FMC_INSERT/DELETE/DISPLACE
+  int sub;      // 0 plain, 4 translate-only, 5 rotate-only
+  int ig, imol, ns, slot;
+  int accepted;
+  double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
+  double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
+  // flattened real-space work: position p = (set, site) of the trial
+  int np;                       // active positions (ns or 2 ns)
+  int nguest;                   // guest atoms of every type
+  int gpre[kMaxGuestTypes + 1]; // guest-type prefix over atoms
+  int pset[2 * kMaxSites], psite[2 * kMaxSites];
+  int plo[2 * kMaxSites], plen[2 * kMaxSites];
+  int ppre[2 * kMaxSites + 1];  // prefix of candidate-list lengths
+};
+
+__device__ __forceinline__ int cell_of(const BSys& Y, double x, double y,
+                                       double z) {
+  const double* rc = Y.S.rcell;
+  int idx[3];
+  for (int d = 0; d < 3; ++d) {
+    double f = rc[d] * x + rc[3 + d] * y + rc[6 + d] * z;
+    f -= floor(f);
+    int i = static_cast<int>(f * Y.gdim[d]);
+    idx[d] = i >= Y.gdim[d] ? Y.gdim[d] - 1 : (i < 0 ? 0 : i);
+  }
+  return (idx[0] * Y.gdim[1] + idx[1]) * Y.gdim[2] + idx[2];
+}
+
+// Mixed-precision pair: FP64 separation and minimum image, FP32 distance
+// and table interpolation.
+__device__ __forceinline__ void pair_mixed(const BSys& Y, float chgea, int ts,
+                                           double dx, double dy, double dz,
+                                           float qj, int tj, float& e2,
+                                           float& ev) {
+  min_image(Y.S, dx, dy, dz);
+  const float fx = static_cast<float>(dx), fy = static_cast<float>(dy),
+              fz = static_cast<float>(dz);
+  const float rsq = fx * fx + fy * fy + fz * fz;
+  if (!(static_cast<float>(Y.S.rcut2) > rsq)) return;
+  const float r = sqrtf(rsq);
+  const int imax = Y.S.mxgrid - 2;
+  const float chgprd = chgea * qj;
+  {
+    const float u = r * static_cast<float>(Y.S.rdrewd);
+    int l = static_cast<int>(u);
+    l = l > imax ? imax : l;
+    const float p = u - static_cast<float>(l);
+    const float v0 = Y.erc32[l], v1 = Y.erc32[l + 1], v2 = Y.erc32[l + 2];
+    const float t1 = v0 + (v1 - v0) * p, t2 = v1 + (v2 - v1) * (p - 1.0f);
+    if (fabsf(chgprd) > 1e-10f) e2 += (t1 + (t2 - t1) * p * 0.5f) * chgprd;
+  }
+  const int k = Y.S.pairtab[ts * Y.S.ntpatm + tj];
+  if (k >= 0) {
+    const float* tab = Y.vvv32 + static_cast<size_t>(k) * (Y.S.mxgrid + 1);
+    const float u = r * static_cast<float>(Y.S.rdlrpot);
+    int l = static_cast<int>(u);
+    l = l > imax ? imax : l;
+    const float p = u - static_cast<float>(l);
+    const float v0 = tab[l], v1 = tab[l + 1], v2 = tab[l + 2];
+    const float t1 = v0 + (v1 - v0) * p, t2 = v1 + (v2 - v1) * (p - 1.0f);
+    ev += t1 + (t2 - t1) * p * 0.5f;
+  }
+}
+
+template <typename T>
+struct Cplx {
+  T x, y;
+};
+
+// e^{i k.r} * q for one site from the LDS phase tables, precision T
+template <typename T>
+__device__ __forceinline__ Cplx<T> sfac(const Cplx<T>* X, const Cplx<T>* Y,
+                                        const Cplx<T>* Z, int l, int m, int n,
+                                        T q) {
+  const Cplx<T> a = X[l];
+  const Cplx<T> b = Y[m < 0 ? -m : m];
+  const Cplx<T> c = Z[n < 0 ? -n : n];
+  T clm, slm;
+  if (m >= 0) {
+    clm = a.x * b.x - a.y * b.y;
+    slm = a.y * b.x + b.y * a.x;
+  } else {
+    clm = a.x * b.x + a.y * b.y;
+    slm = a.y * b.x - b.y * a.x;
+  }
+  if (n >= 0) return {q * (clm * c.x - slm * c.y), q * (slm * c.x + clm * c.y)};
+  return {q * (clm * c.x + slm * c.y), q * (slm * c.x - clm * c.y)};
+}
+
+__device__ __forceinline__ int ul(int v) { return static_cast<signed char>(v & 0xff); }
+__device__ __forceinline__ int um(int v) { return static_cast<signed char>((v >> 8) & 0xff); }
+__device__ __forceinline__ int un(int v) { return static_cast<signed char>((v >> 16) & 0xff); }
+
+// ---------------------------------------------------------------------------
+// The serial lane: move selection, adaptive steps, draws and proposal
+// (gcmc.f:745-847, 855-877, 1010-1060, 1176-1248; utility_pack.f:2545-3047)
+// ---------------------------------------------------------------------------
+__device__ void propose(const BSys& Y, BChain& C, Rng& R, MoveDesc& D) {
+  const double delrmin = 0.1, minangle = kPi / 18.0;
+  int ig = 0;
+  if (C.ntpguest > 1) ig = static_cast<int>(floor(R.next() * C.ntpguest));
+  const double randmov = R.next();
+  const double* cum = C.cum[ig];
+  int kind = 0, sub = 0;
+  if (randmov < cum[0]) kind = FMC_INSERT;
+  else if (randmov < cum[1]) kind = FMC_DELETE;
+  else if (randmov < cum[2]) kind = FMC_DISPLACE;
+  else if (randmov < cum[5]) kind = 0;        // jump (rejected at set-up), flex, swap
+  else if (randmov < cum[6]) { kind = FMC_DISPLACE; sub = 4; }
+  else if (randmov < cum[7]) { kind = FMC_DISPLACE; sub = 5; }
+  else if (randmov < cum[8]) kind = 0;
+  else kind = FMC_DISPLACE;
+  const int ns = Y.ns[ig];
+  const int nmols = C.nmol[ig];
+  if (nmols >= 1 && C.disp_count[ig] >= 1) {
+    if (C.disp_count[ig] % 100 == 0) {
+      if ((C.accept_disp[ig] / static_cast<double>(C.disp_count[ig])) > C.disp_ratio[ig])
+        C.delrdisp[ig] = C.delrdisp[ig] * 1.05;
+      else if (C.delrdisp[ig] > delrmin)
+        C.delrdisp[ig] = C.delrdisp[ig] * 0.95;
+    } else if (C.disp_count[ig] % 50 == 0) {
+      if ((C.accept_disp[ig] / static_cast<double>(C.disp_count[ig])) > C.disp_ratio[ig])
+        C.rotangle = C.rotangle * 1.05;
+      else if (C.rotangle > minangle)
+        C.rotangle = C.rotangle * 0.95;
+    }
+  }
+  if (nmols >= 1 && C.tran_count[ig] >= 1 && C.tran_count[ig] % 100 == 0) {
+    if ((C.accept_tran[ig] / static_cast<double>(C.tran_count[ig])) > C.tran_ratio[ig])
+      C.tran_delr[ig] = C.tran_delr[ig] * 1.05;
+    else if (C.tran_delr[ig] > delrmin)
+      C.tran_delr[ig] = C.tran_delr[ig] * 0.95;
+  }
+  if (nmols >= 1 && C.rota_count >= 1 && C.rota_count % 100 == 0) {
+    if ((C.accept_rota / static_cast<double>(C.rota_count)) > C.rota_ratio)
+      C.rota_rotangle = C.rota_rotangle * 1.05;
+    else if (C.rota_rotangle > minangle)
+      C.rota_rotangle = C.rota_rotangle * 0.95;
+  }
+  if (nmols == 0) { kind = FMC_INSERT; sub = 0; }
+  D.kind = kind;
+  D.sub = sub;
+  D.ig = ig;
+  D.ns = ns;
+  D.imol = -1;
+  D.accepted = 0;
+  if (kind == FMC_INSERT) {
+    int used = 0;
+    for (int g = 0; g < Y.ngt; ++g) used += C.nmol[g] * Y.ns[g];
+    if (used + ns >= C.cap_atoms) {
+      C.fault = 1;
+      D.kind = 0;
+      return;
+    }
+    const double ra = R.next(), rb = R.next(), rc = R.next();
+    const double r1 = R.next(), r2 = R.next(), r3 = R.next(), r4 = R.next();
+    const double* cell = Y.S.cell;
+    const double xc = ra * cell[0] + rb * cell[3] + rc * cell[6];
+    const double yc = ra * cell[1] + rb * cell[4] + rc * cell[7];
+    const double zc = ra * cell[2] + rb * cell[5] + rc * cell[8];
+    for (int s = 0; s < ns; ++s) {
+      D.nx[s] = Y.tx[ig][s];
+      D.ny[s] = Y.ty[ig][s];
+      D.nz[s] = Y.tz[ig][s];
+    }
+    rotate_sites(ns, r1 * 2.0 * kPi, r2, r3, r4, D.nx, D.ny, D.nz);
+    for (int s = 0; s < ns; ++s) {
+      D.nx[s] = D.nx[s] + xc;
+      D.ny[s] = D.ny[s] + yc;
+      D.nz[s] = D.nz[s] + zc;
+    }
+    D.slot = nmols * ns;
+  } else if (kind == FMC_DELETE) {
+    D.imol = static_cast<int>(floor(R.next() * nmols));
+    for (int s = 0; s < ns; ++s) {
+      D.ox[s] = C.gx[ig][D.imol * ns + s];
+      D.oy[s] = C.gy[ig][D.imol * ns + s];
+      D.oz[s] = C.gz[ig][D.imol * ns + s];
+    }
+  } else if (kind == FMC_DISPLACE) {
+    double ra, rb, rc, r1, r2, r3, r4, dd, da;
+    if (sub == 4) {
+      C.tran_count[ig]++;
+      ra = R.next(); rb = R.next(); rc = R.next();
+      r1 = r2 = r3 = r4 = 0.0;
+      dd = C.tran_delr[ig]; da = 0.0;
+    } else if (sub == 5) {
+      C.rota_count++;
+      ra = rb = rc = 0.5;
+      r1 = R.next(); r2 = R.next(); r3 = R.next(); r4 = R.next();
+      dd = 0.0; da = C.rota_rotangle;
+    } else {
+      C.disp_count[ig]++;
+      ra = R.next(); rb = R.next(); rc = R.next();
+      r1 = R.next(); r2 = R.next(); r3 = R.next(); r4 = R.next();
+      dd = C.delrdisp[ig]; da = C.rotangle;
+    }
+    D.imol = static_cast<int>(floor(R.next() * nmols));
+    double* x = D.nx;
+    double* y = D.ny;
+    double* z = D.nz;
+    for (int s = 0; s < ns; ++s) {
+      x[s] = D.ox[s] = C.gx[ig][D.imol * ns + s];
+      y[s] = D.oy[s] = C.gy[ig][D.imol * ns + s];
+      z[s] = D.oz[s] = C.gz[ig][D.imol * ns + s];
+    }
+    const double h = dd * 0.5;
+    const double ta = (2.0 * ra - 1.0) * h, tb = (2.0 * rb - 1.0) * h,
+                 tc = (2.0 * rc - 1.0) * h;
+    double w = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int s = 0; s < ns; ++s) {
+      x[s] = ta + x[s];
+      y[s] = tb + y[s];
+      z[s] = tc + z[s];
+    }
+    for (int s = 0; s < ns; ++s) {
+      const double m = Y.gmass[ig][s];
+      w = w + m;
+      cx = cx + x[s] * m;
+      cy = cy + y[s] * m;
+      cz = cz + z[s] * m;
+    }
+    cx = cx / w; cy = cy / w; cz = cz / w;
+    const double* rcl = Y.S.rcell;
+    const double* cell = Y.S.cell;
+    const double sx = (rcl[0] * cx + rcl[3] * cy + rcl[6] * cz);
+    const double sy = (rcl[1] * cx + rcl[4] * cy + rcl[7] * cz);
+    const double sz = (rcl[2] * cx + rcl[5] * cy + rcl[8] * cz);
+    const double xs = sx - floor(sx), ys = sy - floor(sy), zs = sz - floor(sz);
+    const double X = (cell[0] * xs + cell[3] * ys + cell[6] * zs);
+    const double Yv = (cell[1] * xs + cell[4] * ys + cell[7] * zs);
+    const double Z = (cell[2] * xs + cell[5] * ys + cell[8] * zs);
+    for (int s = 0; s < ns; ++s) {
+      x[s] = x[s] + (X - cx);
+      y[s] = y[s] + (Yv - cy);
+      z[s] = z[s] + (Z - cz);
+    }
+    for (int s = 0; s < ns; ++s) {
+      x[s] = x[s] - X;
+      y[s] = y[s] - Yv;
+      z[s] = z[s] - Z;
+    }
+    rotate_sites(ns, r1 * da, r2, r3, r4, x, y, z);
+    for (int s = 0; s < ns; ++s) {
+      x[s] = x[s] + X;
+      y[s] = y[s] + Yv;
+      z[s] = z[s] + Z;
+    }
+  }
+  int np = 0;
+  if (kind != 0) {
+    if (kind != FMC_INSERT)
+      for (int s = 0; s < ns; ++s) { D.pset[np] = 0; D.psite[np] = s; ++np; }
+    if (kind != FMC_DELETE)
+      for (int s = 0; s < ns; ++s) { D.pset[np] = 1; D.psite[np] = s; ++np; }
+  }
+  D.np = np;
+  int na = 0;
+  for (int g = 0; g < Y.ngt; ++g) {
+    D.gpre[g] = na;
+    na += C.nmol[g] * Y.ns[g];
+  }
+  D.gpre[Y.ngt] = na;
+  D.nguest = na;
+}
+
+// Acceptance and bookkeeping on the serial lane (gcmc.f:938-1002, 1106-1172,
+// 1326-1378, 1814-1870).
+__device__ void decide(const BSys& Y, BChain& C, Rng& R, MoveDesc& D,
+                       double e1, double e2, double ev) {
+  const int ig = D.ig;
+  const int nmols = C.nmol[ig];
+  double estep = 0.0, delrc = 0.0;
+  bool acc = false;
+  const int slot = D.kind == 0 ? 0 : (D.sub ? D.sub : D.kind);
+  if (D.kind == FMC_INSERT || D.kind == FMC_DELETE) {
+    int n2[kMaxGuestTypes];
+    for (int g = 0; g < Y.ngt; ++g) n2[g] = C.nmol[g];
+    n2[ig] += (D.kind == FMC_INSERT) ? 1 : -1;
+    delrc = tail_of(Y, n2) - C.elrc;
+    if (D.kind == FMC_INSERT) {
+      e1 = e1 + Y.engsic[ig];
+      estep = (e1 + e2 + Y.e3[ig] + ev + delrc) / C.engunit;
+    } else {
+      e1 = e1 - Y.engsic[ig];
+      estep = (e1 - e2 - Y.e3[ig] - ev + delrc) / C.engunit;
+    }
+    const double rande = R.next();
+    acc = accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp,
+                      C.beta, D.kind);
+    if (acc) {
+      C.elrc = C.elrc + delrc;
+      C.nmol[ig] = nmols + (D.kind == FMC_INSERT ? 1 : -1);
+    }
+  } else if (D.kind == FMC_DISPLACE) {
+    estep = (e1 + e2 + ev) / C.engunit;
+    if (estep < 0.0) {
+      acc = true;
+    } else {
+      const double rande = R.next();
+      acc = accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp,
+                        C.beta, FMC_DISPLACE);
+    }
+    if (acc) {
+      if (D.sub == 4) C.accept_tran[ig]++;
+      else if (D.sub == 5) C.accept_rota++;
+      else C.accept_disp[ig]++;
+    }
+  }
+  if (acc) C.energy[ig] += estep;
+  C.attempted[slot]++;
+  if (acc) C.accepted[slot]++;
+  D.accepted = acc ? 1 : 0;
+  C.steps++;
+  if (C.steps > C.eq_steps) {
+    C.prod_steps++;
+    for (int g = 0; g < C.ntpguest; ++g) {
+      const double n = C.nmol[g];
+      C.sum_n[g] += n;
+      C.sum_n2[g] += n * n;
+      C.sum_e[g] += C.energy[g];
+    }
+  }
+  if (C.log && C.log_n < C.log_cap) {
+    double* r = C.log + kLogFields * C.log_n;
+    r[0] = slot;
+    r[1] = ig;
+    r[2] = D.accepted;
+    r[3] = C.nmol[ig];
+    r[4] = estep;
+    r[5] = e1;
+    r[6] = e2;
+    r[7] = ev;
+    C.log_n++;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The persistent kernel
+// ---------------------------------------------------------------------------
+template <int BT, int KPT, bool MIXED>
+__global__ void __launch_bounds__(BT)
+    batch_kernel(const BSys* __restrict__ systems, BChain* __restrict__ chains,
+                 const int* __restrict__ ids, long long nmoves, int nkcap) {
+  using T = typename std::conditional<MIXED, float, double>::type;
+  __shared__ Cplx<T> tab[2 * kMaxSites][3][kBTab];   // per position p
+  __shared__ MoveDesc D;
+  __shared__ float rng_u[97];
+  __shared__ double red[3][BT / kWave];
+  __shared__ int shift_lo, shift_hi;
+  __shared__ BChain C;
+
+  const int tid = threadIdx.x;
+  // The system description and the chain's buffer pointers are read from
+  // global memory (uniform addresses: scalar loads); the chain's mutable
+  // state lives in LDS for the serial lane.
+  BChain& Cg = chains[ids[blockIdx.x]];
+  const BSys& Y = systems[Cg.sys];
+  {
+    const int* src = reinterpret_cast<const int*>(&Cg);
+    int* dst = reinterpret_cast<int*>(&C);
+    for (int i = tid; i < static_cast<int>(sizeof(BChain) / 4); i += BT) dst[i] = src[i];
+  }
+  const int nk = Y.nk;
+  // a_k and the packed (l,m,n) of every k-vector in LDS; this thread owns
+  // the contiguous k = tid*KPT + j, whose S(k) stays in registers
+  // dynamic LDS: a_k [nkcap], packed k index [nkcap], then the XY tables
+  extern __shared__ double dyn_lds[];
+  double* ak_l = dyn_lds;
+  int* kv_l = reinterpret_cast<int*>(dyn_lds + nkcap);
+  // X[l]*Y[m] for every (l, m) of the half space, per trial position:
+  // [p][l*(2*kmax2+1) + m + kmax2]
+  const int nlm = (Y.kmax[0] + 1) * (2 * Y.kmax[1] + 1);
+  Cplx<T>* xy_l = reinterpret_cast<Cplx<T>*>(kv_l + ((nkcap + 3) & ~3));
+  // k-vectors spread evenly: this thread owns kpt = ceil(nk/BT) contiguous
+  // k = tid*kpt + j; the index (l, m) is precomputed into kv's upper bits
+  const int kpt = (nk + BT - 1) / BT;
+  for (int k = tid; k < nk; k += BT) {
+    ak_l[k] = Y.ak[k];
+    const int v = Y.kv[k];
+    const int lm = ul(v) * (2 * Y.kmax[1] + 1) + um(v) + Y.kmax[1];
+    kv_l[k] = (v & 0xff0000) | lm;   // n in bits 16..23, (l,m) index below
+  }
+  const int k0 = tid * kpt;
+  double sre[KPT], sim[KPT];
+  double nre[KPT], nim[KPT];   // S_new of the trial (FP64) or S + dS
+#pragma unroll
+  for (int j = 0; j < KPT; ++j) {
+    const int k = k0 + j;
+    const bool in = j < kpt && k < nk;
+    sre[j] = in ? Cg.sre[k] : 0.0;
+    sim[j] = in ? Cg.sim[k] : 0.0;
+  }
+  Rng R;
+  __syncthreads();
+  if (tid == 0) {
+    for (int i = 0; i < 97; ++i) rng_u[i] = C.u[i];
+    R.u = rng_u;
+    R.c = C.c; R.cd = C.cd; R.cm = C.cm; R.ir = C.ir; R.jr = C.jr;
+  }
+  __syncthreads();
+
+  for (long long mv = 0; mv < nmoves; ++mv) {
+    if (tid == 0) propose(Y, C, R, D);
+    __syncthreads();
+    const int kind = D.kind;
+    const int ns = D.ns;
+    const int ig = D.ig;
+    double e1 = 0.0, e2 = 0.0, ev = 0.0;
+    if (kind != 0) {
+      const int np = D.np;
+      // phase factors e^{i 2 pi l f} of every position, dimension and l
+      // (ewald.f:97-137), one sincospi each, in parallel; candidate-list
+      // bounds of every position's cell by the last wave
+      {
+        const int nl = kBTab;
+        for (int t = tid; t < np * 3 * nl; t += BT) {
+          const int p = t / (3 * nl), r = t - p * 3 * nl;
+          const int d = r / nl, l = r - d * nl;
+          if (l > Y.kmax[d]) continue;
+          const int set = D.pset[p], s = D.psite[p];
+          const double x = set ? D.nx[s] : D.ox[s];
+          const double y = set ? D.ny[s] : D.oy[s];
+          const double z = set ? D.nz[s] : D.oz[s];
+          double f = Y.S.rcell[d] * x + Y.S.rcell[3 + d] * y + Y.S.rcell[6 + d] * z;
+          f = f - floor(f);
+          double ph = l * f;
+          ph = ph - rint(ph);
+          double sv, cv;
+          sincospi(2.0 * ph, &sv, &cv);
+          tab[p][d][l] = {static_cast<T>(cv), static_cast<T>(sv)};
+        }
+        if (tid >= BT - kWave && tid < BT - kWave + np) {
+          const int p = tid - (BT - kWave);
+          const int set = D.pset[p], s = D.psite[p];
+          const int c = set ? cell_of(Y, D.nx[s], D.ny[s], D.nz[s])
+                            : cell_of(Y, D.ox[s], D.oy[s], D.oz[s]);
+          const int lo = Y.goff[c];
+          D.plo[p] = lo;
+          D.plen[p] = Y.goff[c + 1] - lo;
+        }
+      }
+      __syncthreads();
+      // X[l] * Y[m] for every (l, m) and position (m < 0 conjugates Y)
+      {
+        const int w2 = 2 * Y.kmax[1] + 1;
+        for (int t = tid; t < np * nlm; t += BT) {
+          const int p = t / nlm, r = t - p * nlm;
+          const int l = r / w2, m = r - l * w2 - Y.kmax[1];
+          const Cplx<T> a = tab[p][0][l];
+          const Cplx<T> b = tab[p][1][m < 0 ? -m : m];
+          Cplx<T> o;
+          if (m >= 0) {
+            o.x = a.x * b.x - a.y * b.y;
+            o.y = a.y * b.x + b.y * a.x;
+          } else {
+            o.x = a.x * b.x + a.y * b.y;
+            o.y = a.y * b.x - b.y * a.x;
+          }
+          xy_l[t] = o;
+        }
+      }
+      __syncthreads();
+      // ---- reciprocal space (ewald1_guest, ewald.f:141-319) ----
+      // dS(k) = sum_new q e^{ik.r} - sum_old q e^{ik.r}
+      {
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+          if (j >= kpt || k0 + j >= nk) break;
+          const int v = kv_l[k0 + j];
+          const int lm = v & 0xffff;
+          const int n = static_cast<signed char>((v >> 16) & 0xff);
+          const int an = n < 0 ? -n : n;
+          T dr = 0, di = 0;
+          for (int p = 0; p < np; ++p) {
+            const int set = D.pset[p];
+            const T q = static_cast<T>(set ? Y.gq[ig][D.psite[p]] : -Y.gq[ig][D.psite[p]]);
+            const Cplx<T> a = xy_l[p * nlm + lm];
+            const Cplx<T> c = tab[p][2][an];
+            if (n >= 0) {
+              dr += q * (a.x * c.x - a.y * c.y);
+              di += q * (a.y * c.x + a.x * c.y);
+            } else {
+              dr += q * (a.x * c.x + a.y * c.y);
+              di += q * (a.y * c.x - a.x * c.y);
+            }
+          }
+          // |S + dS|^2 - |S|^2 = 2 S.dS + |dS|^2: no cancellation of |S|^2
+          const double ddr = static_cast<double>(dr);
+          const double ddi = static_cast<double>(di);
+          nre[j] = sre[j] + ddr;
+          nim[j] = sim[j] + ddi;
+          e1 += ak_l[k0 + j] * (2.0 * (sre[j] * ddr + sim[j] * ddi) + (ddr * ddr + ddi * ddi));
+        }
+      }
+      if (tid == 0) {
+        int acc = 0;
+        for (int p = 0; p < np; ++p) {
+          D.ppre[p] = acc;
+          acc += D.plen[p];
+        }
+        D.ppre[np] = acc;
+      }
+      __syncthreads();
+      // ---- real space: framework candidates of every position's cell and
+      //      every guest atom, one flattened loop (ewald2 + srfrce) ----
+      {
+        const int nfwp = D.ppre[np];
+        const int ng = D.nguest;
+        const int total = nfwp + np * ng;
+        const int own0 = (kind != FMC_INSERT) ? D.gpre[ig] + D.imol * ns : -1;
+        T a2 = 0, av = 0;
+        for (int idx = tid; idx < total; idx += BT) {
+          int p;
+          double ax, ay, az, aq;
+          int at;
+          if (idx < nfwp) {
+            p = 0;
+            while (p + 1 < np && D.ppre[p + 1] <= idx) ++p;
+            const int a = Y.gidx[D.plo[p] + (idx - D.ppre[p])];
+            ax = Y.S.fx[a]; ay = Y.S.fy[a]; az = Y.S.fz[a]; aq = Y.S.fq[a];
+            at = Y.S.ftype[a];
+          } else {
+            const int r = idx - nfwp;
+            p = r / ng;
+            const int a = r - p * ng;
+            if (a >= own0 && a < own0 + ns && own0 >= 0) continue;
+            int g = 0;
+            while (g + 1 < Y.ngt && D.gpre[g + 1] <= a) ++g;
+            const int la = a - D.gpre[g];
+            const int sa = la % Y.ns[g];
+            ax = Cg.gx[g][la]; ay = Cg.gy[g][la]; az = Cg.gz[g][la];
+            aq = Y.gq[g][sa];
+            at = Y.gtype[g][sa];
+          }
+          const int set = D.pset[p], s = D.psite[p];
+          const double px = set ? D.nx[s] : D.ox[s];
+          const double py = set ? D.ny[s] : D.oy[s];
+          const double pz = set ? D.nz[s] : D.oz[s];
+          T b2 = 0, bv = 0;
+          if constexpr (MIXED) {
+            const float ce = static_cast<float>(Y.gq[ig][s] / Y.S.epsq * kR4pie0);
+            pair_mixed(Y, ce, Y.gtype[ig][s], px - ax, py - ay, pz - az,
+                       static_cast<float>(aq), at, b2, bv);
+          } else {
+            const double ce = Y.gq[ig][s] / Y.S.epsq * kR4pie0;
+            pair_fp64(Y.S, ce, Y.gtype[ig][s], px - ax, py - ay, pz - az, aq, at,
+                      b2, bv);
+          }
+          if (kind == FMC_DISPLACE && set == 0) { a2 -= b2; av -= bv; }
+          else { a2 += b2; av += bv; }
+        }
+        e2 = static_cast<double>(a2);
+        ev = static_cast<double>(av);
+      }
+    }
+    // block reductions (fixed tree)
+    e1 = wave_sum(e1);
+    e2 = wave_sum(e2);
+    ev = wave_sum(ev);
+    if ((tid % kWave) == 0) {
+      red[0][tid / kWave] = e1;
+      red[1][tid / kWave] = e2;
+      red[2][tid / kWave] = ev;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0.0, b = 0.0, c = 0.0;
+      for (int w = 0; w < BT / kWave; ++w) {
+        a += red[0][w];
+        b += red[1][w];
+        c += red[2][w];
+      }
+      a = Y.rvolm2 * kR4pie0 * a / Y.S.epsq;
+      decide(Y, C, R, D, a, b, c);
+      shift_lo = 0;
+      shift_hi = 0;
+      if (D.accepted && kind == FMC_DELETE) {
+        shift_lo = D.imol * ns;
+        shift_hi = C.nmol[ig] * ns;   // already decremented
+      }
+    }
+    __syncthreads();
+    if (D.accepted) {
+      // S(k) <- S_new (ewald.f:260-293, gcmc.f:971-974)
+#pragma unroll
+      for (int j = 0; j < KPT; ++j) {
+        if (j < kpt) {
+          sre[j] = nre[j];
+          sim[j] = nim[j];
+        }
+      }
+      if (kind == FMC_INSERT) {
+        if (tid < ns) {
+          Cg.gx[ig][D.slot + tid] = D.nx[tid];
+          Cg.gy[ig][D.slot + tid] = D.ny[tid];
+          Cg.gz[ig][D.slot + tid] = D.nz[tid];
+        }
+      } else if (kind == FMC_DISPLACE) {
+        if (tid < ns) {
+          Cg.gx[ig][D.imol * ns + tid] = D.nx[tid];
+          Cg.gy[ig][D.imol * ns + tid] = D.ny[tid];
+          Cg.gz[ig][D.imol * ns + tid] = D.nz[tid];
+        }
+      } else if (kind == FMC_DELETE) {
+        // successors shift down by one molecule (gcmc.f:1127-1131)
+        for (int base = shift_lo; base < shift_hi; base += BT) {
+          const int i = base + tid;
+          double x = 0, y = 0, z = 0;
+          if (i < shift_hi) {
+            x = Cg.gx[ig][i + ns];
+            y = Cg.gy[ig][i + ns];
+            z = Cg.gz[ig][i + ns];
+          }
+          __syncthreads();
+          if (i < shift_hi) {
+            Cg.gx[ig][i] = x;
+            Cg.gy[ig][i] = y;
+            Cg.gz[ig][i] = z;
+          }
+          __syncthreads();
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // write the state back
+#pragma unroll
+  for (int j = 0; j < KPT; ++j) {
+    const int k = k0 + j;
+    if (j < kpt && k < nk) {
+      Cg.sre[k] = sre[j];
+      Cg.sim[k] = sim[j];
+    }
+  }
+  if (tid == 0) {
+    for (int i = 0; i < 97; ++i) C.u[i] = rng_u[i];
+    C.c = R.c; C.ir = R.ir; C.jr = R.jr;
+  }
+  __syncthreads();
+  {
+    const int* src = reinterpret_cast<const int*>(&C);
+    int* dst = reinterpret_cast<int*>(&Cg);
+    for (int i = tid; i < static_cast<int>(sizeof(BChain) / 4); i += BT) dst[i] = src[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Candidate grid (built once per framework)
+// ---------------------------------------------------------------------------
+struct GridParams {
+  DevSys S;
+  int gdim[3];
+  double reach2;   // (rcut + half the cell diagonal)^2
+  int* count;
+  int* off;
+  unsigned short* idx;
+};
+
+__device__ __forceinline__ void grid_center(const GridParams& P, int c,
+                                            double& x, double& y, double& z) {
+  const int i2 = c % P.gdim[2];
+  const int i1 = (c / P.gdim[2]) % P.gdim[1];
+  const int i0 = c / (P.gdim[2] * P.gdim[1]);
+  const double f0 = (i0 + 0.5) / P.gdim[0], f1 = (i1 + 0.5) / P.gdim[1],
+               f2 = (i2 + 0.5) / P.gdim[2];
+  x = P.S.cell[0] * f0 + P.S.cell[3] * f1 + P.S.cell[6] * f2;
+  y = P.S.cell[1] * f0 + P.S.cell[4] * f1 + P.S.cell[7] * f2;
+  z = P.S.cell[2] * f0 + P.S.cell[5] * f1 + P.S.cell[8] * f2;
+}
+
+__global__ void __launch_bounds__(256) grid_count_kernel(GridParams P) {
+  __shared__ int cnt;
+  const int c = blockIdx.x;
+  double cx, cy, cz;
+  grid_center(P, c, cx, cy, cz);
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  int mine = 0;
+  for (int a = threadIdx.x; a < P.S.nfw; a += 256) {
+    double dx = P.S.fx[a] - cx, dy = P.S.fy[a] - cy, dz = P.S.fz[a] - cz;
+    min_image(P.S, dx, dy, dz);
+    if (dx * dx + dy * dy + dz * dz <= P.reach2) ++mine;
+  }
+  atomicAdd(&cnt, mine);
+  __syncthreads();
+  if (threadIdx.x == 0) P.count[c] = cnt;
+}
+
+// ordered fill: candidates of a cell in increasing atom index
+__global__ void __launch_bounds__(256) grid_fill_kernel(GridParams P) {
+  __shared__ int scan[256];
+  __shared__ int base;
+  const int c = blockIdx.x;
+  double cx, cy, cz;
+  grid_center(P, c, cx, cy, cz);
+  if (threadIdx.x == 0) base = P.off[c];
+  __syncthreads();
+  for (int a0 = 0; a0 < P.S.nfw; a0 += 256) {
+    const int a = a0 + threadIdx.x;
+    int f = 0;
+    if (a < P.S.nfw) {
+      double dx = P.S.fx[a] - cx, dy = P.S.fy[a] - cy, dz = P.S.fz[a] - cz;
+      min_image(P.S, dx, dy, dz);
+      f = (dx * dx + dy * dy + dz * dz <= P.reach2) ? 1 : 0;
+    }
+    scan[threadIdx.x] = f;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      const int v = threadIdx.x >= o ? scan[threadIdx.x - o] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    if (f) P.idx[base + scan[threadIdx.x] - 1] = static_cast<unsigned short>(a);
+    __syncthreads();
+    if (threadIdx.x == 255) base += scan[255];
+    __syncthreads();
+  }
+}
+
+// framework structure factor and guest resync: S(k) = S_fw(k) + guests
+struct ResyncParams {
+  const BSys* sys;
+  BChain* chains;
+  const int* ids;
+  double* maxdev;   // per chain
+};
+
+__global__ void __launch_bounds__(256) resync_kernel(ResyncParams P) {
+  __shared__ double red[256 / kWave];
+  BChain& C = P.chains[P.ids[blockIdx.x]];
+  const BSys& Y = P.sys[C.sys];
+  double dev = 0.0;
+  for (int k = threadIdx.x; k < Y.nk; k += 256) {
+    const int v = Y.kv[k];
+    const int l = ul(v), m = um(v), n = un(v);
+    double sx = Y.sfw_re[k], sy = Y.sfw_im[k];
+    for (int g = 0; g < Y.ngt; ++g) {
+      const int nat = C.nmol[g] * Y.ns[g];
+      for (int a = 0; a < nat; ++a) {
+        const double x = C.gx[g][a], y = C.gy[g][a], z = C.gz[g][a];
+        const double* rc = Y.S.rcell;
+        double ph = l * (rc[0] * x + rc[3] * y + rc[6] * z) +
+                    m * (rc[1] * x + rc[4] * y + rc[7] * z) +
+                    n * (rc[2] * x + rc[5] * y + rc[8] * z);
+        ph -= rint(ph);
+        double s, c;
+        sincospi(2.0 * ph, &s, &c);
+        const double q = Y.gq[g][a % Y.ns[g]];
+        sx += q * c;
+        sy += q * s;
+      }
+    }
+    dev = fmax(dev, fmax(fabs(sx - C.sre[k]), fabs(sy - C.sim[k])));
+    C.sre[k] = sx;
+    C.sim[k] = sy;
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) dev = fmax(dev, __shfl_xor(dev, off, kWave));
+  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = dev;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < 256 / kWave; ++w) m = fmax(m, red[w]);
+    P.maxdev[blockIdx.x] = m;
+  }
+}
+
+// S(k) of n atoms (FP64, direct phases): the framework part used by resync
+__global__ void __launch_bounds__(256) sfac_kernel(int nk, const int* kv,
+                                                   const double* rcell, int n,
+                                                   const double* x, const double* y,
+                                                   const double* z, const double* q,
+                                                   double* re, double* im) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nk) return;
+  const int v = kv[k];
+  const int l = ul(v), m = um(v), nn = un(v);
+  double sx = 0.0, sy = 0.0;
+  for (int a = 0; a < n; ++a) {
+    double ph = l * (rcell[0] * x[a] + rcell[3] * y[a] + rcell[6] * z[a]) +
+                m * (rcell[1] * x[a] + rcell[4] * y[a] + rcell[7] * z[a]) +
+                nn * (rcell[2] * x[a] + rcell[5] * y[a] + rcell[8] * z[a]);
+    ph -= rint(ph);
+    double sv, cv;
+    sincospi(2.0 * ph, &sv, &cv);
+    sx += q[a] * cv;
+    sy += q[a] * sv;
+  }
+  re[k] = sx;
+  im[k] = sy;
+}
+
+}  // namespace fmc
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+using namespace fmc;
+
+#define BCHECK(x)                                                        \
+  do {                                                                   \
+    hipError_t e_ = (x);                                                 \
+    if (e_ != hipSuccess) {                                              \
+      std::fprintf(stderr, "fastmc_amd batch: HIP error %s at %s:%d\n",  \
+                   hipGetErrorString(e_), __FILE__, __LINE__);           \
+      return FMC_ERR_HIP;                                                \
+    }                                                                    \
+  } while (0)
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+template <typename T>
+int dput(std::vector<std::unique_ptr<DevBuf>>& keep, T** out, const T* h,
+         size_t n) {
+  auto b = std::make_unique<DevBuf>();
+  const size_t bytes = std::max<size_t>(1, n) * sizeof(T);
+  BCHECK(hipMalloc(&b->p, bytes));
+  b->n = bytes;
+  if (h && n) BCHECK(hipMemcpy(b->p, h, n * sizeof(T), hipMemcpyHostToDevice));
+  *out = static_cast<T*>(b->p);
+  keep.push_back(std::move(b));
+  return FMC_OK;
+}
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+uint64_t system_key(const SystemDesc& s) {
+  uint64_t h = 1469598103934665603ull;
+  h = fnv(h, &s.imcon, sizeof(int));
+  h = fnv(h, s.cell, sizeof(s.cell));
+  h = fnv(h, &s.alpha, sizeof(double));
+  h = fnv(h, s.kmax, sizeof(s.kmax));
+  h = fnv(h, &s.rcut, sizeof(double));
+  h = fnv(h, &s.rvdw, sizeof(double));
+  h = fnv(h, &s.epsq, sizeof(double));
+  h = fnv(h, s.fx.data(), s.fx.size() * sizeof(double));
+  h = fnv(h, s.fy.data(), s.fy.size() * sizeof(double));
+  h = fnv(h, s.fz.data(), s.fz.size() * sizeof(double));
+  h = fnv(h, s.fq.data(), s.fq.size() * sizeof(double));
+  h = fnv(h, s.ftype.data(), s.ftype.size() * sizeof(int));
+  h = fnv(h, s.vdw_prm.data(), s.vdw_prm.size() * sizeof(double));
+  h = fnv(h, s.vdw_key.data(), s.vdw_key.size() * sizeof(int));
+  h = fnv(h, s.vdw_a.data(), s.vdw_a.size() * sizeof(int));
+  h = fnv(h, s.vdw_b.data(), s.vdw_b.size() * sizeof(int));
+  for (const auto& g : s.guests) h = fnv(h, &g, sizeof(GuestTemplate));
+  return h;
+}
+
+struct SysHost {
+  uint64_t key;
+  BSys dev;
+  size_t bytes = 0;
+  std::vector<std::unique_ptr<DevBuf>> keep;
+};
+
+struct ChainHost {
+  bool active = false;
+  int sys = -1;
+  std::vector<std::unique_ptr<DevBuf>> keep;
+  size_t bytes = 0;
+};
+
+void duni_setup(int i, int j, int k, int l, BChain* C) {
+  // gcmc.f:2760-2784, then one discarded draw (gcmc.f:335-336)
+  C->ir = 97;
+  C->jr = 33;
+  for (int ii = 0; ii < 97; ++ii) {
+    float s = 0.0f, t = 0.5f;
+    for (int jj = 0; jj < 24; ++jj) {
+      const int m = ((i * j) % 179) * k % 179;
+      i = j;
+      j = k;
+      k = m;
+      l = (53 * l + 1) % 169;
+      if ((l * m) % 64 >= 32) s = s + t;
+      t = 0.5f * t;
+    }
+    C->u[ii] = s;
+  }
+  C->c = 362436.0f / 16777216.0f;
+  C->cd = 7654321.0f / 16777216.0f;
+  C->cm = 16777213.0f / 16777216.0f;
+  float uni = C->u[C->ir - 1] - C->u[C->jr - 1];
+  if (uni < 0.0f) uni += 1.0f;
+  C->u[C->ir - 1] = uni;
+  C->ir = C->ir - 1 == 0 ? 97 : C->ir - 1;
+  C->jr = C->jr - 1 == 0 ? 97 : C->jr - 1;
+  C->c = C->c - C->cd;
+  if (C->c < 0.0f) C->c += C->cm;
+}
+
+}  // namespace
+
+struct fmc_batch {
+  int max_chains = 0;
+  int precision = FMC_PREC_FP64;
+  int bt = 0, kpt = 0;   // 0 = autotune at the next run
+  std::vector<std::unique_ptr<SysHost>> systems;
+  std::vector<std::unique_ptr<ChainHost>> chains;
+  BSys* d_sys = nullptr;
+  int d_sys_cap = 0;
+  BChain* d_chains = nullptr;
+  int* d_ids = nullptr;
+  double* d_dev = nullptr;
+  std::vector<BChain> h_chains;
+  bool dirty_sys = true;
+  hipStream_t stream = nullptr;
+  int max_nk = 0;
+  int max_nlm = 0;   // (kmax1+1)(2 kmax2+1) over the systems
+  int max_pos = 0;   // trial positions: 2 x the largest guest
+  std::vector<int> active_ids;
+};
+
+namespace {
+
+using KernelFn = void (*)(const BSys*, BChain*, const int*, long long, int);
+
+struct KCfg {
+  int bt, kpt;
+  bool mixed;
+  KernelFn fn;
+};
+
+#define KC(BT_, KPT_)                                                      \
+  {BT_, KPT_, false, batch_kernel<BT_, KPT_, false>},                       \
+      {BT_, KPT_, true, batch_kernel<BT_, KPT_, true>}
+
+const KCfg kConfigs[] = {
+    KC(256, 8),  KC(256, 16), KC(256, 28), KC(512, 4),  KC(512, 8),
+    KC(512, 14), KC(1024, 2), KC(1024, 4), KC(1024, 7), KC(1024, 10),
+};
+
+const KCfg* find_cfg(int bt, int kpt, bool mixed) {
+  for (const auto& c : kConfigs)
+    if (c.bt == bt && c.kpt == kpt && c.mixed == mixed) return &c;
+  return nullptr;
+}
+
+// smallest template KPT for block size bt that covers nk
+int kpt_for(int bt, int nk) {
+  int best = 0;
+  for (const auto& c : kConfigs)
+    if (c.bt == bt && !c.mixed && c.bt * c.kpt >= nk && (best == 0 || c.kpt < best))
+      best = c.kpt;
+  return best;
+}
+
+int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
+  const SystemDesc& s = ctx->sys;
+  const uint64_t key = system_key(s);
+  for (size_t i = 0; i < b->systems.size(); ++i)
+    if (b->systems[i]->key == key) {
+      *out = static_cast<int>(i);
+      return FMC_OK;
+    }
+  auto sh = std::make_unique<SysHost>();
+  sh->key = key;
+  BSys& Y = sh->dev;
+  std::memset(&Y, 0, sizeof(Y));
+  auto& K = sh->keep;
+  int r;
+  const int nfw = static_cast<int>(s.fx.size());
+  DevSys& S = Y.S;
+  S.imcon = s.imcon;
+  std::memcpy(S.cell, s.cell, sizeof(S.cell));
+  std::memcpy(S.rcell, s.rcell, sizeof(S.rcell));
+  S.inv[0] = 1.0 / s.cell[0];
+  S.inv[1] = 1.0 / s.cell[4];
+  S.inv[2] = 1.0 / s.cell[8];
+  S.rcut2 = s.rcut * s.rcut;
+  S.rdrewd = 1.0 / s.drewd;
+  S.rdlrpot = 1.0 / s.dlrpot;
+  S.mxgrid = s.mxgrid;
+  S.ntpatm = s.ntpatm;
+  S.epsq = s.epsq;
+  S.nfw = nfw;
+  std::vector<int> ft0(nfw);
+  for (int i = 0; i < nfw; ++i) ft0[i] = s.ftype[i] - 1;
+  double *fx, *fy, *fz, *fq, *erc, *vvv, *ak;
+  int *ftype, *pairtab, *kv;
+  if ((r = dput(K, &fx, s.fx.data(), nfw)) || (r = dput(K, &fy, s.fy.data(), nfw)) ||
+      (r = dput(K, &fz, s.fz.data(), nfw)) || (r = dput(K, &fq, s.fq.data(), nfw)) ||
+      (r = dput(K, &ftype, ft0.data(), nfw)))
+    return r;
+  if ((r = dput(K, &erc, s.erc.data(), s.erc.size())) ||
+      (r = dput(K, &vvv, s.vvv.data(), s.vvv.size())) ||
+      (r = dput(K, &pairtab, s.pairtab.data(), s.pairtab.size())))
+    return r;
+  std::vector<float> erc32(s.erc.begin(), s.erc.end()), vvv32(s.vvv.begin(), s.vvv.end());
+  float *e32, *v32;
+  if ((r = dput(K, &e32, erc32.data(), erc32.size())) ||
+      (r = dput(K, &v32, vvv32.data(), vvv32.size())))
+    return r;
+  S.fx = fx; S.fy = fy; S.fz = fz; S.fq = fq; S.ftype = ftype;
+  S.erc = erc; S.vvv = vvv; S.pairtab = pairtab;
+  Y.erc32 = e32; Y.vvv32 = v32;
+  const int nk = static_cast<int>(s.kvecs.size());
+  std::vector<int> kvh(nk);
+  std::vector<double> akh(nk);
+  for (int k = 0; k < nk; ++k) {
+    kvh[k] = (s.kvecs[k].l & 0xff) | ((s.kvecs[k].m & 0xff) << 8) |
+             ((s.kvecs[k].n & 0xff) << 16);
+    akh[k] = s.kvecs[k].ak;
+  }
+  if ((r = dput(K, &kv, kvh.data(), nk)) || (r = dput(K, &ak, akh.data(), nk))) return r;
+  Y.nk = nk;
+  Y.kv = kv;
+  Y.ak = ak;
+  for (int d = 0; d < 3; ++d) Y.kmax[d] = s.kmax[d];
+  if (s.kmax[0] >= kBTab || s.kmax[1] >= kBTab || s.kmax[2] >= kBTab)
+    return FMC_ERR_UNSUPPORTED;
+  Y.volm = s.volm;
+  Y.rvolm2 = 2.0 * (2.0 * kPi / s.volm);
+  // framework S(k)
+  double *sre, *sim, *rc;
+  if ((r = dput(K, &sre, static_cast<double*>(nullptr), nk)) ||
+      (r = dput(K, &sim, static_cast<double*>(nullptr), nk)) ||
+      (r = dput(K, &rc, s.rcell, 9)))
+    return r;
+  hipLaunchKernelGGL(sfac_kernel, dim3((nk + 255) / 256), dim3(256), 0, b->stream,
+                     nk, kv, rc, nfw, fx, fy, fz, fq, sre, sim);
+  BCHECK(hipGetLastError());
+  Y.sfw_re = sre;
+  Y.sfw_im = sim;
+  // candidate grid: cells of ~1.5 A along each perpendicular width
+  double p[10];
+  dcell(s.cell, p);
+  for (int d = 0; d < 3; ++d) Y.gdim[d] = std::max(1, static_cast<int>(p[6 + d] / 1.5));
+  while (static_cast<long long>(Y.gdim[0]) * Y.gdim[1] * Y.gdim[2] > 250000)
+    for (int d = 0; d < 3; ++d) Y.gdim[d] = std::max(1, Y.gdim[d] * 4 / 5);
+  const int ncell = Y.gdim[0] * Y.gdim[1] * Y.gdim[2];
+  double hd = 0.0;
+  for (int sa = -1; sa <= 1; sa += 2)
+    for (int sb = -1; sb <= 1; sb += 2) {
+      double v[3];
+      for (int d = 0; d < 3; ++d)
+        v[d] = (s.cell[d] / Y.gdim[0] + sa * s.cell[3 + d] / Y.gdim[1] +
+                sb * s.cell[6 + d] / Y.gdim[2]) * 0.5;
+      hd = std::max(hd, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    }
+  if (nfw >= 65536) return FMC_ERR_CAPACITY;
+  GridParams G;
+  G.S = S;
+  for (int d = 0; d < 3; ++d) G.gdim[d] = Y.gdim[d];
+  const double reach = s.rcut + hd + 1e-6;
+  G.reach2 = reach * reach;
+  int *cnt, *off;
+  if ((r = dput(K, &cnt, static_cast<int*>(nullptr), ncell)) ||
+      (r = dput(K, &off, static_cast<int*>(nullptr), ncell + 1)))
+    return r;
+  G.count = cnt;
+  hipLaunchKernelGGL(grid_count_kernel, dim3(ncell), dim3(256), 0, b->stream, G);
+  BCHECK(hipGetLastError());
+  std::vector<int> hc(ncell), ho(ncell + 1, 0);
+  BCHECK(hipMemcpyAsync(hc.data(), cnt, ncell * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  BCHECK(hipStreamSynchronize(b->stream));
+  for (int c = 0; c < ncell; ++c) ho[c + 1] = ho[c] + hc[c];
+  BCHECK(hipMemcpy(off, ho.data(), (ncell + 1) * sizeof(int), hipMemcpyHostToDevice));
+  unsigned short* idx;
+  if ((r = dput(K, &idx, static_cast<unsigned short*>(nullptr), ho[ncell]))) return r;
+  G.off = off;
+  G.idx = idx;
+  hipLaunchKernelGGL(grid_fill_kernel, dim3(ncell), dim3(256), 0, b->stream, G);
+  BCHECK(hipGetLastError());
+  BCHECK(hipStreamSynchronize(b->stream));
+  Y.goff = off;
+  Y.gidx = idx;
+  // guests
+  Y.ngt = static_cast<int>(s.guests.size());
+  for (int g = 0; g < Y.ngt; ++g) {
+    const GuestTemplate& t = s.guests[g];
+    Y.ns[g] = t.ns;
+    for (int i = 0; i < t.ns; ++i) {
+      Y.gq[g][i] = t.q[i];
+      Y.gtype[g][i] = t.type[i] - 1;
+      Y.gmass[g][i] = t.mass[i];
+      Y.tx[g][i] = t.x[i];
+      Y.ty[g][i] = t.y[i];
+      Y.tz[g][i] = t.z[i];
+    }
+    Y.e3[g] = t.e3;
+    Y.engsic[g] = t.engsic;
+  }
+  // tail as a quadratic form in the molecule counts
+  {
+    const int T = s.ntpatm;
+    std::vector<double> F(T, 0.0), Z(T, 0.0);
+    std::vector<std::vector<double>> cg(Y.ngt, std::vector<double>(T, 0.0));
+    for (int i = 0; i < nfw; ++i) {
+      F[s.ftype[i] - 1] += 1.0;
+      if (s.ffrz[i]) Z[s.ftype[i] - 1] += 1.0;
+    }
+    for (int g = 0; g < Y.ngt; ++g)
+      for (int i = 0; i < s.guests[g].ns; ++i) cg[g][s.guests[g].type[i] - 1] += 1.0;
+    double t0 = 0.0, tb[kMaxGuestTypes] = {0}, tA[kMaxGuestTypes][kMaxGuestTypes] = {{0}};
+    const bool on = !(s.imcon == 0 || s.imcon == 6 || s.ntpvdw == 0);
+    int iv = 0;
+    for (int i = 0; i < T; ++i)
+      for (int j = 0; j <= i; ++j, ++iv) {
+        if (!on) continue;
+        const double w = 2.0 * kPi * s.steadd[iv] / s.volm;
+        t0 += w * (F[i] * F[j] - Z[i] * Z[j]);
+        for (int g = 0; g < Y.ngt; ++g) {
+          tb[g] += w * (F[i] * cg[g][j] + F[j] * cg[g][i]);
+          for (int h = 0; h < Y.ngt; ++h) tA[g][h] += w * cg[g][i] * cg[h][j];
+        }
+      }
+    Y.t0 = t0;
+    for (int g = 0; g < Y.ngt; ++g) {
+      Y.tb[g] = tb[g];
+      for (int h = 0; h < Y.ngt; ++h) Y.tA[g][h] = tA[g][h];
+    }
+  }
+  for (auto& kb : K) sh->bytes += kb->n;
+  b->max_nk = std::max(b->max_nk, nk);
+  b->max_nlm = std::max(b->max_nlm, (s.kmax[0] + 1) * (2 * s.kmax[1] + 1));
+  for (const auto& g : s.guests) b->max_pos = std::max(b->max_pos, 2 * g.ns);
+  b->systems.push_back(std::move(sh));
+  b->dirty_sys = true;
+  *out = static_cast<int>(b->systems.size()) - 1;
+  return FMC_OK;
+}
+
+int sync_tables(fmc_batch* b) {
+  if (b->dirty_sys) {
+    const int n = static_cast<int>(b->systems.size());
+    if (n > b->d_sys_cap) {
+      if (b->d_sys) BCHECK(hipFree(b->d_sys));
+      b->d_sys_cap = std::max(16, 2 * n);
+      BCHECK(hipMalloc(&b->d_sys, b->d_sys_cap * sizeof(BSys)));
+    }
+    std::vector<BSys> h(n);
+    for (int i = 0; i < n; ++i) h[i] = b->systems[i]->dev;
+    BCHECK(hipMemcpy(b->d_sys, h.data(), n * sizeof(BSys), hipMemcpyHostToDevice));
+    b->dirty_sys = false;
+  }
+  b->active_ids.clear();
+  for (int i = 0; i < b->max_chains; ++i)
+    if (b->chains[i] && b->chains[i]->active) b->active_ids.push_back(i);
+  if (!b->active_ids.empty())
+    BCHECK(hipMemcpy(b->d_ids, b->active_ids.data(), b->active_ids.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
+int launch(fmc_batch* b, long long n, int bt, int kpt) {
+  const KCfg* cfg = find_cfg(bt, kpt, b->precision == FMC_PREC_MIXED);
+  if (!cfg) return FMC_ERR_UNSUPPORTED;
+  if (b->active_ids.empty()) return FMC_OK;
+  const size_t elt = b->precision == FMC_PREC_MIXED ? 8 : 16;
+  const int nkcap = b->max_nk;
+  const size_t lds = static_cast<size_t>(nkcap) * sizeof(double) +
+                     static_cast<size_t>((nkcap + 3) & ~3) * sizeof(int) +
+                     static_cast<size_t>(b->max_pos) * b->max_nlm * elt;
+  int dev = 0, lmax = 0;
+  BCHECK(hipGetDevice(&dev));
+  BCHECK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (lds + 32768 > static_cast<size_t>(lmax)) return FMC_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL(cfg->fn, dim3(static_cast<unsigned>(b->active_ids.size())),
+                     dim3(cfg->bt), lds, b->stream, b->d_sys, b->d_chains, b->d_ids, n,
+                     nkcap);
+  BCHECK(hipGetLastError());
+  return FMC_OK;
+}
+
+// snapshot / restore of every active chain's device state (autotuning runs
+// real moves and must leave the chains untouched)
+struct Snapshot {
+  std::vector<BChain> chains;
+  std::vector<std::vector<double>> data;
+};
+
+int snapshot(fmc_batch* b, Snapshot* s, bool restore) {
+  const int nmax = b->max_chains;
+  if (!restore) {
+    s->chains.resize(nmax);
+    BCHECK(hipMemcpy(s->chains.data(), b->d_chains, nmax * sizeof(BChain),
+                     hipMemcpyDeviceToHost));
+    s->data.clear();
+  } else {
+    BCHECK(hipMemcpy(b->d_chains, s->chains.data(), nmax * sizeof(BChain),
+                     hipMemcpyHostToDevice));
+  }
+  size_t slot = 0;
+  for (int id : b->active_ids) {
+    const BChain& C = s->chains[id];
+    const BSys& Y = b->systems[C.sys]->dev;
+    std::vector<std::pair<double*, size_t>> bufs = {{C.sre, (size_t)Y.nk},
+                                                    {C.sim, (size_t)Y.nk}};
+    for (int g = 0; g < Y.ngt; ++g) {
+      bufs.push_back({C.gx[g], (size_t)C.cap_atoms});
+      bufs.push_back({C.gy[g], (size_t)C.cap_atoms});
+      bufs.push_back({C.gz[g], (size_t)C.cap_atoms});
+    }
+    for (auto& pr : bufs) {
+      if (!restore) {
+        s->data.emplace_back(pr.second);
+        BCHECK(hipMemcpy(s->data.back().data(), pr.first, pr.second * sizeof(double),
+                         hipMemcpyDeviceToHost));
+      } else {
+        BCHECK(hipMemcpy(pr.first, s->data[slot].data(), pr.second * sizeof(double),
+                         hipMemcpyHostToDevice));
+      }
+      ++slot;
+    }
+  }
+  return FMC_OK;
+}
+
+// Project autotuner: time each block size (with the smallest register tile
+// that covers the largest Nk) on real moves, keep the fastest, restore state.
+int autotune(fmc_batch* b) {
+  std::vector<int> bts = {256, 512, 1024};
+  Snapshot snap;
+  int r;
+  if ((r = snapshot(b, &snap, false))) return r;
+  double best = 1e300;
+  int bbt = 0, bkpt = 0;
+  for (int bt : bts) {
+    const int kpt = kpt_for(bt, b->max_nk);
+    if (!kpt) continue;
+    // warm-up + timed run (a tile whose LDS does not fit is skipped)
+    r = launch(b, 4, bt, kpt);
+    if (r == FMC_ERR_UNSUPPORTED) continue;
+    if (r) return r;
+    BCHECK(hipStreamSynchronize(b->stream));
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((r = launch(b, 24, bt, kpt))) return r;
+    BCHECK(hipStreamSynchronize(b->stream));
+    const double dt =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (dt < best) {
+      best = dt;
+      bbt = bt;
+      bkpt = kpt;
+    }
+    if ((r = snapshot(b, &snap, true))) return r;
+  }
+  if (!bbt) return FMC_ERR_UNSUPPORTED;
+  b->bt = bbt;
+  b->kpt = bkpt;
+  return FMC_OK;
+}
+
+}  // namespace
+
+extern "C" int fmc_batch_create(fmc_batch** out, int max_chains, int precision) {
+  if (!out || max_chains <= 0 || (precision != FMC_PREC_FP64 && precision != FMC_PREC_MIXED))
+    return FMC_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FMC_ERR_NODEVICE;
+  auto b = std::make_unique<fmc_batch>();
+  b->max_chains = max_chains;
+  b->precision = precision;
+  b->chains.resize(max_chains);
+  b->h_chains.resize(max_chains);
+  std::memset(b->h_chains.data(), 0, max_chains * sizeof(BChain));
+  BCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  BCHECK(hipMalloc(&b->d_chains, max_chains * sizeof(BChain)));
+  BCHECK(hipMemset(b->d_chains, 0, max_chains * sizeof(BChain)));
+  BCHECK(hipMalloc(&b->d_ids, max_chains * sizeof(int)));
+  BCHECK(hipMalloc(&b->d_dev, max_chains * sizeof(double)));
+  *out = b.release();
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_destroy(fmc_batch* b) {
+  if (!b) return FMC_ERR_ARG;
+  (void)hipStreamSynchronize(b->stream);
+  b->chains.clear();
+  b->systems.clear();
+  if (b->d_sys) (void)hipFree(b->d_sys);
+  if (b->d_chains) (void)hipFree(b->d_chains);
+  if (b->d_ids) (void)hipFree(b->d_ids);
+  if (b->d_dev) (void)hipFree(b->d_dev);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return FMC_OK;
+}
+
+extern "C" int fmc_chain_footprint(const fmc_ctx* ctx, int precision, long long* bytes) {
+  if (!ctx || !bytes || !ctx->sys.finalized) return FMC_ERR_ARG;
+  (void)precision;
+  const SystemDesc& s = ctx->sys;
+  const long long nk = static_cast<long long>(s.kvecs.size());
+  long long b = 2 * nk * 8;                                         // S(k)
+  b += static_cast<long long>(s.guests.size()) * 3 * s.max_guest_atoms * 8;  // guests
+  b += sizeof(BChain);
+  // system share: framework, tables, k-vectors, grid (counted per chain as
+  // an upper bound when the framework is not shared)
+  b += static_cast<long long>(s.fx.size()) * (4 * 8 + 4);
+  b += static_cast<long long>(s.vvv.size() + s.erc.size()) * 12;
+  b += nk * (4 + 8 + 16);
+  b += static_cast<long long>(s.fx.size()) * 2 * 60;   // candidate lists (~60 cells/atom)
+  *bytes = b;
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
+                                   const fmc_chain_params* p, int* chain_id) {
+  if (!b || !ctx || !p || !ctx->sys.finalized) return FMC_ERR_ARG;
+  if (p->ntpguest != static_cast<int>(ctx->sys.guests.size()) || p->ntpguest < 1 ||
+      p->ntpguest > kMaxGuestTypes)
+    return FMC_ERR_ARG;
+  for (int g = 0; g < p->ntpguest; ++g)
+    if (p->freq[g][3] > 0.0 || p->freq[g][8] > 0.0) return FMC_ERR_UNSUPPORTED;
+  int slot = -1;
+  for (int i = 0; i < b->max_chains; ++i)
+    if (!b->chains[i] || !b->chains[i]->active) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) return FMC_ERR_CAPACITY;
+  int r, sys;
+  if ((r = upload_system(b, ctx, &sys))) return r;
+  auto ch = std::make_unique<ChainHost>();
+  ch->sys = sys;
+  BChain C;
+  std::memset(&C, 0, sizeof(C));
+  C.sys = sys;
+  C.ntpguest = p->ntpguest;
+  C.cap_atoms = ctx->cap_atoms;
+  const int nk = ctx->nk;
+  auto& K = ch->keep;
+  if ((r = dput(K, &C.sre, static_cast<double*>(nullptr), nk)) ||
+      (r = dput(K, &C.sim, static_cast<double*>(nullptr), nk)))
+    return r;
+  BCHECK(hipMemcpy(C.sre, ctx->d_sre[ctx->cur], nk * sizeof(double), hipMemcpyDeviceToDevice));
+  BCHECK(hipMemcpy(C.sim, ctx->d_sim[ctx->cur], nk * sizeof(double), hipMemcpyDeviceToDevice));
+  for (int g = 0; g < p->ntpguest; ++g) {
+    if ((r = dput(K, &C.gx[g], ctx->hx[g].data(), C.cap_atoms)) ||
+        (r = dput(K, &C.gy[g], ctx->hy[g].data(), C.cap_atoms)) ||
+        (r = dput(K, &C.gz[g], ctx->hz[g].data(), C.cap_atoms)))
+      return r;
+    C.nmol[g] = ctx->nmol[g];
+  }
+  C.temp = p->temp;
+  C.beta = 1.0 / (kKboltz * p->temp);
+  C.statvolm = ctx->sys.volm * 1e-30;
+  C.engunit = p->engunit > 0 ? p->engunit : 418.4;
+  C.eq_steps = p->eq_steps;
+  for (int g = 0; g < p->ntpguest; ++g) {
+    C.press[g] = p->press[g];
+    double f[9], norm = 0.0;
+    for (int i = 0; i < 9; ++i) {
+      f[i] = p->freq[g][i];
+      norm += f[i];
+    }
+    if (norm == 0.0) {
+      for (int i = 0; i < 9; ++i) f[i] = 0.0;
+      f[0] = f[1] = f[2] = 1.0 / 3.0;
+      norm = 1.0;
+    }
+    C.cum[g][0] = f[0] / norm;
+    for (int i = 1; i < 9; ++i) C.cum[g][i] = C.cum[g][i - 1] + (f[i] / norm);
+    C.disp_ratio[g] = p->disp_ratio[g] > 0 ? p->disp_ratio[g] : 0.5;
+    C.tran_ratio[g] = p->tran_ratio[g] > 0 ? p->tran_ratio[g] : 0.5;
+    C.delrdisp[g] = p->delr[g];
+    C.tran_delr[g] = p->delr[g];
+  }
+  C.rota_ratio = p->rota_ratio > 0 ? p->rota_ratio : 0.5;
+  C.rotangle = kPi / 3.0;
+  C.rota_rotangle = C.rotangle;
+  C.elrc = 0.0;
+  duni_setup(p->seed[0], p->seed[1], p->seed[2], p->seed[3], &C);
+  for (auto& kb : K) ch->bytes += kb->n;
+  ch->active = true;
+  b->chains[slot] = std::move(ch);
+  b->h_chains[slot] = C;
+  BCHECK(hipMemcpy(b->d_chains + slot, &C, sizeof(BChain), hipMemcpyHostToDevice));
+  if (chain_id) *chain_id = slot;
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_remove_chain(fmc_batch* b, int id) {
+  if (!b || id < 0 || id >= b->max_chains || !b->chains[id] || !b->chains[id]->active)
+    return FMC_ERR_ARG;
+  BCHECK(hipStreamSynchronize(b->stream));
+  b->chains[id].reset();
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_set_tile(fmc_batch* b, int threads) {
+  if (!b) return FMC_ERR_ARG;
+  if (threads == 0) {
+    b->bt = b->kpt = 0;
+    return FMC_OK;
+  }
+  const int kpt = kpt_for(threads, std::max(1, b->max_nk));
+  if (!kpt) return FMC_ERR_UNSUPPORTED;
+  b->bt = threads;
+  b->kpt = kpt;
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_get_tile(const fmc_batch* b, int* threads) {
+  if (!b || !threads) return FMC_ERR_ARG;
+  *threads = b->bt;
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_launch(fmc_batch* b, long long n) {
+  if (!b || n < 0) return FMC_ERR_ARG;
+  int r;
+  if ((r = sync_tables(b))) return r;
+  if (b->active_ids.empty()) return FMC_OK;
+  if (b->bt == 0 || b->kpt == 0 || b->bt * b->kpt < b->max_nk) {
+    if ((r = autotune(b))) return r;
+  }
+  return launch(b, n, b->bt, b->kpt);
+}
+
+extern "C" int fmc_batch_sync(fmc_batch* b) {
+  if (!b) return FMC_ERR_ARG;
+  BCHECK(hipStreamSynchronize(b->stream));
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_run(fmc_batch* b, long long n) {
+  int r = fmc_batch_launch(b, n);
+  if (r) return r;
+  return fmc_batch_sync(b);
+}
+
+extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
+  if (!b || !st || id < 0 || id >= b->max_chains || !b->chains[id]) return FMC_ERR_ARG;
+  BChain C;
+  BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
+  std::memset(st, 0, sizeof(*st));
+  st->steps = C.steps;
+  st->prod_steps = C.prod_steps;
+  for (int i = 0; i < 6; ++i) {
+    st->accepted[i] = C.accepted[i];
+    st->attempted[i] = C.attempted[i];
+  }
+  for (int g = 0; g < kMaxGuestTypes; ++g) {
+    st->nmol[g] = C.nmol[g];
+    st->energy[g] = C.energy[g];
+    st->sum_n[g] = C.sum_n[g];
+    st->sum_n2[g] = C.sum_n2[g];
+    st->sum_e[g] = C.sum_e[g];
+    st->delrdisp[g] = C.delrdisp[g];
+    st->tran_delr[g] = C.tran_delr[g];
+  }
+  st->rotangle = C.rotangle;
+  st->rota_rotangle = C.rota_rotangle;
+  return C.fault ? FMC_ERR_CAPACITY : FMC_OK;
+}
+
+extern "C" int fmc_batch_enable_log(fmc_batch* b, int id, long long cap) {
+  if (!b || id < 0 || id >= b->max_chains || !b->chains[id] || cap < 0) return FMC_ERR_ARG;
+  BChain C;
+  BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
+  double* lg = nullptr;
+  int r;
+  if ((r = dput(b->chains[id]->keep, &lg, static_cast<double*>(nullptr),
+                static_cast<size_t>(cap) * kLogFields)))
+    return r;
+  C.log = lg;
+  C.log_cap = cap;
+  C.log_n = 0;
+  BCHECK(hipMemcpy(b->d_chains + id, &C, sizeof(BChain), hipMemcpyHostToDevice));
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_get_log(fmc_batch* b, int id, long long n, double* rec) {
+  if (!b || !rec || id < 0 || id >= b->max_chains || !b->chains[id]) return FMC_ERR_ARG;
+  BChain C;
+  BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
+  if (!C.log || n > C.log_n) return FMC_ERR_ARG;
+  BCHECK(hipMemcpy(rec, C.log, static_cast<size_t>(n) * kLogFields * sizeof(double),
+                   hipMemcpyDeviceToHost));
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_resync(fmc_batch* b, double* max_dev) {
+  if (!b) return FMC_ERR_ARG;
+  int r;
+  if ((r = sync_tables(b))) return r;
+  const int n = static_cast<int>(b->active_ids.size());
+  if (max_dev) *max_dev = 0.0;
+  if (!n) return FMC_OK;
+  ResyncParams P;
+  P.sys = b->d_sys;
+  P.chains = b->d_chains;
+  P.ids = b->d_ids;
+  P.maxdev = b->d_dev;
+  hipLaunchKernelGGL(resync_kernel, dim3(n), dim3(256), 0, b->stream, P);
+  BCHECK(hipGetLastError());
+  std::vector<double> dev(n);
+  BCHECK(hipMemcpyAsync(dev.data(), b->d_dev, n * sizeof(double), hipMemcpyDeviceToHost,
+                        b->stream));
+  BCHECK(hipStreamSynchronize(b->stream));
+  if (max_dev) *max_dev = *std::max_element(dev.begin(), dev.end());
+  return FMC_OK;
 }

@@ -14,6 +14,7 @@ struct DevSys {
   int imcon;
   double cell[9];
   double rcell[9];
+  double inv[3];         // 1/cell(1), 1/cell(5), 1/cell(9) (images, imcon 1-2)
   double rcut2;
   double rdrewd, rdlrpot;
   int mxgrid;
@@ -43,15 +44,13 @@ __device__ __forceinline__ void min_image(const DevSys& S, double& x,
     y = S.cell[1] * ax + S.cell[4] * ay + S.cell[7] * az;
     z = S.cell[2] * ax + S.cell[5] * ay + S.cell[8] * az;
   } else if (S.imcon == 2) {
-    const double a = 1.0 / S.cell[0], b = 1.0 / S.cell[4], c = 1.0 / S.cell[8];
-    x -= S.cell[0] * round(a * x);
-    y -= S.cell[4] * round(b * y);
-    z -= S.cell[8] * round(c * z);
+    x -= S.cell[0] * round(S.inv[0] * x);
+    y -= S.cell[4] * round(S.inv[1] * y);
+    z -= S.cell[8] * round(S.inv[2] * z);
   } else {
-    const double a = 1.0 / S.cell[0];
-    x -= S.cell[0] * round(a * x);
-    y -= S.cell[0] * round(a * y);
-    z -= S.cell[0] * round(a * z);
+    x -= S.cell[0] * round(S.inv[0] * x);
+    y -= S.cell[0] * round(S.inv[0] * y);
+    z -= S.cell[0] * round(S.inv[0] * z);
   }
 }
 

@@ -382,6 +382,9 @@ void fill_devsys(fmc_ctx* c, DevSys* S) {
   S->imcon = s.imcon;
   std::memcpy(S->cell, s.cell, sizeof(S->cell));
   std::memcpy(S->rcell, s.rcell, sizeof(S->rcell));
+  S->inv[0] = 1.0 / s.cell[0];
+  S->inv[1] = 1.0 / s.cell[4];
+  S->inv[2] = 1.0 / s.cell[8];
   S->rcut2 = s.rcut * s.rcut;
   S->rdrewd = 1.0 / s.drewd;
   S->rdlrpot = 1.0 / s.dlrpot;

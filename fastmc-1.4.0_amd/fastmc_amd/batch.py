@@ -1,0 +1,116 @@
+"""Batched device-resident chains (fmc_batch_* C ABI)."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .capi import (ChainParams, ChainStats, DP, FmcError, PREC_FP64, check,
+                   lib)
+
+
+class Batch:
+    """Many independent GCMC chains advanced together on the GPU, one chain
+    per work-group of a persistent kernel."""
+
+    def __init__(self, max_chains: int, precision: int = PREC_FP64):
+        self.L = lib()
+        h = C.c_void_p()
+        check(self.L.fmc_batch_create(C.byref(h), max_chains, precision),
+              "fmc_batch_create")
+        self.h = h
+        self.precision = precision
+        self.ns = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fmc_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_chain(self, ctx, temp: float, press_pa: Sequence[float],
+                  seed: Sequence[int], freq: Optional[Sequence[Sequence[float]]] = None,
+                  delr: float = 1.0, eq_steps: int = 0, engunit: float = 418.4,
+                  ratios=(0.5, 0.5, 0.5)) -> int:
+        p = ChainParams()
+        p.temp = temp
+        p.ntpguest = len(press_pa)
+        for k in range(4):
+            p.seed[k] = int(seed[k])
+        for g, pr in enumerate(press_pa):
+            p.press[g] = pr
+            p.delr[g] = delr
+            p.disp_ratio[g] = ratios[0]
+            p.tran_ratio[g] = ratios[1]
+            if freq is not None:
+                for k in range(9):
+                    p.freq[g][k] = freq[g][k] if k < len(freq[g]) else 0.0
+        p.rota_ratio = ratios[2]
+        p.engunit = engunit
+        p.eq_steps = eq_steps
+        cid = C.c_int()
+        check(self.L.fmc_batch_add_chain(self.h, ctx.h, C.byref(p), C.byref(cid)),
+              "fmc_batch_add_chain")
+        self.ns[cid.value] = list(ctx.nsites)
+        return cid.value
+
+    def remove_chain(self, cid: int):
+        check(self.L.fmc_batch_remove_chain(self.h, cid), "fmc_batch_remove_chain")
+        self.ns.pop(cid, None)
+
+    def run(self, n: int):
+        check(self.L.fmc_batch_run(self.h, n), "fmc_batch_run")
+
+    def launch(self, n: int):
+        check(self.L.fmc_batch_launch(self.h, n), "fmc_batch_launch")
+
+    def sync(self):
+        check(self.L.fmc_batch_sync(self.h), "fmc_batch_sync")
+
+    def stats(self, cid: int) -> dict:
+        st = ChainStats()
+        r = self.L.fmc_batch_stats(self.h, cid, C.byref(st))
+        if r not in (0, -4):
+            raise FmcError(r, "fmc_batch_stats")
+        ng = len(self.ns.get(cid, [1]))
+        return dict(steps=st.steps, prod_steps=st.prod_steps,
+                    accepted=list(st.accepted), attempted=list(st.attempted),
+                    nmol=list(st.nmol)[:ng], energy=list(st.energy)[:ng],
+                    sum_n=list(st.sum_n)[:ng], sum_n2=list(st.sum_n2)[:ng],
+                    sum_e=list(st.sum_e)[:ng], delrdisp=list(st.delrdisp)[:ng],
+                    rotangle=st.rotangle, capacity_fault=(r == -4))
+
+    def enable_log(self, cid: int, cap: int):
+        check(self.L.fmc_batch_enable_log(self.h, cid, cap), "fmc_batch_enable_log")
+
+    def get_log(self, cid: int, n: int) -> np.ndarray:
+        out = np.zeros(8 * n)
+        check(self.L.fmc_batch_get_log(self.h, cid, n, out.ctypes.data_as(DP)),
+              "fmc_batch_get_log")
+        return out.reshape(n, 8)
+
+    def resync(self) -> float:
+        v = C.c_double()
+        check(self.L.fmc_batch_resync(self.h, C.byref(v)), "fmc_batch_resync")
+        return v.value
+
+    def set_tile(self, threads: int):
+        check(self.L.fmc_batch_set_tile(self.h, threads), "fmc_batch_set_tile")
+
+    def tile(self) -> int:
+        v = C.c_int()
+        check(self.L.fmc_batch_get_tile(self.h, C.byref(v)), "fmc_batch_get_tile")
+        return v.value
+
+
+def footprint(ctx, precision: int = PREC_FP64) -> int:
+    v = C.c_longlong()
+    check(lib().fmc_chain_footprint(ctx.h, precision, C.byref(v)),
+          "fmc_chain_footprint")
+    return v.value

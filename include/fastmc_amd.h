@@ -189,9 +189,10 @@ int fmc_batch_run(fmc_batch *b, long long n_moves);
 int fmc_batch_launch(fmc_batch *b, long long n_moves);
 int fmc_batch_sync(fmc_batch *b);
 int fmc_batch_stats(fmc_batch *b, int chain_id, fmc_chain_stats *st);
-/* per-move log of the last run for one chain (FP64 parity checks):
- * rec[5*i..5*i+4] = {kind, iguest, accepted, nmol_after, estep}; enable
- * before running with capacity moves */
+/* per-move log for one chain (parity checks): rec[8*i..8*i+7] = {kind,
+ * iguest, accepted, nmol_after, estep (kcal/mol), ewld1eng incl. self term,
+ * ewld2sum, vdwsum (internal units)}; enable before running, with capacity
+ * moves */
 int fmc_batch_enable_log(fmc_batch *b, int chain_id, long long capacity);
 int fmc_batch_get_log(fmc_batch *b, int chain_id, long long n, double *rec);
 /* FP64 full recompute of every chain's S(k) (periodic resync); returns the
