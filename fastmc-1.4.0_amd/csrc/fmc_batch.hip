@@ -532,7 +532,7 @@ __global__ void __launch_bounds__(BT)
     batch_kernel(const BSys* __restrict__ systems, BChain* __restrict__ chains,
                  const int* __restrict__ ids, long long nmoves, int nkcap) {
   using T = typename std::conditional<MIXED, float, double>::type;
-  __shared__ Cplx<T> tab[2 * kMaxSites][3][kBTab];   // per position p
+  __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
   __shared__ MoveDesc D;
   __shared__ float rng_u[97];
   __shared__ double red[3][BT / kWave];
@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(BT)
           ph = ph - rint(ph);
           double sv, cv;
           sincospi(2.0 * ph, &sv, &cv);
-          tab[p][d][l] = {static_cast<T>(cv), static_cast<T>(sv)};
+          tab[p][d][l] = {cv, sv};
         }
         if (tid >= BT - kWave && tid < BT - kWave + np) {
           const int p = tid - (BT - kWave);
@@ -636,17 +636,17 @@ __global__ void __launch_bounds__(BT)
         for (int t = tid; t < np * nlm; t += BT) {
           const int p = t / nlm, r = t - p * nlm;
           const int l = r / w2, m = r - l * w2 - Y.kmax[1];
-          const Cplx<T> a = tab[p][0][l];
-          const Cplx<T> b = tab[p][1][m < 0 ? -m : m];
-          Cplx<T> o;
+          const Cplx<double> a = tab[p][0][l];
+          const Cplx<double> b = tab[p][1][m < 0 ? -m : m];
+          double ox, oy;
           if (m >= 0) {
-            o.x = a.x * b.x - a.y * b.y;
-            o.y = a.y * b.x + b.y * a.x;
+            ox = a.x * b.x - a.y * b.y;
+            oy = a.y * b.x + b.y * a.x;
           } else {
-            o.x = a.x * b.x + a.y * b.y;
-            o.y = a.y * b.x - b.y * a.x;
+            ox = a.x * b.x + a.y * b.y;
+            oy = a.y * b.x - b.y * a.x;
           }
-          xy_l[t] = o;
+          xy_l[t] = {static_cast<T>(ox), static_cast<T>(oy)};
         }
       }
       __syncthreads();
@@ -660,18 +660,20 @@ __global__ void __launch_bounds__(BT)
           const int lm = v & 0xffff;
           const int n = static_cast<signed char>((v >> 16) & 0xff);
           const int an = n < 0 ? -n : n;
-          T dr = 0, di = 0;
+          // per-position terms in T, accumulated in FP64
+          double dr = 0, di = 0;
           for (int p = 0; p < np; ++p) {
             const int set = D.pset[p];
             const T q = static_cast<T>(set ? Y.gq[ig][D.psite[p]] : -Y.gq[ig][D.psite[p]]);
             const Cplx<T> a = xy_l[p * nlm + lm];
-            const Cplx<T> c = tab[p][2][an];
+            const Cplx<double> c64 = tab[p][2][an];
+            const Cplx<T> c = {static_cast<T>(c64.x), static_cast<T>(c64.y)};
             if (n >= 0) {
-              dr += q * (a.x * c.x - a.y * c.y);
-              di += q * (a.y * c.x + a.x * c.y);
+              dr += static_cast<double>(q * (a.x * c.x - a.y * c.y));
+              di += static_cast<double>(q * (a.y * c.x + a.x * c.y));
             } else {
-              dr += q * (a.x * c.x + a.y * c.y);
-              di += q * (a.y * c.x - a.x * c.y);
+              dr += static_cast<double>(q * (a.x * c.x + a.y * c.y));
+              di += static_cast<double>(q * (a.y * c.x - a.x * c.y));
             }
           }
           // |S + dS|^2 - |S|^2 = 2 S.dS + |dS|^2: no cancellation of |S|^2
@@ -698,7 +700,7 @@ __global__ void __launch_bounds__(BT)
         const int ng = D.nguest;
         const int total = nfwp + np * ng;
         const int own0 = (kind != FMC_INSERT) ? D.gpre[ig] + D.imol * ns : -1;
-        T a2 = 0, av = 0;
+        double a2 = 0, av = 0;
         for (int idx = tid; idx < total; idx += BT) {
           int p;
           double ax, ay, az, aq;
@@ -736,11 +738,16 @@ __global__ void __launch_bounds__(BT)
             pair_fp64(Y.S, ce, Y.gtype[ig][s], px - ax, py - ay, pz - az, aq, at,
                       b2, bv);
           }
-          if (kind == FMC_DISPLACE && set == 0) { a2 -= b2; av -= bv; }
-          else { a2 += b2; av += bv; }
+          if (kind == FMC_DISPLACE && set == 0) {
+            a2 -= static_cast<double>(b2);
+            av -= static_cast<double>(bv);
+          } else {
+            a2 += static_cast<double>(b2);
+            av += static_cast<double>(bv);
+          }
         }
-        e2 = static_cast<double>(a2);
-        ev = static_cast<double>(av);
+        e2 = a2;
+        ev = av;
       }
     }
     // block reductions (fixed tree)
@@ -1110,6 +1117,10 @@ struct fmc_batch {
   std::vector<BChain> h_chains;
   bool dirty_sys = true;
   hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timing_pending = false;
+  double kernel_ms = 0.0;
+  long long launches = 0;
   int max_nk = 0;
   int max_nlm = 0;   // (kmax1+1)(2 kmax2+1) over the systems
   int max_pos = 0;   // trial positions: 2 x the largest guest
@@ -1461,6 +1472,8 @@ extern "C" int fmc_batch_create(fmc_batch** out, int max_chains, int precision) 
   b->h_chains.resize(max_chains);
   std::memset(b->h_chains.data(), 0, max_chains * sizeof(BChain));
   BCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  BCHECK(hipEventCreate(&b->ev0));
+  BCHECK(hipEventCreate(&b->ev1));
   BCHECK(hipMalloc(&b->d_chains, max_chains * sizeof(BChain)));
   BCHECK(hipMemset(b->d_chains, 0, max_chains * sizeof(BChain)));
   BCHECK(hipMalloc(&b->d_ids, max_chains * sizeof(int)));
@@ -1478,6 +1491,8 @@ extern "C" int fmc_batch_destroy(fmc_batch* b) {
   if (b->d_chains) (void)hipFree(b->d_chains);
   if (b->d_ids) (void)hipFree(b->d_ids);
   if (b->d_dev) (void)hipFree(b->d_dev);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return FMC_OK;
@@ -1607,17 +1622,42 @@ extern "C" int fmc_batch_get_tile(const fmc_batch* b, int* threads) {
 extern "C" int fmc_batch_launch(fmc_batch* b, long long n) {
   if (!b || n < 0) return FMC_ERR_ARG;
   int r;
+  if ((r = fmc_batch_sync(b))) return r;
   if ((r = sync_tables(b))) return r;
   if (b->active_ids.empty()) return FMC_OK;
   if (b->bt == 0 || b->kpt == 0 || b->bt * b->kpt < b->max_nk) {
     if ((r = autotune(b))) return r;
   }
-  return launch(b, n, b->bt, b->kpt);
+  // kernel time on the batch's own stream
+  BCHECK(hipEventRecord(b->ev0, b->stream));
+  if ((r = launch(b, n, b->bt, b->kpt))) return r;
+  BCHECK(hipEventRecord(b->ev1, b->stream));
+  b->timing_pending = true;
+  return FMC_OK;
 }
 
 extern "C" int fmc_batch_sync(fmc_batch* b) {
   if (!b) return FMC_ERR_ARG;
   BCHECK(hipStreamSynchronize(b->stream));
+  if (b->timing_pending) {
+    float ms = 0.0f;
+    BCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->kernel_ms += ms;
+    b->launches++;
+    b->timing_pending = false;
+  }
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_timing(fmc_batch* b, int reset, double* ms,
+                                long long* launches) {
+  if (!b) return FMC_ERR_ARG;
+  if (ms) *ms = b->kernel_ms;
+  if (launches) *launches = b->launches;
+  if (reset) {
+    b->kernel_ms = 0.0;
+    b->launches = 0;
+  }
   return FMC_OK;
 }
 

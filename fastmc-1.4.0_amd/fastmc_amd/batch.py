@@ -100,6 +100,15 @@ class Batch:
         check(self.L.fmc_batch_resync(self.h, C.byref(v)), "fmc_batch_resync")
         return v.value
 
+    def timing(self, reset: bool = False):
+        """(kernel milliseconds, launches) since the last reset, from HIP
+        events on the batch stream."""
+        ms = C.c_double()
+        n = C.c_longlong()
+        check(self.L.fmc_batch_timing(self.h, int(reset), C.byref(ms), C.byref(n)),
+              "fmc_batch_timing")
+        return ms.value, n.value
+
     def set_tile(self, threads: int):
         check(self.L.fmc_batch_set_tile(self.h, threads), "fmc_batch_set_tile")
 

@@ -85,6 +85,7 @@ _SIGS = {
     "fmc_batch_enable_log": [C.c_void_p, C.c_int, C.c_longlong],
     "fmc_batch_get_log": [C.c_void_p, C.c_int, C.c_longlong, DP],
     "fmc_batch_resync": [C.c_void_p, DP],
+    "fmc_batch_timing": [C.c_void_p, C.c_int, DP, C.POINTER(C.c_longlong)],
     "fmc_batch_set_tile": [C.c_void_p, C.c_int],
     "fmc_batch_get_tile": [C.c_void_p, IP],
     "fmc_chain_footprint": [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)],

@@ -198,6 +198,10 @@ int fmc_batch_get_log(fmc_batch *b, int chain_id, long long n, double *rec);
 /* FP64 full recompute of every chain's S(k) (periodic resync); returns the
  * largest |S_running - S_exact| seen */
 int fmc_batch_resync(fmc_batch *b, double *max_dev);
+/* GPU time of the batch kernel launches since the last reset, measured
+ * with HIP events on the batch's stream (one launch = one fmc_batch_launch
+ * completed by fmc_batch_sync); reset != 0 clears the totals */
+int fmc_batch_timing(fmc_batch *b, int reset, double *ms, long long *launches);
 /* tile shape: threads per chain work-group (64..1024); 0 = autotuned */
 int fmc_batch_set_tile(fmc_batch *b, int threads);
 int fmc_batch_get_tile(const fmc_batch *b, int *threads);
