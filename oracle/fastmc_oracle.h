@@ -183,7 +183,17 @@ double orc_sys_tail_delta(orc_sys *s, int gtype, int sign, double elrc);
 void orc_sys_energy_full(orc_sys *s, double out[6]);
 
 /* ---- whole chain (gcmc.f move loop, restated for insert/delete/displace
- * incl. translate/rotate-only moves) ---- */
+ * incl. translate/rotate-only moves) ----
+ *
+ * Parity status of the chain: the per-move energies are pinned (they are
+ * sums of the pinned pieces above; the bookkeeping invariant "accumulated
+ * estep == full-energy difference" is tested), but the ORDER of the duni
+ * draws, the adaptive-step schedule (gcmc.f:794-834), the forced insert at
+ * N=0 and the production switch are "parity unpinned": the reference seeds
+ * duni from the wall clock and its source does not compile unmodified
+ * here, so no fixed-seed reference trajectory exists to compare with.  They
+ * are pinned only by the statistical known answer (mean loading of the
+ * reference's example run) and by restating gcmc.f line by line. */
 typedef struct orc_chain_params {
   double temp;          /* K */
   int ntpguest;
