@@ -3,10 +3,12 @@
 Headline (``value``): whole-job GCMC trial moves per second of the batched
 workload -- 256 independent chains (32 synthetic frameworks of 300-6000
 atoms, orthorhombic 20-60 A cells replicated to >= 2 rcut, x 8 CO2
-pressures 0.01-10 bar, 298 K, kmax 8-14) advanced in one persistent launch
-per step, mixed precision (FP32 pair terms and phases, FP64 S(k) and
-totals).  A "step" is one launch that advances every chain by
-``--moves`` trial moves; inputs are resident in HBM before timing starts.
+pressures 0.01-10 bar, 298 K, kmax 8-14), mixed precision (FP32 pair terms
+and phases, FP64 S(k) and totals).  A "step" is one replay of the captured
+move cycle: one persistent launch that advances every chain by ``--moves``
+trial moves, then the FP64 full recompute of every chain (S(k) resync and
+energy drift).  Inputs are resident in HBM before timing starts; the
+warmup steps equilibrate the chains.
 
 Reported beside it: the plain single-chain FP64 drop-in rate (the Fortran
 driver over the C ABI on the 3392-atom CO2/N2 supercell), the ratio, the
@@ -41,16 +43,23 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
 
 
-def _chain_flops(nk, ns, kinds):
-    """Algorithmic FLOPs of one trial move: P*n_s*Nk*8 (phase products and
-    S accumulation) + Nk*6 (|S|^2 difference * a_k) + P*n_s*N_rc*30 (pair
-    distance, minimum image, two interpolations), N_rc = 202 pairs per site
-    within rcut at the workload density, P = 1 (insert/delete) or 2
-    (displace)."""
+PEAK_HBM_GBS = 8000.0
+
+
+def chain_work(nk, ns, n_rc, kinds, acc):
+    """Algorithmic bytes and FLOPs of one trial move, averaged over the
+    chain's move mix: bytes = 24 Nk (S(k) re/im + a_k read) + 16 Nk acc
+    (S(k) written back on accept) + 16 P n_s N_rc (pair neighbour x/y/z +
+    charge/type) + 12 P n_s (site coordinates); FLOPs = 8 P n_s Nk (phase
+    products and S accumulation) + 6 Nk (|S|^2 difference * a_k) + 30 P n_s
+    N_rc (distance, minimum image, two interpolations).  P = 1 for insert
+    and delete, 2 for displace; N_rc = pairs per site within rcut."""
     ins, dele, disp = kinds
     tot = max(1, ins + dele + disp)
     p = (ins + dele + 2 * disp) / tot
-    return p * ns * nk * 8 + nk * 6 + p * ns * 202 * 30
+    byt = 24 * nk + 16 * nk * acc + 16 * p * ns * n_rc + 12 * p * ns
+    flo = 8 * p * ns * nk + 6 * nk + 30 * p * ns * n_rc
+    return byt, flo
 
 
 # ---------------------------------------------------------------------------
@@ -83,13 +92,17 @@ def cpu_baseline(budget_s=20.0):
     moves = max(200, int(budget_s / per_move / 4))
     with ctxm.Pool(cores) as pool:
         t0 = time.perf_counter()
-        res = pool.map(_cpu_worker, [((8 * i) % 256, moves) for i in range(cores)])
+        # chain 17 i: framework (17 i) // 8, pressure (17 i) % 8 -- spans
+        # the framework sizes and all 8 pressures
+        res = pool.map(_cpu_worker, [((17 * i) % 256, moves) for i in range(cores)])
         wall = time.perf_counter() - t0
     total = sum(r[0] for r in res)
     return {"value": total / wall, "unit": "moves/s", "cores": cores,
             "kind": "port",
-            "sample": f"{cores} chains of the 256-chain workload (every 8th, one "
-                      f"per core), {moves} FP64 moves each, oracle move loop"}
+            "sample": f"{cores} chains of the 256-chain workload (chain 17*i: "
+                      f"every other framework, all 8 pressures), one per core, "
+                      f"{moves} FP64 moves each from an empty framework, oracle "
+                      f"move loop"}
 
 
 # ---------------------------------------------------------------------------
@@ -136,7 +149,9 @@ def build_batch(precision, n_chains=256):
         ctx = ctxs[key]
         b.add_chain(ctx, w.temp, [p * 1e5 for p in w.press_bar], w.seed,
                     delr=w.delr)
-        info.append((ctx.info()["nk"], len(ctx.nsites) and ctx.nsites[0]))
+        inf = ctx.info()
+        info.append(dict(nk=inf["nk"], ns=ctx.nsites[0], nfw=w.fw.n,
+                         volm=inf["volm"], rcut=w.rcut))
     return b, ws, info
 
 
@@ -171,9 +186,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--moves", type=int, default=200,
-                    help="trial moves per chain per step (one launch)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--moves", type=int, default=1000,
+                    help="trial moves per chain per step (one cycle: moves, "
+                         "then the FP64 recompute)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
@@ -183,7 +199,6 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     from fastmc_amd.capi import PREC_MIXED
 
     cpu = None if args.no_cpu else cpu_baseline()
@@ -191,34 +206,45 @@ def main():
     b, ws, info = build_batch(PREC_MIXED)
     n_chains = len(ws)
     for _ in range(args.warmup):
-        b.run(args.moves)
-    # timed region: every chain resident in HBM, one launch per step
-    import ctypes
-    hip = ctypes.CDLL("libamdhip64.so")
-    hip.hipDeviceSynchronize()
+        b.launch_cycles(1, args.moves)
+        b.sync()
+    before = [b.stats(c) for c in range(n_chains)]
+    # timed region: every chain resident in HBM, one graph replay per step
     b.timing(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        b.launch(args.moves)
+        b.launch_cycles(1, args.moves)
     b.sync()
-    hip.hipDeviceSynchronize()
     dt = time.perf_counter() - t0
     kernel_ms, launches = b.timing()
-    total_moves = args.steps * args.moves * n_chains
+    after = [b.stats(c) for c in range(n_chains)]
+    total_moves = sum(a["steps"] - z["steps"] for a, z in zip(after, before))
     value = total_moves / dt
     ms_per_step = dt / args.steps * 1e3
 
-    # algorithmic flops of the timed region from the move mix of every chain
-    flops = 0.0
+    # algorithmic bytes and flops of the timed region, chain by chain, from
+    # its own Nk, density and move mix
+    byt = flo = 0.0
     for c in range(n_chains):
-        st = b.stats(c)
-        kinds = (st["attempted"][1], st["attempted"][2],
-                 st["attempted"][3] + st["attempted"][4] + st["attempted"][5])
-        nk, ns = info[c]
-        flops += _chain_flops(nk, ns, kinds) * args.steps * args.moves
-    # per launch: algorithmic flops / average kernel duration (HIP events
-    # on the batch stream)
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
+        z, a = before[c], after[c]
+        att = [a["attempted"][i] - z["attempted"][i] for i in range(6)]
+        acc = [a["accepted"][i] - z["accepted"][i] for i in range(6)]
+        moves = a["steps"] - z["steps"]
+        kinds = (att[1], att[2], att[3] + att[4] + att[5])
+        accr = sum(acc) / max(1, sum(att))
+        I = info[c]
+        nat = I["nfw"] + 0.5 * (a["nmol"][0] + z["nmol"][0]) * I["ns"]
+        n_rc = nat / I["volm"] * 4.0 / 3.0 * math.pi * I["rcut"] ** 3
+        bb, ff = chain_work(I["nk"], I["ns"], n_rc, kinds, accr)
+        byt += bb * moves
+        flo += ff * moves
+    # per launch: algorithmic bytes / the batch kernel's average duration
+    # (HIP events on the batch stream, recorded inside the captured cycle)
+    k_s = kernel_ms * 1e-3
+    achieved = byt / k_s / 1e9
+    valu = flo / k_s / 1e12
+    drift = max(abs(a["drift"]) for a in after)
+    nmol = [a["nmol"][0] for a in after]
     tile = b.tile()
     b.close()
 
@@ -240,17 +266,23 @@ def main():
         "data": "synthetic (seeded generator, reference CONTROL/FIELD/CONFIG formats)",
         "config": {"workload": "256 batched CO2 chains: 32 frameworks 300-6000 "
                                "atoms x 8 pressures 0.01-10 bar, 298 K, kmax 8-14, "
-                               "rcut 12.5 A, mixed precision",
-                   "chains": n_chains, "moves_per_step_per_chain": args.moves,
-                   "tile_threads": tile},
+                               "rcut 12.5 A, mixed precision, FP64 recompute "
+                               "every cycle",
+                   "chains": n_chains, "moves_per_cycle": args.moves,
+                   "tile_threads": tile,
+                   "guest_molecules_mean": sum(nmol) / len(nmol),
+                   "guest_molecules_max": max(nmol)},
         "plain_fp64_moves_per_s": plain,
         "speedup_vs_plain_fp64": (value / plain) if plain else None,
         "mixed_vs_fp64_dE_rel_max": err,
         "mixed_vs_fp64_identical_moves": same,
+        "energy_drift_max_kcal": drift,
         "kernel_ms_per_launch": kernel_ms / max(1, launches),
-        "roofline": {"bound": "mfma", "achieved": achieved,
-                     "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": None},
+        "valu_fp32": {"achieved": valu, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                      "frac": valu / PEAK_FP32_TFLOPS},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -105,6 +106,12 @@ struct BChain {
   double* log;
   long long log_cap, log_n;
   int fault;   // 1: guest capacity reached (reference error 75)
+  long long stop_at;   // the chain idles once steps reaches this
+  // FP64 full-energy bookkeeping: e_acc is the running sum of accepted
+  // estep from the full energy at admission; e_full the last full
+  // recompute (kcal/mol); drift = e_acc - e_full at that recompute
+  double e_acc, e_full, drift, sk_dev;
+  long long n_recompute;
 };
 
 // ---- RNG on one lane (gcmc.f:2786-2799) ----
@@ -274,6 +281,48 @@ __device__ __forceinline__ Cplx<T> sfac(const Cplx<T>* X, const Cplx<T>* Y,
   }
   if (n >= 0) return {q * (clm * c.x - slm * c.y), q * (slm * c.x + clm * c.y)};
   return {q * (clm * c.x + slm * c.y), q * (slm * c.x - clm * c.y)};
+}
+
+__device__ __forceinline__ float fmat(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double fmat(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+
+// dS(k) of the trial's positions: sum_p q_p X[l]Y[m] Z[n] (conjugates for
+// negative indices), terms in T, sums in FP64.  Both kernel forms call this
+// one routine, contraction off and fused multiply-adds written out, so they
+// round identically.
+template <typename T, typename D_>
+__device__ __forceinline__ void trial_ds(const BSys& Y, const D_& D, int ig, int np,
+                                         const Cplx<T>* xy, int nlm, int lm, int n,
+                                         const Cplx<double> (*tab)[3][kBTab], double& dr,
+                                         double& di) {
+#pragma clang fp contract(off)
+  const int an = n < 0 ? -n : n;
+  const T sg = n < 0 ? T(-1) : T(1);
+  dr = 0.0;
+  di = 0.0;
+  for (int p = 0; p < np; ++p) {
+    const double q64 = Y.gq[ig][D.psite[p]];
+    const T q = static_cast<T>(D.pset[p] ? q64 : -q64);
+    const Cplx<T> a = xy[p * nlm + lm];
+    const Cplx<double> c64 = tab[p][2][an];
+    const T cx = static_cast<T>(c64.x), cy = sg * static_cast<T>(c64.y);
+    const T re = fmat(a.x, cx, -(a.y * cy));
+    const T im = fmat(a.y, cx, a.x * cy);
+    dr += static_cast<double>(q * re);
+    di += static_cast<double>(q * im);
+  }
+}
+
+// a_k (|S + dS|^2 - |S|^2) = a_k (2 S.dS + |dS|^2): no cancellation of |S|^2
+__device__ __forceinline__ double de_k(double ak, double s0, double s1, double dr, double di) {
+#pragma clang fp contract(off)
+  const double sd = fmat(s0, dr, s1 * di);
+  const double dd = fmat(dr, dr, di * di);
+  return ak * fmat(2.0, sd, dd);
 }
 
 __device__ __forceinline__ int ul(int v) { return static_cast<signed char>(v & 0xff); }
@@ -496,7 +545,10 @@ __device__ void decide(const BSys& Y, BChain& C, Rng& R, MoveDesc& D,
       else C.accept_disp[ig]++;
     }
   }
-  if (acc) C.energy[ig] += estep;
+  if (acc) {
+    C.energy[ig] += estep;
+    C.e_acc += estep;
+  }
   C.attempted[slot]++;
   if (acc) C.accepted[slot]++;
   D.accepted = acc ? 1 : 0;
@@ -527,7 +579,13 @@ __device__ void decide(const BSys& Y, BChain& C, Rng& R, MoveDesc& D,
 // ---------------------------------------------------------------------------
 // The persistent kernel
 // ---------------------------------------------------------------------------
-template <int BT, int KPT, bool MIXED>
+// STREAM: S(k), a_k and the k indices stay in global memory (L2-resident)
+// and dS(k) goes to the chain's scratch; for Nk beyond what registers and
+// LDS hold (kmax 16: Nk ~ 10^4).  Otherwise S(k) lives in registers, KPT
+// per thread, a_k and k indices in LDS.  Both forms give a thread the same
+// contiguous k = tid*kpt + j and reduce in the same order, so a chain's
+// results do not depend on the form its launch used.
+template <int BT, int KPT, bool MIXED, bool STREAM>
 __global__ void __launch_bounds__(BT)
     batch_kernel(const BSys* __restrict__ systems, BChain* __restrict__ chains,
                  const int* __restrict__ ids, long long nmoves, int nkcap) {
@@ -556,29 +614,35 @@ __global__ void __launch_bounds__(BT)
   // dynamic LDS: a_k [nkcap], packed k index [nkcap], then the XY tables
   extern __shared__ double dyn_lds[];
   double* ak_l = dyn_lds;
-  int* kv_l = reinterpret_cast<int*>(dyn_lds + nkcap);
+  int* kv_l = reinterpret_cast<int*>(dyn_lds + (STREAM ? 0 : nkcap));
   // X[l]*Y[m] for every (l, m) of the half space, per trial position:
   // [p][l*(2*kmax2+1) + m + kmax2]
   const int nlm = (Y.kmax[0] + 1) * (2 * Y.kmax[1] + 1);
-  Cplx<T>* xy_l = reinterpret_cast<Cplx<T>*>(kv_l + ((nkcap + 3) & ~3));
+  Cplx<T>* xy_l =
+      reinterpret_cast<Cplx<T>*>(STREAM ? reinterpret_cast<int*>(dyn_lds)
+                                        : kv_l + ((nkcap + 3) & ~3));
   // k-vectors spread evenly: this thread owns kpt = ceil(nk/BT) contiguous
   // k = tid*kpt + j; the index (l, m) is precomputed into kv's upper bits
   const int kpt = (nk + BT - 1) / BT;
-  for (int k = tid; k < nk; k += BT) {
-    ak_l[k] = Y.ak[k];
-    const int v = Y.kv[k];
-    const int lm = ul(v) * (2 * Y.kmax[1] + 1) + um(v) + Y.kmax[1];
-    kv_l[k] = (v & 0xff0000) | lm;   // n in bits 16..23, (l,m) index below
+  if (!STREAM) {
+    for (int k = tid; k < nk; k += BT) {
+      ak_l[k] = Y.ak[k];
+      const int v = Y.kv[k];
+      const int lm = ul(v) * (2 * Y.kmax[1] + 1) + um(v) + Y.kmax[1];
+      kv_l[k] = (v & 0xff0000) | lm;   // n in bits 16..23, (l,m) index below
+    }
   }
   const int k0 = tid * kpt;
   double sre[KPT], sim[KPT];
   double nre[KPT], nim[KPT];   // S_new of the trial (FP64) or S + dS
+  if (!STREAM) {
 #pragma unroll
-  for (int j = 0; j < KPT; ++j) {
-    const int k = k0 + j;
-    const bool in = j < kpt && k < nk;
-    sre[j] = in ? Cg.sre[k] : 0.0;
-    sim[j] = in ? Cg.sim[k] : 0.0;
+    for (int j = 0; j < KPT; ++j) {
+      const int k = k0 + j;
+      const bool in = j < kpt && k < nk;
+      sre[j] = in ? Cg.sre[k] : 0.0;
+      sim[j] = in ? Cg.sim[k] : 0.0;
+    }
   }
   Rng R;
   __syncthreads();
@@ -590,6 +654,9 @@ __global__ void __launch_bounds__(BT)
   __syncthreads();
 
   for (long long mv = 0; mv < nmoves; ++mv) {
+    // C.steps was last written before the barrier ending the previous move:
+    // every thread takes the same branch
+    if (C.steps >= C.stop_at) break;
     if (tid == 0) propose(Y, C, R, D);
     __syncthreads();
     const int kind = D.kind;
@@ -652,36 +719,31 @@ __global__ void __launch_bounds__(BT)
       __syncthreads();
       // ---- reciprocal space (ewald1_guest, ewald.f:141-319) ----
       // dS(k) = sum_new q e^{ik.r} - sum_old q e^{ik.r}
-      {
+      if (STREAM) {
+        const int w2 = 2 * Y.kmax[1] + 1;
+        const int k1 = min(k0 + kpt, nk);
+        for (int k = k0; k < k1; ++k) {
+          const int v = Y.kv[k];
+          const int lm = ul(v) * w2 + um(v) + Y.kmax[1];
+          double dr, di;
+          trial_ds<T>(Y, D, ig, np, xy_l, nlm, lm, un(v), tab, dr, di);
+          const double s0 = Cg.sre[k], s1 = Cg.sim[k];
+          Cg.dsre[k] = dr;
+          Cg.dsim[k] = di;
+          e1 += de_k(Y.ak[k], s0, s1, dr, di);
+        }
+      } else {
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
           if (j >= kpt || k0 + j >= nk) break;
           const int v = kv_l[k0 + j];
           const int lm = v & 0xffff;
           const int n = static_cast<signed char>((v >> 16) & 0xff);
-          const int an = n < 0 ? -n : n;
-          // per-position terms in T, accumulated in FP64
-          double dr = 0, di = 0;
-          for (int p = 0; p < np; ++p) {
-            const int set = D.pset[p];
-            const T q = static_cast<T>(set ? Y.gq[ig][D.psite[p]] : -Y.gq[ig][D.psite[p]]);
-            const Cplx<T> a = xy_l[p * nlm + lm];
-            const Cplx<double> c64 = tab[p][2][an];
-            const Cplx<T> c = {static_cast<T>(c64.x), static_cast<T>(c64.y)};
-            if (n >= 0) {
-              dr += static_cast<double>(q * (a.x * c.x - a.y * c.y));
-              di += static_cast<double>(q * (a.y * c.x + a.x * c.y));
-            } else {
-              dr += static_cast<double>(q * (a.x * c.x + a.y * c.y));
-              di += static_cast<double>(q * (a.y * c.x - a.x * c.y));
-            }
-          }
-          // |S + dS|^2 - |S|^2 = 2 S.dS + |dS|^2: no cancellation of |S|^2
-          const double ddr = static_cast<double>(dr);
-          const double ddi = static_cast<double>(di);
-          nre[j] = sre[j] + ddr;
-          nim[j] = sim[j] + ddi;
-          e1 += ak_l[k0 + j] * (2.0 * (sre[j] * ddr + sim[j] * ddi) + (ddr * ddr + ddi * ddi));
+          double dr, di;
+          trial_ds<T>(Y, D, ig, np, xy_l, nlm, lm, n, tab, dr, di);
+          nre[j] = sre[j] + dr;
+          nim[j] = sim[j] + di;
+          e1 += de_k(ak_l[k0 + j], sre[j], sim[j], dr, di);
         }
       }
       if (tid == 0) {
@@ -779,11 +841,19 @@ __global__ void __launch_bounds__(BT)
     __syncthreads();
     if (D.accepted) {
       // S(k) <- S_new (ewald.f:260-293, gcmc.f:971-974)
+      if (STREAM) {
+        const int k1 = min(k0 + kpt, nk);
+        for (int k = k0; k < k1; ++k) {
+          Cg.sre[k] = Cg.sre[k] + Cg.dsre[k];
+          Cg.sim[k] = Cg.sim[k] + Cg.dsim[k];
+        }
+      } else {
 #pragma unroll
-      for (int j = 0; j < KPT; ++j) {
-        if (j < kpt) {
-          sre[j] = nre[j];
-          sim[j] = nim[j];
+        for (int j = 0; j < KPT; ++j) {
+          if (j < kpt) {
+            sre[j] = nre[j];
+            sim[j] = nim[j];
+          }
         }
       }
       if (kind == FMC_INSERT) {
@@ -821,12 +891,14 @@ __global__ void __launch_bounds__(BT)
     __syncthreads();
   }
   // write the state back
+  if (!STREAM) {
 #pragma unroll
-  for (int j = 0; j < KPT; ++j) {
-    const int k = k0 + j;
-    if (j < kpt && k < nk) {
-      Cg.sre[k] = sre[j];
-      Cg.sim[k] = sim[j];
+    for (int j = 0; j < KPT; ++j) {
+      const int k = k0 + j;
+      if (j < kpt && k < nk) {
+        Cg.sre[k] = sre[j];
+        Cg.sim[k] = sim[j];
+      }
     }
   }
   if (tid == 0) {
@@ -865,6 +937,29 @@ __device__ __forceinline__ void grid_center(const GridParams& P, int c,
   z = P.S.cell[2] * f0 + P.S.cell[5] * f1 + P.S.cell[8] * f2;
 }
 
+// Is some periodic image of the separation within reach?  For imcon 1/2
+// the per-axis nearest image is the shortest; for a parallelepiped the
+// fractional-nint image of images (utility_pack.f:1221-1238) need not be, so
+// its 26 neighbours are tried too (enough while reach < every
+// perpendicular width: the shortest image then has fractional components
+// within one of the nint image's).
+__device__ __forceinline__ bool within_reach(const GridParams& P, double dx, double dy,
+                                             double dz) {
+  min_image(P.S, dx, dy, dz);
+  if (dx * dx + dy * dy + dz * dz <= P.reach2) return true;
+  if (P.S.imcon != 3) return false;
+  const double* c = P.S.cell;
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j)
+      for (int k = -1; k <= 1; ++k) {
+        const double x = dx + i * c[0] + j * c[3] + k * c[6];
+        const double y = dy + i * c[1] + j * c[4] + k * c[7];
+        const double z = dz + i * c[2] + j * c[5] + k * c[8];
+        if (x * x + y * y + z * z <= P.reach2) return true;
+      }
+  return false;
+}
+
 __global__ void __launch_bounds__(256) grid_count_kernel(GridParams P) {
   __shared__ int cnt;
   const int c = blockIdx.x;
@@ -873,11 +968,8 @@ __global__ void __launch_bounds__(256) grid_count_kernel(GridParams P) {
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   int mine = 0;
-  for (int a = threadIdx.x; a < P.S.nfw; a += 256) {
-    double dx = P.S.fx[a] - cx, dy = P.S.fy[a] - cy, dz = P.S.fz[a] - cz;
-    min_image(P.S, dx, dy, dz);
-    if (dx * dx + dy * dy + dz * dz <= P.reach2) ++mine;
-  }
+  for (int a = threadIdx.x; a < P.S.nfw; a += 256)
+    if (within_reach(P, P.S.fx[a] - cx, P.S.fy[a] - cy, P.S.fz[a] - cz)) ++mine;
   atomicAdd(&cnt, mine);
   __syncthreads();
   if (threadIdx.x == 0) P.count[c] = cnt;
@@ -895,11 +987,8 @@ __global__ void __launch_bounds__(256) grid_fill_kernel(GridParams P) {
   for (int a0 = 0; a0 < P.S.nfw; a0 += 256) {
     const int a = a0 + threadIdx.x;
     int f = 0;
-    if (a < P.S.nfw) {
-      double dx = P.S.fx[a] - cx, dy = P.S.fy[a] - cy, dz = P.S.fz[a] - cz;
-      min_image(P.S, dx, dy, dz);
-      f = (dx * dx + dy * dy + dz * dz <= P.reach2) ? 1 : 0;
-    }
+    if (a < P.S.nfw)
+      f = within_reach(P, P.S.fx[a] - cx, P.S.fy[a] - cy, P.S.fz[a] - cz) ? 1 : 0;
     scan[threadIdx.x] = f;
     __syncthreads();
     for (int o = 1; o < 256; o <<= 1) {
@@ -915,50 +1004,137 @@ __global__ void __launch_bounds__(256) grid_fill_kernel(GridParams P) {
   }
 }
 
-// framework structure factor and guest resync: S(k) = S_fw(k) + guests
-struct ResyncParams {
+// Periodic FP64 full recompute of a chain (no reference counterpart; the
+// reference only accumulates estep, gcmc.f:981,1151,1368).  It replaces the
+// running S(k) by S_fw(k) + the guests' exact sum (ewald.f:368-630 for the
+// guest atoms) and evaluates the chain's whole guest energy as
+// single_point does (gcmc.f:2518-2710): reciprocal (all - framework) incl.
+// the guests' self terms, real space over every guest-framework pair and
+// every guest-guest pair of different molecules (each once), the ewald3
+// exclusions and the tail.  FP64 throughout, whatever the batch precision.
+// base != 0 (admission): the chain's accumulators start from this energy,
+// every guest type's energy(i) = spenergy as gcmc.f:555-560 sets it.
+struct RecomputeParams {
   const BSys* sys;
   BChain* chains;
   const int* ids;
-  double* maxdev;   // per chain
+  double* maxdev;   // per work-group: largest |S_running - S_exact|
+  int base;
+  int one_id;       // >= 0: a single chain (admission) ...
+  const BSys* one;  // ... whose system is this device copy
 };
 
-__global__ void __launch_bounds__(256) resync_kernel(ResyncParams P) {
-  __shared__ double red[256 / kWave];
-  BChain& C = P.chains[P.ids[blockIdx.x]];
-  const BSys& Y = P.sys[C.sys];
-  double dev = 0.0;
-  for (int k = threadIdx.x; k < Y.nk; k += 256) {
+constexpr int kRecBT = 256;
+
+__global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
+  __shared__ double red[4][kRecBT / kWave];
+  __shared__ int gpre[kMaxGuestTypes + 1];
+  const int tid = threadIdx.x;
+  BChain& C = P.chains[P.one_id >= 0 ? P.one_id : P.ids[blockIdx.x]];
+  const BSys& Y = P.one ? *P.one : P.sys[C.sys];
+  if (tid == 0) {
+    int na = 0;
+    for (int g = 0; g < Y.ngt; ++g) {
+      gpre[g] = na;
+      na += C.nmol[g] * Y.ns[g];
+    }
+    gpre[Y.ngt] = na;
+  }
+  __syncthreads();
+  const int nga = gpre[Y.ngt];
+  const double* rc = Y.S.rcell;
+  // reciprocal space
+  double erec = 0.0, dev = 0.0;
+  for (int k = tid; k < Y.nk; k += kRecBT) {
     const int v = Y.kv[k];
     const int l = ul(v), m = um(v), n = un(v);
-    double sx = Y.sfw_re[k], sy = Y.sfw_im[k];
+    const double fr = Y.sfw_re[k], fi = Y.sfw_im[k];
+    double sx = fr, sy = fi;
     for (int g = 0; g < Y.ngt; ++g) {
       const int nat = C.nmol[g] * Y.ns[g];
       for (int a = 0; a < nat; ++a) {
         const double x = C.gx[g][a], y = C.gy[g][a], z = C.gz[g][a];
-        const double* rc = Y.S.rcell;
         double ph = l * (rc[0] * x + rc[3] * y + rc[6] * z) +
                     m * (rc[1] * x + rc[4] * y + rc[7] * z) +
                     n * (rc[2] * x + rc[5] * y + rc[8] * z);
         ph -= rint(ph);
-        double s, c;
-        sincospi(2.0 * ph, &s, &c);
+        double sv, cv;
+        sincospi(2.0 * ph, &sv, &cv);
         const double q = Y.gq[g][a % Y.ns[g]];
-        sx += q * c;
-        sy += q * s;
+        sx += q * cv;
+        sy += q * sv;
       }
     }
     dev = fmax(dev, fmax(fabs(sx - C.sre[k]), fabs(sy - C.sim[k])));
     C.sre[k] = sx;
     C.sim[k] = sy;
+    // |S|^2 - |S_fw|^2 = (S - S_fw).(S + S_fw)
+    erec += Y.ak[k] * ((sx - fr) * (sx + fr) + (sy - fi) * (sy + fi));
   }
+  // real space: guest atom i against its cell's framework candidates and
+  // against every later guest atom of another molecule
+  double e2 = 0.0, ev = 0.0;
+  for (int i = 0; i < nga; ++i) {
+    int g = 0;
+    while (g + 1 < Y.ngt && gpre[g + 1] <= i) ++g;
+    const int la = i - gpre[g];
+    const int s = la % Y.ns[g];
+    const int mol = la / Y.ns[g];
+    const double xi = C.gx[g][la], yi = C.gy[g][la], zi = C.gz[g][la];
+    const double ce = Y.gq[g][s] / Y.S.epsq * kR4pie0;
+    const int ts = Y.gtype[g][s];
+    const int c = cell_of(Y, xi, yi, zi);
+    const int lo = Y.goff[c], len = Y.goff[c + 1] - lo;
+    for (int t = tid; t < len; t += kRecBT) {
+      const int a = Y.gidx[lo + t];
+      pair_fp64(Y.S, ce, ts, xi - Y.S.fx[a], yi - Y.S.fy[a], zi - Y.S.fz[a], Y.S.fq[a],
+                Y.S.ftype[a], e2, ev);
+    }
+    for (int j = i + 1 + tid; j < nga; j += kRecBT) {
+      int h = 0;
+      while (h + 1 < Y.ngt && gpre[h + 1] <= j) ++h;
+      const int lb = j - gpre[h];
+      if (h == g && lb / Y.ns[h] == mol) continue;
+      const int sb = lb % Y.ns[h];
+      pair_fp64(Y.S, ce, ts, xi - C.gx[h][lb], yi - C.gy[h][lb], zi - C.gz[h][lb],
+                Y.gq[h][sb], Y.gtype[h][sb], e2, ev);
+    }
+  }
+  erec = wave_sum(erec);
+  e2 = wave_sum(e2);
+  ev = wave_sum(ev);
   for (int off = kWave / 2; off > 0; off >>= 1) dev = fmax(dev, __shfl_xor(dev, off, kWave));
-  if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = dev;
+  if (tid % kWave == 0) {
+    red[0][tid / kWave] = erec;
+    red[1][tid / kWave] = e2;
+    red[2][tid / kWave] = ev;
+    red[3][tid / kWave] = dev;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = 0.0;
-    for (int w = 0; w < 256 / kWave; ++w) m = fmax(m, red[w]);
-    P.maxdev[blockIdx.x] = m;
+  if (tid == 0) {
+    double a = 0.0, b = 0.0, cc = 0.0, d = 0.0;
+    for (int w = 0; w < kRecBT / kWave; ++w) {
+      a += red[0][w];
+      b += red[1][w];
+      cc += red[2][w];
+      d = fmax(d, red[3][w]);
+    }
+    double e = Y.rvolm2 * kR4pie0 * a / Y.S.epsq;
+    for (int g = 0; g < Y.ngt; ++g) e += C.nmol[g] * (Y.engsic[g] + Y.e3[g]);
+    e += b + cc;
+    // the move loop's tail bookkeeping starts from elrc = 0 and adds
+    // delrc = tail(N +/- 1) - elrc (gcmc.f:891-901): compare the running
+    // sum with the same tail it carries, report the true one in e_full
+    const double book = (e + C.elrc) / C.engunit;
+    C.e_full = (e + tail_of(Y, C.nmol)) / C.engunit;
+    if (P.base) {
+      C.e_acc = book;
+      for (int g = 0; g < Y.ngt; ++g) C.energy[g] = C.e_full;
+    }
+    C.drift = C.e_acc - book;
+    C.sk_dev = d;
+    C.n_recompute++;
+    P.maxdev[blockIdx.x] = d;
   }
 }
 
@@ -1062,6 +1238,9 @@ struct SysHost {
   uint64_t key;
   BSys dev;
   size_t bytes = 0;
+  int refs = 0;     // chains using it; freed at zero
+  int nlm = 0, npos = 0;
+  BSys* d_self = nullptr;   // device copy of dev (admission recompute)
   std::vector<std::unique_ptr<DevBuf>> keep;
 };
 
@@ -1106,7 +1285,9 @@ void duni_setup(int i, int j, int k, int l, BChain* C) {
 struct fmc_batch {
   int max_chains = 0;
   int precision = FMC_PREC_FP64;
-  int bt = 0, kpt = 0;   // 0 = autotune at the next run
+  int bt = 0, kpt = 0;   // block size; register tile (0: streaming form)
+  bool tuned = false;    // false: autotune at the next launch
+  bool force_stream = false;
   std::vector<std::unique_ptr<SysHost>> systems;
   std::vector<std::unique_ptr<ChainHost>> chains;
   BSys* d_sys = nullptr;
@@ -1114,10 +1295,25 @@ struct fmc_batch {
   BChain* d_chains = nullptr;
   int* d_ids = nullptr;
   double* d_dev = nullptr;
+  double* d_dev1 = nullptr;   // admission recompute's output
   std::vector<BChain> h_chains;
   bool dirty_sys = true;
   hipStream_t stream = nullptr;
+  hipStream_t setup = nullptr;   // table/grid building while a launch runs
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // graph replays: one event pair around each replay of the move graph
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> cyc_ev;
+  long long cyc_pending = 0;
+  int timing_mode = 0;   // 1: ev0/ev1 bracket one kernel; 2: cyc_ev
+  // captured move cycle: the batch kernel (graph) and the FP64 recompute
+  // (graph_rec), and what they were captured for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph_rec = nullptr;
+  hipGraphExec_t graph_rec_exec = nullptr;
+  std::vector<int> graph_ids;
+  long long graph_moves = -1;
+  int graph_bt = 0, graph_kpt = -1;
   bool timing_pending = false;
   double kernel_ms = 0.0;
   long long launches = 0;
@@ -1132,18 +1328,22 @@ namespace {
 using KernelFn = void (*)(const BSys*, BChain*, const int*, long long, int);
 
 struct KCfg {
-  int bt, kpt;
+  int bt, kpt;   // kpt 0: the streaming form
   bool mixed;
   KernelFn fn;
 };
 
 #define KC(BT_, KPT_)                                                      \
-  {BT_, KPT_, false, batch_kernel<BT_, KPT_, false>},                       \
-      {BT_, KPT_, true, batch_kernel<BT_, KPT_, true>}
+  {BT_, KPT_, false, batch_kernel<BT_, KPT_, false, false>},                \
+      {BT_, KPT_, true, batch_kernel<BT_, KPT_, true, false>}
+#define KS(BT_)                                                            \
+  {BT_, 0, false, batch_kernel<BT_, 1, false, true>},                       \
+      {BT_, 0, true, batch_kernel<BT_, 1, true, true>}
 
 const KCfg kConfigs[] = {
     KC(256, 8),  KC(256, 16), KC(256, 28), KC(512, 4),  KC(512, 8),
     KC(512, 14), KC(1024, 2), KC(1024, 4), KC(1024, 7), KC(1024, 10),
+    KS(256),     KS(512),
 };
 
 const KCfg* find_cfg(int bt, int kpt, bool mixed) {
@@ -1156,7 +1356,8 @@ const KCfg* find_cfg(int bt, int kpt, bool mixed) {
 int kpt_for(int bt, int nk) {
   int best = 0;
   for (const auto& c : kConfigs)
-    if (c.bt == bt && !c.mixed && c.bt * c.kpt >= nk && (best == 0 || c.kpt < best))
+    if (c.bt == bt && !c.mixed && c.kpt > 0 && c.bt * c.kpt >= nk &&
+        (best == 0 || c.kpt < best))
       best = c.kpt;
   return best;
 }
@@ -1165,7 +1366,7 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   const SystemDesc& s = ctx->sys;
   const uint64_t key = system_key(s);
   for (size_t i = 0; i < b->systems.size(); ++i)
-    if (b->systems[i]->key == key) {
+    if (b->systems[i] && b->systems[i]->key == key) {
       *out = static_cast<int>(i);
       return FMC_OK;
     }
@@ -1233,7 +1434,7 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
       (r = dput(K, &sim, static_cast<double*>(nullptr), nk)) ||
       (r = dput(K, &rc, s.rcell, 9)))
     return r;
-  hipLaunchKernelGGL(sfac_kernel, dim3((nk + 255) / 256), dim3(256), 0, b->stream,
+  hipLaunchKernelGGL(sfac_kernel, dim3((nk + 255) / 256), dim3(256), 0, b->setup,
                      nk, kv, rc, nfw, fx, fy, fz, fq, sre, sim);
   BCHECK(hipGetLastError());
   Y.sfw_re = sre;
@@ -1259,26 +1460,27 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   G.S = S;
   for (int d = 0; d < 3; ++d) G.gdim[d] = Y.gdim[d];
   const double reach = s.rcut + hd + 1e-6;
+  if (reach >= std::min(p[6], std::min(p[7], p[8]))) return FMC_ERR_CELL;
   G.reach2 = reach * reach;
   int *cnt, *off;
   if ((r = dput(K, &cnt, static_cast<int*>(nullptr), ncell)) ||
       (r = dput(K, &off, static_cast<int*>(nullptr), ncell + 1)))
     return r;
   G.count = cnt;
-  hipLaunchKernelGGL(grid_count_kernel, dim3(ncell), dim3(256), 0, b->stream, G);
+  hipLaunchKernelGGL(grid_count_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
   BCHECK(hipGetLastError());
   std::vector<int> hc(ncell), ho(ncell + 1, 0);
-  BCHECK(hipMemcpyAsync(hc.data(), cnt, ncell * sizeof(int), hipMemcpyDeviceToHost, b->stream));
-  BCHECK(hipStreamSynchronize(b->stream));
+  BCHECK(hipMemcpyAsync(hc.data(), cnt, ncell * sizeof(int), hipMemcpyDeviceToHost, b->setup));
+  BCHECK(hipStreamSynchronize(b->setup));
   for (int c = 0; c < ncell; ++c) ho[c + 1] = ho[c] + hc[c];
   BCHECK(hipMemcpy(off, ho.data(), (ncell + 1) * sizeof(int), hipMemcpyHostToDevice));
   unsigned short* idx;
   if ((r = dput(K, &idx, static_cast<unsigned short*>(nullptr), ho[ncell]))) return r;
   G.off = off;
   G.idx = idx;
-  hipLaunchKernelGGL(grid_fill_kernel, dim3(ncell), dim3(256), 0, b->stream, G);
+  hipLaunchKernelGGL(grid_fill_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
   BCHECK(hipGetLastError());
-  BCHECK(hipStreamSynchronize(b->stream));
+  BCHECK(hipStreamSynchronize(b->setup));
   Y.goff = off;
   Y.gidx = idx;
   // guests
@@ -1327,18 +1529,49 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
       for (int h = 0; h < Y.ngt; ++h) Y.tA[g][h] = tA[g][h];
     }
   }
+  if ((r = dput(K, &sh->d_self, &Y, 1))) return r;
   for (auto& kb : K) sh->bytes += kb->n;
-  b->max_nk = std::max(b->max_nk, nk);
-  b->max_nlm = std::max(b->max_nlm, (s.kmax[0] + 1) * (2 * s.kmax[1] + 1));
-  for (const auto& g : s.guests) b->max_pos = std::max(b->max_pos, 2 * g.ns);
-  b->systems.push_back(std::move(sh));
+  sh->nlm = (s.kmax[0] + 1) * (2 * s.kmax[1] + 1);
+  for (const auto& g : s.guests) sh->npos = std::max(sh->npos, 2 * g.ns);
   b->dirty_sys = true;
+  for (size_t i = 0; i < b->systems.size(); ++i)
+    if (!b->systems[i]) {
+      b->systems[i] = std::move(sh);
+      *out = static_cast<int>(i);
+      return FMC_OK;
+    }
+  b->systems.push_back(std::move(sh));
   *out = static_cast<int>(b->systems.size()) - 1;
   return FMC_OK;
 }
 
+void invalidate_graph(fmc_batch* b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->graph_rec_exec) (void)hipGraphExecDestroy(b->graph_rec_exec);
+  if (b->graph_rec) (void)hipGraphDestroy(b->graph_rec);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  b->graph_rec_exec = nullptr;
+  b->graph_rec = nullptr;
+  b->graph_moves = -1;
+}
+
+// LDS sizing follows the systems in use
+void refresh_limits(fmc_batch* b) {
+  invalidate_graph(b);
+  b->max_nk = b->max_nlm = b->max_pos = 0;
+  for (const auto& sh : b->systems) {
+    if (!sh || sh->refs <= 0) continue;
+    b->max_nk = std::max(b->max_nk, sh->dev.nk);
+    b->max_nlm = std::max(b->max_nlm, sh->nlm);
+    b->max_pos = std::max(b->max_pos, sh->npos);
+  }
+}
+
 int sync_tables(fmc_batch* b) {
   if (b->dirty_sys) {
+    invalidate_graph(b);
     const int n = static_cast<int>(b->systems.size());
     if (n > b->d_sys_cap) {
       if (b->d_sys) BCHECK(hipFree(b->d_sys));
@@ -1346,7 +1579,10 @@ int sync_tables(fmc_batch* b) {
       BCHECK(hipMalloc(&b->d_sys, b->d_sys_cap * sizeof(BSys)));
     }
     std::vector<BSys> h(n);
-    for (int i = 0; i < n; ++i) h[i] = b->systems[i]->dev;
+    for (int i = 0; i < n; ++i) {
+      if (b->systems[i]) h[i] = b->systems[i]->dev;
+      else std::memset(&h[i], 0, sizeof(BSys));
+    }
     BCHECK(hipMemcpy(b->d_sys, h.data(), n * sizeof(BSys), hipMemcpyHostToDevice));
     b->dirty_sys = false;
   }
@@ -1359,24 +1595,57 @@ int sync_tables(fmc_batch* b) {
   return FMC_OK;
 }
 
+// dynamic LDS of a form: (a_k, packed k index unless streaming) + X.Y
+// tables; FMC_ERR_UNSUPPORTED when it and the kernel's static LDS exceed
+// the device's per-work-group limit
+int lds_need(fmc_batch* b, const KCfg* cfg, size_t* out) {
+  const size_t elt = b->precision == FMC_PREC_MIXED ? 8 : 16;
+  const int nkcap = b->max_nk;
+  size_t lds = static_cast<size_t>(b->max_pos) * b->max_nlm * elt;
+  if (cfg->kpt > 0)
+    lds += static_cast<size_t>(nkcap) * sizeof(double) +
+           static_cast<size_t>((nkcap + 3) & ~3) * sizeof(int);
+  int dev = 0, lmax = 0;
+  BCHECK(hipGetDevice(&dev));
+  BCHECK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  hipFuncAttributes fa;
+  BCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(cfg->fn)));
+  if (lds + fa.sharedSizeBytes > static_cast<size_t>(lmax)) return FMC_ERR_UNSUPPORTED;
+  *out = lds;
+  return FMC_OK;
+}
+
 int launch(fmc_batch* b, long long n, int bt, int kpt) {
   const KCfg* cfg = find_cfg(bt, kpt, b->precision == FMC_PREC_MIXED);
   if (!cfg) return FMC_ERR_UNSUPPORTED;
   if (b->active_ids.empty()) return FMC_OK;
-  const size_t elt = b->precision == FMC_PREC_MIXED ? 8 : 16;
-  const int nkcap = b->max_nk;
-  const size_t lds = static_cast<size_t>(nkcap) * sizeof(double) +
-                     static_cast<size_t>((nkcap + 3) & ~3) * sizeof(int) +
-                     static_cast<size_t>(b->max_pos) * b->max_nlm * elt;
-  int dev = 0, lmax = 0;
-  BCHECK(hipGetDevice(&dev));
-  BCHECK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  if (lds + 32768 > static_cast<size_t>(lmax)) return FMC_ERR_UNSUPPORTED;
+  if (kpt > 0 && bt * kpt < b->max_nk) return FMC_ERR_UNSUPPORTED;
+  size_t lds = 0;
+  int r;
+  if ((r = lds_need(b, cfg, &lds))) return r;
   hipLaunchKernelGGL(cfg->fn, dim3(static_cast<unsigned>(b->active_ids.size())),
                      dim3(cfg->bt), lds, b->stream, b->d_sys, b->d_chains, b->d_ids, n,
-                     nkcap);
+                     b->max_nk);
   BCHECK(hipGetLastError());
   return FMC_OK;
+}
+
+// the register form for block size bt if it covers the largest Nk and fits
+// in LDS, else the streaming form (256/512 threads); 0 if neither
+int pick_kpt(fmc_batch* b, int bt, bool allow_stream, int* kpt) {
+  const bool mixed = b->precision == FMC_PREC_MIXED;
+  const int k = kpt_for(bt, std::max(1, b->max_nk));
+  size_t lds;
+  if (k > 0 && lds_need(b, find_cfg(bt, k, mixed), &lds) == FMC_OK) {
+    *kpt = k;
+    return FMC_OK;
+  }
+  const KCfg* sc = find_cfg(bt, 0, mixed);
+  if (allow_stream && sc && lds_need(b, sc, &lds) == FMC_OK) {
+    *kpt = 0;
+    return FMC_OK;
+  }
+  return FMC_ERR_UNSUPPORTED;
 }
 
 // snapshot / restore of every active chain's device state (autotuning runs
@@ -1433,9 +1702,9 @@ int autotune(fmc_batch* b) {
   double best = 1e300;
   int bbt = 0, bkpt = 0;
   for (int bt : bts) {
-    const int kpt = kpt_for(bt, b->max_nk);
-    if (!kpt) continue;
-    // warm-up + timed run (a tile whose LDS does not fit is skipped)
+    int kpt = 0;
+    if (pick_kpt(b, bt, true, &kpt)) continue;
+    // warm-up + timed run
     r = launch(b, 4, bt, kpt);
     if (r == FMC_ERR_UNSUPPORTED) continue;
     if (r) return r;
@@ -1455,7 +1724,22 @@ int autotune(fmc_batch* b) {
   if (!bbt) return FMC_ERR_UNSUPPORTED;
   b->bt = bbt;
   b->kpt = bkpt;
+  b->tuned = true;
   return FMC_OK;
+}
+
+// before a launch: autotune once; a fixed tile follows the systems in use
+// (register form while it covers the largest Nk and fits, else streaming)
+int ensure_tile(fmc_batch* b) {
+  if (!b->tuned) return autotune(b);
+  if (b->force_stream) {
+    size_t lds;
+    const KCfg* sc = find_cfg(b->bt, 0, b->precision == FMC_PREC_MIXED);
+    if (!sc) return FMC_ERR_UNSUPPORTED;
+    b->kpt = 0;
+    return lds_need(b, sc, &lds);
+  }
+  return pick_kpt(b, b->bt, true, &b->kpt);
 }
 
 }  // namespace
@@ -1478,6 +1762,8 @@ extern "C" int fmc_batch_create(fmc_batch** out, int max_chains, int precision) 
   BCHECK(hipMemset(b->d_chains, 0, max_chains * sizeof(BChain)));
   BCHECK(hipMalloc(&b->d_ids, max_chains * sizeof(int)));
   BCHECK(hipMalloc(&b->d_dev, max_chains * sizeof(double)));
+  BCHECK(hipMalloc(&b->d_dev1, sizeof(double)));
+  BCHECK(hipStreamCreateWithFlags(&b->setup, hipStreamNonBlocking));
   *out = b.release();
   return FMC_OK;
 }
@@ -1485,13 +1771,21 @@ extern "C" int fmc_batch_create(fmc_batch** out, int max_chains, int precision) 
 extern "C" int fmc_batch_destroy(fmc_batch* b) {
   if (!b) return FMC_ERR_ARG;
   (void)hipStreamSynchronize(b->stream);
+  if (b->setup) (void)hipStreamSynchronize(b->setup);
   b->chains.clear();
   b->systems.clear();
   if (b->d_sys) (void)hipFree(b->d_sys);
   if (b->d_chains) (void)hipFree(b->d_chains);
   if (b->d_ids) (void)hipFree(b->d_ids);
   if (b->d_dev) (void)hipFree(b->d_dev);
+  if (b->d_dev1) (void)hipFree(b->d_dev1);
+  invalidate_graph(b);
+  if (b->setup) (void)hipStreamDestroy(b->setup);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
+  for (auto& e : b->cyc_ev) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1503,7 +1797,7 @@ extern "C" int fmc_chain_footprint(const fmc_ctx* ctx, int precision, long long*
   (void)precision;
   const SystemDesc& s = ctx->sys;
   const long long nk = static_cast<long long>(s.kvecs.size());
-  long long b = 2 * nk * 8;                                         // S(k)
+  long long b = 4 * nk * 8;                                   // S(k), dS(k)
   b += static_cast<long long>(s.guests.size()) * 3 * s.max_guest_atoms * 8;  // guests
   b += sizeof(BChain);
   // system share: framework, tables, k-vectors, grid (counted per chain as
@@ -1543,7 +1837,9 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
   const int nk = ctx->nk;
   auto& K = ch->keep;
   if ((r = dput(K, &C.sre, static_cast<double*>(nullptr), nk)) ||
-      (r = dput(K, &C.sim, static_cast<double*>(nullptr), nk)))
+      (r = dput(K, &C.sim, static_cast<double*>(nullptr), nk)) ||
+      (r = dput(K, &C.dsre, static_cast<double*>(nullptr), nk)) ||
+      (r = dput(K, &C.dsim, static_cast<double*>(nullptr), nk)))
     return r;
   BCHECK(hipMemcpy(C.sre, ctx->d_sre[ctx->cur], nk * sizeof(double), hipMemcpyDeviceToDevice));
   BCHECK(hipMemcpy(C.sim, ctx->d_sim[ctx->cur], nk * sizeof(double), hipMemcpyDeviceToDevice));
@@ -1582,12 +1878,31 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
   C.rotangle = kPi / 3.0;
   C.rota_rotangle = C.rotangle;
   C.elrc = 0.0;
+  C.stop_at = p->max_moves > 0 ? p->max_moves : std::numeric_limits<long long>::max();
   duni_setup(p->seed[0], p->seed[1], p->seed[2], p->seed[3], &C);
   for (auto& kb : K) ch->bytes += kb->n;
   ch->active = true;
+  SysHost& sh = *b->systems[sys];
+  sh.refs++;
+  refresh_limits(b);
   b->chains[slot] = std::move(ch);
   b->h_chains[slot] = C;
-  BCHECK(hipMemcpy(b->d_chains + slot, &C, sizeof(BChain), hipMemcpyHostToDevice));
+  // the slot is outside any launch in flight: its struct and the admission
+  // recompute (full FP64 energy, the accumulators' starting point) go
+  // through the set-up stream
+  BCHECK(hipMemcpyAsync(b->d_chains + slot, &C, sizeof(BChain), hipMemcpyHostToDevice,
+                        b->setup));
+  RecomputeParams P;
+  P.sys = nullptr;
+  P.chains = b->d_chains;
+  P.ids = nullptr;
+  P.maxdev = b->d_dev1;
+  P.base = 1;
+  P.one_id = slot;
+  P.one = sh.d_self;
+  hipLaunchKernelGGL(recompute_kernel, dim3(1), dim3(kRecBT), 0, b->setup, P);
+  BCHECK(hipGetLastError());
+  BCHECK(hipStreamSynchronize(b->setup));
   if (chain_id) *chain_id = slot;
   return FMC_OK;
 }
@@ -1596,7 +1911,13 @@ extern "C" int fmc_batch_remove_chain(fmc_batch* b, int id) {
   if (!b || id < 0 || id >= b->max_chains || !b->chains[id] || !b->chains[id]->active)
     return FMC_ERR_ARG;
   BCHECK(hipStreamSynchronize(b->stream));
+  const int sys = b->chains[id]->sys;
   b->chains[id].reset();
+  if (sys >= 0 && b->systems[sys] && --b->systems[sys]->refs <= 0) {
+    b->systems[sys].reset();   // nobody references it any more
+    b->dirty_sys = true;
+  }
+  refresh_limits(b);
   return FMC_OK;
 }
 
@@ -1604,18 +1925,24 @@ extern "C" int fmc_batch_set_tile(fmc_batch* b, int threads) {
   if (!b) return FMC_ERR_ARG;
   if (threads == 0) {
     b->bt = b->kpt = 0;
+    b->tuned = false;
+    b->force_stream = false;
     return FMC_OK;
   }
-  const int kpt = kpt_for(threads, std::max(1, b->max_nk));
-  if (!kpt) return FMC_ERR_UNSUPPORTED;
-  b->bt = threads;
-  b->kpt = kpt;
+  const bool stream = (threads & FMC_TILE_STREAM) != 0;
+  const int bt = threads & ~FMC_TILE_STREAM;
+  if (bt != 256 && bt != 512 && bt != 1024) return FMC_ERR_UNSUPPORTED;
+  if (stream && bt == 1024) return FMC_ERR_UNSUPPORTED;
+  b->bt = bt;
+  b->kpt = 0;
+  b->tuned = true;
+  b->force_stream = stream;
   return FMC_OK;
 }
 
 extern "C" int fmc_batch_get_tile(const fmc_batch* b, int* threads) {
   if (!b || !threads) return FMC_ERR_ARG;
-  *threads = b->bt;
+  *threads = b->bt | ((b->tuned && b->kpt == 0 && b->bt) ? FMC_TILE_STREAM : 0);
   return FMC_OK;
 }
 
@@ -1625,14 +1952,13 @@ extern "C" int fmc_batch_launch(fmc_batch* b, long long n) {
   if ((r = fmc_batch_sync(b))) return r;
   if ((r = sync_tables(b))) return r;
   if (b->active_ids.empty()) return FMC_OK;
-  if (b->bt == 0 || b->kpt == 0 || b->bt * b->kpt < b->max_nk) {
-    if ((r = autotune(b))) return r;
-  }
+  if ((r = ensure_tile(b))) return r;
   // kernel time on the batch's own stream
   BCHECK(hipEventRecord(b->ev0, b->stream));
   if ((r = launch(b, n, b->bt, b->kpt))) return r;
   BCHECK(hipEventRecord(b->ev1, b->stream));
   b->timing_pending = true;
+  b->timing_mode = 1;
   return FMC_OK;
 }
 
@@ -1640,12 +1966,37 @@ extern "C" int fmc_batch_sync(fmc_batch* b) {
   if (!b) return FMC_ERR_ARG;
   BCHECK(hipStreamSynchronize(b->stream));
   if (b->timing_pending) {
+    // mode 2: one batch-kernel graph replay per event pair
     float ms = 0.0f;
-    BCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-    b->kernel_ms += ms;
-    b->launches++;
+    if (b->timing_mode == 2) {
+      for (long long c = 0; c < b->cyc_pending; ++c) {
+        BCHECK(hipEventElapsedTime(&ms, b->cyc_ev[c].first, b->cyc_ev[c].second));
+        b->kernel_ms += ms;
+        b->launches++;
+      }
+    } else {
+      BCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+      b->kernel_ms += ms;
+      b->launches++;
+    }
     b->timing_pending = false;
   }
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_query(fmc_batch* b, int* done) {
+  if (!b || !done) return FMC_ERR_ARG;
+  const hipError_t e = hipStreamQuery(b->stream);
+  if (e == hipSuccess) {
+    *done = 1;
+    return FMC_OK;
+  }
+  if (e == hipErrorNotReady) {
+    (void)hipGetLastError();   // not-ready is not an error to keep
+    *done = 0;
+    return FMC_OK;
+  }
+  BCHECK(e);
   return FMC_OK;
 }
 
@@ -1689,6 +2040,11 @@ extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
   }
   st->rotangle = C.rotangle;
   st->rota_rotangle = C.rota_rotangle;
+  st->e_acc = C.e_acc;
+  st->e_full = C.e_full;
+  st->drift = C.drift;
+  st->sk_dev = C.sk_dev;
+  st->recomputes = C.n_recompute;
   return C.fault ? FMC_ERR_CAPACITY : FMC_OK;
 }
 
@@ -1718,24 +2074,127 @@ extern "C" int fmc_batch_get_log(fmc_batch* b, int id, long long n, double* rec)
   return FMC_OK;
 }
 
-extern "C" int fmc_batch_resync(fmc_batch* b, double* max_dev) {
-  if (!b) return FMC_ERR_ARG;
-  int r;
-  if ((r = sync_tables(b))) return r;
+namespace {
+
+int enqueue_recompute(fmc_batch* b) {
   const int n = static_cast<int>(b->active_ids.size());
-  if (max_dev) *max_dev = 0.0;
   if (!n) return FMC_OK;
-  ResyncParams P;
+  RecomputeParams P;
   P.sys = b->d_sys;
   P.chains = b->d_chains;
   P.ids = b->d_ids;
   P.maxdev = b->d_dev;
-  hipLaunchKernelGGL(resync_kernel, dim3(n), dim3(256), 0, b->stream, P);
+  P.base = 0;
+  P.one_id = -1;
+  P.one = nullptr;
+  hipLaunchKernelGGL(recompute_kernel, dim3(n), dim3(kRecBT), 0, b->stream, P);
   BCHECK(hipGetLastError());
+  return FMC_OK;
+}
+
+}  // namespace
+
+extern "C" int fmc_batch_recompute(fmc_batch* b, double* max_sk_dev, double* max_drift) {
+  if (!b) return FMC_ERR_ARG;
+  int r;
+  if ((r = fmc_batch_sync(b))) return r;
+  if ((r = sync_tables(b))) return r;
+  if (max_sk_dev) *max_sk_dev = 0.0;
+  if (max_drift) *max_drift = 0.0;
+  const int n = static_cast<int>(b->active_ids.size());
+  if (!n) return FMC_OK;
+  if ((r = enqueue_recompute(b))) return r;
   std::vector<double> dev(n);
   BCHECK(hipMemcpyAsync(dev.data(), b->d_dev, n * sizeof(double), hipMemcpyDeviceToHost,
                         b->stream));
+  BCHECK(hipMemcpyAsync(b->h_chains.data(), b->d_chains, b->max_chains * sizeof(BChain),
+                        hipMemcpyDeviceToHost, b->stream));
   BCHECK(hipStreamSynchronize(b->stream));
-  if (max_dev) *max_dev = *std::max_element(dev.begin(), dev.end());
+  if (max_sk_dev) *max_sk_dev = *std::max_element(dev.begin(), dev.end());
+  if (max_drift) {
+    double m = 0.0;
+    for (int id : b->active_ids) m = std::max(m, std::fabs(b->h_chains[id].drift));
+    *max_drift = m;
+  }
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_resync(fmc_batch* b, double* max_dev) {
+  return fmc_batch_recompute(b, max_dev, nullptr);
+}
+
+extern "C" int fmc_batch_launch_cycles(fmc_batch* b, long long n_cycles,
+                                       long long moves_per_cycle) {
+  if (!b || n_cycles < 0 || moves_per_cycle < 0) return FMC_ERR_ARG;
+  int r;
+  if ((r = fmc_batch_sync(b))) return r;
+  if ((r = sync_tables(b))) return r;
+  if (b->active_ids.empty() || n_cycles == 0) return FMC_OK;
+  if ((r = ensure_tile(b))) return r;
+  if (!b->graph_exec || b->graph_ids != b->active_ids || b->graph_moves != moves_per_cycle ||
+      b->graph_bt != b->bt || b->graph_kpt != b->kpt) {
+    invalidate_graph(b);
+    for (int part = 0; part < 2; ++part) {
+      BCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      const int r1 = part == 0 ? launch(b, moves_per_cycle, b->bt, b->kpt)
+                               : enqueue_recompute(b);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (r1) {
+        if (g) (void)hipGraphDestroy(g);
+        return r1;
+      }
+      BCHECK(e);
+      hipGraphExec_t x = nullptr;
+      const hipError_t e2 = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (part == 0) {
+        b->graph = g;
+        b->graph_exec = x;
+      } else {
+        b->graph_rec = g;
+        b->graph_rec_exec = x;
+      }
+      BCHECK(e2);
+    }
+    b->graph_ids = b->active_ids;
+    b->graph_moves = moves_per_cycle;
+    b->graph_bt = b->bt;
+    b->graph_kpt = b->kpt;
+  }
+  while (static_cast<long long>(b->cyc_ev.size()) < n_cycles) {
+    hipEvent_t e0, e1;
+    BCHECK(hipEventCreate(&e0));
+    BCHECK(hipEventCreate(&e1));
+    b->cyc_ev.push_back({e0, e1});
+  }
+  // each cycle: [event] moves [event] FP64 recompute -- the events sit
+  // between replays on the batch stream and time the batch kernel alone
+  for (long long c = 0; c < n_cycles; ++c) {
+    BCHECK(hipEventRecord(b->cyc_ev[c].first, b->stream));
+    BCHECK(hipGraphLaunch(b->graph_exec, b->stream));
+    BCHECK(hipEventRecord(b->cyc_ev[c].second, b->stream));
+    BCHECK(hipGraphLaunch(b->graph_rec_exec, b->stream));
+  }
+  b->cyc_pending = n_cycles;
+  b->timing_pending = true;
+  b->timing_mode = 2;
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_poll(fmc_batch* b, int* ids, int cap, int* n) {
+  if (!b || !n || (cap > 0 && !ids)) return FMC_ERR_ARG;
+  int r;
+  if ((r = fmc_batch_sync(b))) return r;
+  *n = 0;
+  BCHECK(hipMemcpy(b->h_chains.data(), b->d_chains, b->max_chains * sizeof(BChain),
+                   hipMemcpyDeviceToHost));
+  for (int i = 0; i < b->max_chains; ++i) {
+    if (!b->chains[i] || !b->chains[i]->active) continue;
+    const BChain& C = b->h_chains[i];
+    if (C.steps >= C.stop_at || C.fault) {
+      if (*n < cap) ids[*n] = i;
+      ++*n;
+    }
+  }
   return FMC_OK;
 }

@@ -37,7 +37,7 @@ class Batch:
     def add_chain(self, ctx, temp: float, press_pa: Sequence[float],
                   seed: Sequence[int], freq: Optional[Sequence[Sequence[float]]] = None,
                   delr: float = 1.0, eq_steps: int = 0, engunit: float = 418.4,
-                  ratios=(0.5, 0.5, 0.5)) -> int:
+                  ratios=(0.5, 0.5, 0.5), max_moves: int = 0) -> int:
         p = ChainParams()
         p.temp = temp
         p.ntpguest = len(press_pa)
@@ -54,6 +54,7 @@ class Batch:
         p.rota_ratio = ratios[2]
         p.engunit = engunit
         p.eq_steps = eq_steps
+        p.max_moves = max_moves
         cid = C.c_int()
         check(self.L.fmc_batch_add_chain(self.h, ctx.h, C.byref(p), C.byref(cid)),
               "fmc_batch_add_chain")
@@ -73,6 +74,12 @@ class Batch:
     def sync(self):
         check(self.L.fmc_batch_sync(self.h), "fmc_batch_sync")
 
+    def done(self) -> bool:
+        """True when no launch is running (does not wait)."""
+        v = C.c_int()
+        check(self.L.fmc_batch_query(self.h, C.byref(v)), "fmc_batch_query")
+        return bool(v.value)
+
     def stats(self, cid: int) -> dict:
         st = ChainStats()
         r = self.L.fmc_batch_stats(self.h, cid, C.byref(st))
@@ -84,7 +91,9 @@ class Batch:
                     nmol=list(st.nmol)[:ng], energy=list(st.energy)[:ng],
                     sum_n=list(st.sum_n)[:ng], sum_n2=list(st.sum_n2)[:ng],
                     sum_e=list(st.sum_e)[:ng], delrdisp=list(st.delrdisp)[:ng],
-                    rotangle=st.rotangle, capacity_fault=(r == -4))
+                    rotangle=st.rotangle, e_acc=st.e_acc, e_full=st.e_full,
+                    drift=st.drift, sk_dev=st.sk_dev, recomputes=st.recomputes,
+                    capacity_fault=(r == -4))
 
     def enable_log(self, cid: int, cap: int):
         check(self.L.fmc_batch_enable_log(self.h, cid, cap), "fmc_batch_enable_log")
@@ -99,6 +108,28 @@ class Batch:
         v = C.c_double()
         check(self.L.fmc_batch_resync(self.h, C.byref(v)), "fmc_batch_resync")
         return v.value
+
+    def recompute(self):
+        """FP64 full recompute of every active chain: (largest S(k)
+        deviation replaced, largest |energy drift| in kcal/mol)."""
+        d = C.c_double()
+        e = C.c_double()
+        check(self.L.fmc_batch_recompute(self.h, C.byref(d), C.byref(e)),
+              "fmc_batch_recompute")
+        return d.value, e.value
+
+    def launch_cycles(self, n_cycles: int, moves_per_cycle: int):
+        """Enqueue n_cycles replays of the captured (moves + recompute)
+        cycle; complete with sync()."""
+        check(self.L.fmc_batch_launch_cycles(self.h, n_cycles, moves_per_cycle),
+              "fmc_batch_launch_cycles")
+
+    def poll(self, cap: int = 4096):
+        """Ids of chains that reached their max_moves (or faulted)."""
+        ids = (C.c_int * max(1, cap))()
+        n = C.c_int()
+        check(self.L.fmc_batch_poll(self.h, ids, cap, C.byref(n)), "fmc_batch_poll")
+        return [ids[i] for i in range(min(n.value, cap))]
 
     def timing(self, reset: bool = False):
         """(kernel milliseconds, launches) since the last reset, from HIP

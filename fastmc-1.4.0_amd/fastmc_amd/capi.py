@@ -35,7 +35,7 @@ class ChainParams(C.Structure):
                 ("freq", (C.c_double * 9) * 4), ("delr", C.c_double * 4),
                 ("disp_ratio", C.c_double * 4), ("tran_ratio", C.c_double * 4),
                 ("rota_ratio", C.c_double), ("engunit", C.c_double),
-                ("eq_steps", C.c_longlong)]
+                ("eq_steps", C.c_longlong), ("max_moves", C.c_longlong)]
 
 
 class ChainStats(C.Structure):
@@ -45,7 +45,9 @@ class ChainStats(C.Structure):
                 ("sum_n", C.c_double * 4), ("sum_n2", C.c_double * 4),
                 ("sum_e", C.c_double * 4), ("delrdisp", C.c_double * 4),
                 ("rotangle", C.c_double), ("tran_delr", C.c_double * 4),
-                ("rota_rotangle", C.c_double)]
+                ("rota_rotangle", C.c_double), ("e_acc", C.c_double),
+                ("e_full", C.c_double), ("drift", C.c_double),
+                ("sk_dev", C.c_double), ("recomputes", C.c_longlong)]
 
 
 _lib = None
@@ -81,10 +83,14 @@ _SIGS = {
     "fmc_batch_run": [C.c_void_p, C.c_longlong],
     "fmc_batch_launch": [C.c_void_p, C.c_longlong],
     "fmc_batch_sync": [C.c_void_p],
+    "fmc_batch_query": [C.c_void_p, IP],
     "fmc_batch_stats": [C.c_void_p, C.c_int, C.POINTER(ChainStats)],
     "fmc_batch_enable_log": [C.c_void_p, C.c_int, C.c_longlong],
     "fmc_batch_get_log": [C.c_void_p, C.c_int, C.c_longlong, DP],
     "fmc_batch_resync": [C.c_void_p, DP],
+    "fmc_batch_recompute": [C.c_void_p, DP, DP],
+    "fmc_batch_launch_cycles": [C.c_void_p, C.c_longlong, C.c_longlong],
+    "fmc_batch_poll": [C.c_void_p, IP, C.c_int, IP],
     "fmc_batch_timing": [C.c_void_p, C.c_int, DP, C.POINTER(C.c_longlong)],
     "fmc_batch_set_tile": [C.c_void_p, C.c_int],
     "fmc_batch_get_tile": [C.c_void_p, IP],

@@ -1,0 +1,169 @@
+"""Continuous-batching isotherm server over the batched chains.
+
+A request is one isotherm point: a framework, its guest species, their
+partial pressures, a temperature and a number of trial moves.  Requests
+are admitted as chain slots and device memory free up (admission by the
+chain's projected HBM footprint, ``fmc_chain_footprint``); an admitted
+request becomes one device-resident chain (coordinates, S(k), RNG and
+statistics stay in HBM until it finishes).  The GPU advances every
+admitted chain in slices of ``slice_moves`` moves; requests that arrive
+while a slice runs are built and admitted on the host meanwhile and join
+the next slice, and finished chains leave at slice boundaries.
+
+A chain's arithmetic does not depend on which other chains share its
+launch (one work-group per chain, a fixed tile), so a request's result is
+the same alone and in the mix.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Dict, List, Optional, Sequence
+
+from .batch import Batch, footprint
+from .capi import Context, PREC_MIXED
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float                 # seconds from the start of the run
+    arrays: dict                   # Context.from_arrays input
+    temp: float
+    press_pa: Sequence[float]
+    seed: Sequence[int]
+    moves: int
+    delr: float = 1.0
+    eq_steps: int = 0
+
+
+@dataclass
+class Result:
+    rid: int
+    arrival: float
+    admitted: float
+    done: float
+    stats: dict
+    footprint: int
+
+    @property
+    def latency(self) -> float:
+        return self.done - self.arrival
+
+
+@dataclass
+class Server:
+    slots: int = 256
+    precision: int = PREC_MIXED
+    slice_moves: int = 250
+    hbm_budget: int = 64 << 30     # bytes of chain state admitted at once
+    tile: int = 256
+    clock: Callable[[], float] = time.perf_counter
+    results: Dict[int, Result] = field(default_factory=dict)
+
+    def __post_init__(self):
+        self.batch = Batch(self.slots, self.precision)
+        self.batch.set_tile(self.tile)
+        self._live: Dict[int, tuple] = {}   # chain id -> (request, admitted, bytes)
+        self._used = 0
+        self._ctx: Dict[int, tuple] = {}    # rid -> (context, footprint) built
+        self._spent: List[Context] = []     # freed at the next slice boundary
+        self.slices = 0
+        self.moves_done = 0
+
+    def close(self):
+        self.batch.close()
+
+    def _admit(self, req: Request, now: float) -> bool:
+        if req.rid not in self._ctx:
+            ctx = Context.from_arrays(req.arrays)
+            self._ctx[req.rid] = (ctx, footprint(ctx, self.precision))
+        ctx, need = self._ctx[req.rid]
+        if len(self._live) >= self.slots or self._used + need > self.hbm_budget:
+            return False
+        cid = self.batch.add_chain(ctx, req.temp, req.press_pa, req.seed,
+                                   delr=req.delr, eq_steps=req.eq_steps,
+                                   max_moves=req.moves)
+        # the chain holds its own copy; the context's device buffers are
+        # released at the next slice boundary, not under a running slice
+        self._spent.append(self._ctx.pop(req.rid)[0])
+        self._live[cid] = (req, now, need)
+        self._used += need
+        return True
+
+    def _retire(self, now: float):
+        for cid in self.batch.poll():
+            req, t_adm, need = self._live.pop(cid)
+            st = self.batch.stats(cid)
+            self.batch.remove_chain(cid)
+            self._used -= need
+            self.moves_done += st["steps"]
+            self.results[req.rid] = Result(req.rid, req.arrival, t_adm, now, st, need)
+
+    def run(self, requests: Sequence[Request], realtime: bool = True) -> List[Result]:
+        """Serve the requests; with realtime, each becomes visible at its
+        arrival time (seconds after the start), else all at once."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        t0 = self.clock()
+        nxt = 0
+        waiting: List[Request] = []
+        in_flight = False
+        while nxt < len(pending) or waiting or self._live:
+            now = self.clock() - t0
+            while nxt < len(pending) and (not realtime or pending[nxt].arrival <= now):
+                waiting.append(pending[nxt])
+                nxt += 1
+            # admission (host work overlaps the slice in flight)
+            while waiting and self._admit(waiting[0], self.clock() - t0):
+                waiting.pop(0)
+            if in_flight:
+                if (realtime and nxt < len(pending) and not self.batch.done()
+                        and pending[nxt].arrival <= self.clock() - t0 + 2e-4):
+                    continue   # an arrival is due: admit it under this slice
+                if realtime and not self.batch.done():
+                    time.sleep(1e-4)
+                    continue
+                self.batch.sync()
+                in_flight = False
+                self._retire(self.clock() - t0)
+                for c in self._spent:
+                    c.close()
+                self._spent.clear()
+                continue
+            if self._live:
+                self.batch.launch(self.slice_moves)
+                self.slices += 1
+                in_flight = True
+            elif nxt < len(pending):
+                delay = pending[nxt].arrival - (self.clock() - t0)
+                if delay > 0:
+                    time.sleep(min(delay, 0.01))
+        return [self.results[r.rid] for r in requests]
+
+
+def percentile(v: Sequence[float], q: float) -> float:
+    s = sorted(v)
+    if not s:
+        return float("nan")
+    k = (len(s) - 1) * q / 100.0
+    lo = int(k)
+    hi = min(lo + 1, len(s) - 1)
+    return s[lo] + (s[hi] - s[lo]) * (k - lo)
+
+
+def poisson_requests(n: int, rate: float, seed: int = 7, moves: int = 5000,
+                     species=None) -> List[Request]:
+    """n isotherm-point requests with Poisson arrivals at `rate` per second:
+    framework 300-10000 atoms (log-uniform), 1-3 of CO2/N2/CH4, total
+    pressure 0.01-30 bar, 273-348 K."""
+    import numpy as np
+    from . import synth
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate)
+        w = synth.c4_request(rng, i)
+        out.append(Request(i, t, w.system_arrays(), w.temp,
+                           [p * 1e5 for p in w.press_bar], w.seed, moves, delr=w.delr))
+    return out

@@ -160,6 +160,8 @@ typedef struct fmc_chain_params {
   double rota_ratio;
   double engunit;       /* 418.4 for kcal */
   long long eq_steps;   /* moves before production averaging starts */
+  long long max_moves;  /* the chain stops after this many moves (0 = no
+                         * limit); a stopped chain idles in later launches */
 } fmc_chain_params;
 
 typedef struct fmc_chain_stats {
@@ -172,6 +174,13 @@ typedef struct fmc_chain_stats {
   double sum_n[4], sum_n2[4]; /* production sums of N, N^2 */
   double sum_e[4];            /* production sum of energy */
   double delrdisp[4], rotangle, tran_delr[4], rota_rotangle;
+  /* FP64 full recompute (fmc_batch_recompute; also run at admission):
+   * e_acc = running sum of accepted estep from the admission energy,
+   * e_full = full guest energy at the last recompute (kcal/mol),
+   * drift = e_acc - the recomputed energy with the same tail bookkeeping,
+   * sk_dev = largest |S_running(k) - S_exact(k)| the recompute replaced */
+  double e_acc, e_full, drift, sk_dev;
+  long long recomputes;
 } fmc_chain_stats;
 
 /* max_chains chains; precision FMC_PREC_*.  */
@@ -188,6 +197,9 @@ int fmc_batch_run(fmc_batch *b, long long n_moves);
 /* enqueue without waiting / wait for completion */
 int fmc_batch_launch(fmc_batch *b, long long n_moves);
 int fmc_batch_sync(fmc_batch *b);
+/* *done = 1 when nothing enqueued on the batch is still running (does not
+ * wait) */
+int fmc_batch_query(fmc_batch *b, int *done);
 int fmc_batch_stats(fmc_batch *b, int chain_id, fmc_chain_stats *st);
 /* per-move log for one chain (parity checks): rec[8*i..8*i+7] = {kind,
  * iguest, accepted, nmol_after, estep (kcal/mol), ewld1eng incl. self term,
@@ -195,14 +207,35 @@ int fmc_batch_stats(fmc_batch *b, int chain_id, fmc_chain_stats *st);
  * moves */
 int fmc_batch_enable_log(fmc_batch *b, int chain_id, long long capacity);
 int fmc_batch_get_log(fmc_batch *b, int chain_id, long long n, double *rec);
-/* FP64 full recompute of every chain's S(k) (periodic resync); returns the
- * largest |S_running - S_exact| seen */
+/* FP64 full recompute of every active chain (periodic resync; no
+ * reference counterpart, the reference only accumulates estep,
+ * gcmc.f:981,1151,1368): S(k) <- S_framework + exact guest sum, full guest
+ * energy as single_point (gcmc.f:2518-2710) computes it, drift against the
+ * running sum.  Returns the largest |S_running - S_exact| and |drift|. */
+int fmc_batch_recompute(fmc_batch *b, double *max_sk_dev, double *max_drift);
+/* same, S(k) deviation only (kept for callers of the first version) */
 int fmc_batch_resync(fmc_batch *b, double *max_dev);
+/* enqueue n_cycles cycles of (moves_per_cycle moves on every active chain,
+ * then the FP64 recompute); both launches are captured once as HIP graphs
+ * and replayed each cycle, with a HIP event pair on the batch stream around
+ * each move replay (fmc_batch_timing).  Complete with fmc_batch_sync.  The
+ * graphs are re-captured when the active set, the tile or moves_per_cycle
+ * change. */
+int fmc_batch_launch_cycles(fmc_batch *b, long long n_cycles,
+                            long long moves_per_cycle);
+/* chains that reached max_moves: up to cap ids into ids, count in *n
+ * (waits for the launch in flight) */
+int fmc_batch_poll(fmc_batch *b, int *ids, int cap, int *n);
 /* GPU time of the batch kernel launches since the last reset, measured
  * with HIP events on the batch's stream (one launch = one fmc_batch_launch
  * completed by fmc_batch_sync); reset != 0 clears the totals */
 int fmc_batch_timing(fmc_batch *b, int reset, double *ms, long long *launches);
-/* tile shape: threads per chain work-group (64..1024); 0 = autotuned */
+/* tile shape: threads per chain work-group (256, 512 or 1024); 0 =
+ * autotuned.  S(k) is register-resident when the largest Nk fits the tile,
+ * else the streaming form runs (S(k) in HBM/L2, any Nk); threads |
+ * FMC_TILE_STREAM forces the streaming form (256 or 512).  Results do not
+ * depend on the form. */
+#define FMC_TILE_STREAM 0x10000
 int fmc_batch_set_tile(fmc_batch *b, int threads);
 int fmc_batch_get_tile(const fmc_batch *b, int *threads);
 /* bytes of device memory one chain of this context needs (admission) */

@@ -40,7 +40,7 @@ def _compare(log, ref, tol=1e-10):
     return rel.max()
 
 
-@pytest.mark.parametrize("tile", [256, 512])
+@pytest.mark.parametrize("tile", [256, 512, 256 | 0x10000])
 def test_batch_fp64_matches_oracle_chain(fmc, oracle, tile):
     from fastmc_amd.batch import Batch
     from fastmc_amd import synth
@@ -105,3 +105,146 @@ def test_batch_mixed_precision_error(fmc, oracle):
           f"{np.mean(raw < 1e-6):.1%}); floored(1e-3 kT) rel max {floored.max():.2e}")
     assert err.max() < 1e-3 * kT
     assert floored.max() < 1e-2
+
+
+def test_batch_recompute_matches_oracle_energy(fmc, oracle):
+    """FP64 recompute after 1500 moves: the full guest energy equals the
+    oracle's single_point of the same (identical-trajectory) configuration,
+    the running S(k) equals the exact one, and the running energy sum has
+    not drifted."""
+    from fastmc_amd.batch import Batch
+    A = _porous(seed=4, guests=("co2",), press=(20.0,)).system_arrays()
+    n = 1500
+    ref, ch, s = _oracle_recs(oracle, A, 298.0, [20.0e5], (101, 23, 17, 5), None, n)
+    b = Batch(1)
+    ctx = fmc.Context.from_arrays(A)
+    cid = b.add_chain(ctx, 298.0, [20.0e5], (101, 23, 17, 5), delr=A["delr"])
+    st0 = b.stats(cid)
+    assert st0["e_full"] == 0.0 and st0["recomputes"] == 1   # admission
+    b.run(n)
+    skdev, drift = b.recompute()
+    st = b.stats(cid)
+    assert st["nmol"][0] == ref[-1, 3] and st["nmol"][0] > 5
+    eo = s.energy_full()
+    e_ref = (eo[1] + eo[2] + eo[3] + eo[4] + eo[5]) / 418.4
+    print(f"recompute: e_full {st['e_full']:.10f} oracle {e_ref:.10f} "
+          f"drift {st['drift']:.2e} skdev {skdev:.2e}")
+    assert abs(st["e_full"] - e_ref) <= 1e-9 * max(1.0, abs(e_ref))
+    assert abs(drift) < 1e-8 * max(1.0, abs(e_ref))
+    assert skdev < 1e-9
+
+
+def test_batch_admission_energy_is_revive_kat(fmc, restart_example):
+    """A chain admitted with the REVIVE configuration (90 CO2) starts from
+    the reference's accumulated energy -731.5072668178669 kcal/mol."""
+    from fastmc_amd.batch import Batch
+    from conftest import revive_config
+    ex = restart_example
+    field, config, e_rev = revive_config(ex)
+    ctx = fmc.context_from_inputs(field, config, ex["control"])
+    for prec in (0, 1):
+        b = Batch(2, prec)
+        cid = b.add_chain(ctx, 303.15, [0.994808e5], (1, 2, 3, 4), delr=1.0)
+        st = b.stats(cid)
+        assert abs(st["e_full"] - e_rev) < 1e-7, (prec, st["e_full"], e_rev)
+        assert st["energy"][0] == st["e_full"]
+        b.close()
+
+
+def test_graph_cycles_equal_eager(fmc):
+    """Replays of the captured (moves + FP64 recompute) cycle, with eager
+    work and synchronisation between them, equal the eager sequence on an
+    identical batch, replay by replay."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd.capi import PREC_MIXED
+    ws = [_porous(seed=s, guests=("co2",), press=(p,)) for s, p in ((4, 20.0), (9, 2.0))]
+    bs = []
+    for _ in range(2):
+        b = Batch(2, PREC_MIXED)
+        for i, w in enumerate(ws):
+            A = w.system_arrays()
+            b.add_chain(fmc.Context.from_arrays(A), 298.0, [w.press_bar[0] * 1e5],
+                        (11 + i, 22, 33, 44), delr=A["delr"])
+        b.set_tile(256)
+        bs.append(b)
+    g, e = bs
+    for rep in range(4):
+        g.launch_cycles(1, 150)
+        g.sync()
+        e.run(150)
+        e.recompute()
+        for cid in range(2):
+            sg, se = g.stats(cid), e.stats(cid)
+            for key in ("steps", "nmol", "energy", "e_full", "drift", "recomputes",
+                        "accepted"):
+                assert sg[key] == se[key], (rep, cid, key, sg[key], se[key])
+    # several replays in one call
+    g.launch_cycles(3, 150)
+    g.sync()
+    for _ in range(3):
+        e.run(150)
+        e.recompute()
+    for cid in range(2):
+        assert g.stats(cid)["energy"] == e.stats(cid)["energy"]
+    ms, nl = g.timing()
+    assert nl >= 5 and ms > 0.0
+
+
+def test_max_moves_and_poll(fmc):
+    from fastmc_amd.batch import Batch
+    from fastmc_amd.capi import PREC_MIXED
+    A = _porous(seed=4, guests=("co2",), press=(20.0,)).system_arrays()
+    b = Batch(3, PREC_MIXED)
+    ids = [b.add_chain(fmc.Context.from_arrays(A), 298.0, [20.0e5], (5, 6, 7, i + 1),
+                       delr=A["delr"], max_moves=m) for i, m in enumerate((120, 0, 300))]
+    b.run(200)
+    assert b.poll() == [ids[0]]
+    b.run(200)
+    assert b.poll() == [ids[0], ids[2]]
+    assert [b.stats(i)["steps"] for i in ids] == [120, 400, 300]
+    b.remove_chain(ids[0])
+    b.run(10)
+    assert b.stats(ids[1])["steps"] == 410
+
+
+def test_stream_form_equals_register_form(fmc):
+    """Mixed precision: the streaming form (S(k) in HBM) and the
+    register-resident form give bit-identical move logs."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd.capi import PREC_MIXED
+    A = _porous(seed=4, guests=("co2",), press=(20.0,)).system_arrays()
+    logs = []
+    for tile in (512, 512 | 0x10000):
+        b = Batch(1, PREC_MIXED)
+        cid = b.add_chain(fmc.Context.from_arrays(A), 298.0, [20.0e5], (3, 1, 4, 1),
+                          delr=A["delr"])
+        b.set_tile(tile)
+        b.enable_log(cid, 800)
+        b.run(800)
+        assert b.tile() == tile
+        logs.append(b.get_log(cid, 800))
+        b.close()
+    assert np.array_equal(logs[0], logs[1])
+
+
+def test_batch_fp64_triclinic_revive(fmc, oracle, restart_example):
+    """The batch on the reference's triclinic example, started from its
+    REVIVE configuration (90 CO2): identical trajectory to the oracle chain,
+    and the admission energy is known answer 2."""
+    from fastmc_amd.batch import Batch
+    from conftest import revive_config
+    ex = restart_example
+    field, config, e_rev = revive_config(ex)
+    n = 400
+    s, _, _ = oracle.system_from_inputs(field, config, ex["control"])
+    ch = oracle.OracleChain(s, 303.15, [0.994808e5], (7, 8, 9, 10), delr=1.0)
+    recs = ch.run(n)
+    ref = np.array([[r.kind, r.iguest, r.accepted, r.nmol_after, r.estep] for r in recs])
+    ctx = fmc.context_from_inputs(field, config, ex["control"])
+    b = Batch(1)
+    cid = b.add_chain(ctx, 303.15, [0.994808e5], (7, 8, 9, 10), delr=1.0)
+    assert abs(b.stats(cid)["e_full"] - e_rev) < 1e-7
+    b.enable_log(cid, n)
+    b.run(n)
+    worst = _compare(b.get_log(cid, n), ref)
+    print(f"triclinic revive: {int(ref[:, 2].sum())} accepted, worst estep rel {worst:.2e}")

@@ -1,0 +1,31 @@
+"""Continuous-batching server: every request completes, admission and
+retirement keep the slots consistent, and each request's result is the
+same alone and in the mix (bit for bit: chains do not share arithmetic)."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("steps", "nmol", "energy", "accepted", "attempted", "sum_n", "sum_e",
+        "e_acc", "e_full")
+
+
+def test_serve_same_alone_and_in_mix():
+    import fastmc_amd
+    from fastmc_amd import serve
+    fastmc_amd.lib()
+    reqs = serve.poisson_requests(6, rate=2000.0, seed=11, moves=700)
+    srv = serve.Server(slots=4, slice_moves=200)
+    res = srv.run(reqs, realtime=True)
+    assert len(res) == 6 and not srv._live
+    assert all(r.stats["steps"] == 700 for r in res)
+    assert all(r.done >= r.admitted >= 0.0 and r.latency > 0 for r in res)
+    # with 4 slots and 6 requests some waited for a slot
+    assert max(r.admitted - r.arrival for r in res) > 0.0
+    srv.close()
+    for k in (0, 3, 5):
+        one = serve.Server(slots=1, slice_moves=300)
+        alone = one.run([reqs[k]], realtime=False)[0]
+        one.close()
+        for key in KEYS:
+            assert alone.stats[key] == res[k].stats[key], (k, key)
+    print("latency ms:", [round(1e3 * r.latency, 1) for r in res])
