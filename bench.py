@@ -182,6 +182,29 @@ def de_error(moves=400):
     return worst, same_moves
 
 
+def serving_leg(n_req=400, rate=200.0, moves=5000):
+    """Isotherm-point serving: Poisson arrivals at `rate`/s, each request a
+    framework of 300-10000 atoms with 1-3 of CO2/N2/CH4, 0.01-30 bar,
+    273-348 K, `moves` trial moves, mixed precision; continuous batching
+    (admission by HBM footprint, slices of 250 moves).  Latency = arrival to
+    result."""
+    from fastmc_amd import serve
+    reqs = serve.poisson_requests(n_req, rate, seed=2024, moves=moves)
+    srv = serve.Server(slots=256, slice_moves=250, tile=512)
+    t0 = time.perf_counter()
+    res = srv.run(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = [r.latency for r in res]
+    wait = [r.admitted - r.arrival for r in res]
+    return {"requests": n_req, "rate_per_s": rate, "moves_per_request": moves,
+            "p50_latency_s": serve.percentile(lat, 50),
+            "p99_latency_s": serve.percentile(lat, 99),
+            "p50_admission_wait_s": serve.percentile(wait, 50),
+            "moves_per_s": sum(r.stats["steps"] for r in res) / wall,
+            "makespan_s": wall, "slices": srv.slices}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,6 +215,7 @@ def main():
                          "then the FP64 recompute)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; chains shard by chain "
@@ -250,6 +274,7 @@ def main():
 
     plain = None if args.no_plain else plain_fp64_rate()
     err, same = de_error()
+    serving = None if args.no_serve else serving_leg()
 
     line = {
         "metric": METRIC,
@@ -277,6 +302,7 @@ def main():
         "mixed_vs_fp64_dE_rel_max": err,
         "mixed_vs_fp64_identical_moves": same,
         "energy_drift_max_kcal": drift,
+        "serving": serving,
         "kernel_ms_per_launch": kernel_ms / max(1, launches),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,

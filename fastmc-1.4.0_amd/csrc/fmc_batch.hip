@@ -42,22 +42,17 @@ namespace fmc {
 
 constexpr int kBTab = 17;   // phase-table entries per dimension (kmax <= 16)
 constexpr int kLogFields = 8;
+constexpr int kLdsTypes = 32;   // atom types whose pair rows sit in LDS
 
-struct BSys {
-  DevSys S;
-  int nk;
-  const int* kv;
-  const double* ak;
-  const double* sfw_re;   // framework S(k) (constant)
-  const double* sfw_im;
-  int gdim[3];
-  const int* goff;        // [ncell+1]
-  const unsigned short* gidx;
-  const float* erc32;
-  const float* vvv32;
-  int kmax[3];
+// The small per-system constants the serial lane and the per-move set-up
+// read; the batch kernel keeps a copy in LDS for the whole launch.
+struct SysQ {
+  double cell[9], rcell[9];
+  double epsq;
   double rvolm2;          // 2 * 2 pi / V
   double volm;
+  int kmax[3];
+  int gdim[3];
   int ngt;
   int ns[kMaxGuestTypes];
   double gq[kMaxGuestTypes][kMaxSites];
@@ -69,6 +64,20 @@ struct BSys {
   double engsic[kMaxGuestTypes];
   // tail(n) = t0 + sum_g tb[g] n_g + sum_gh tA[g][h] n_g n_h
   double t0, tb[kMaxGuestTypes], tA[kMaxGuestTypes][kMaxGuestTypes];
+};
+
+struct BSys {
+  DevSys S;
+  int nk;
+  const int* kv;
+  const double* ak;
+  const double* sfw_re;   // framework S(k) (constant)
+  const double* sfw_im;
+  const int* goff;        // [ncell+1]
+  const unsigned short* gidx;
+  const float* erc32;
+  const float* vvv32;
+  SysQ q;
 };
 
 struct BChain {
@@ -184,11 +193,11 @@ __device__ bool accept_test(double eng, double rande, double volm,
   return rande < test;
 }
 
-__device__ double tail_of(const BSys& Y, const int* n) {
-  double t = Y.t0;
-  for (int g = 0; g < Y.ngt; ++g) t += Y.tb[g] * n[g];
-  for (int g = 0; g < Y.ngt; ++g)
-    for (int h = 0; h < Y.ngt; ++h) t += Y.tA[g][h] * n[g] * n[h];
+__device__ double tail_of(const SysQ& Z, const int* n) {
+  double t = Z.t0;
+  for (int g = 0; g < Z.ngt; ++g) t += Z.tb[g] * n[g];
+  for (int g = 0; g < Z.ngt; ++g)
+    for (int h = 0; h < Z.ngt; ++h) t += Z.tA[g][h] * n[g] * n[h];
   return t;
 }
 
@@ -207,27 +216,34 @@ struct MoveDesc {
   int pset[2 * kMaxSites], psite[2 * kMaxSites];
   int plo[2 * kMaxSites], plen[2 * kMaxSites];
   int ppre[2 * kMaxSites + 1];  // prefix of candidate-list lengths
+  // per position: coordinates, q/epsq*r4pie0, site type, sign of its terms
+  double ppx[2 * kMaxSites], ppy[2 * kMaxSites], ppz[2 * kMaxSites];
+  double pce[2 * kMaxSites];
+  double pq[2 * kMaxSites];     // +q (new set) or -q (old set)
+  int pts[2 * kMaxSites], psg[2 * kMaxSites];
+  // pairtab row of each position's site type (ntpatm <= kLdsTypes)
+  short ptab[2 * kMaxSites][kLdsTypes];
 };
 
-__device__ __forceinline__ int cell_of(const BSys& Y, double x, double y,
+__device__ __forceinline__ int cell_of(const SysQ& Z, double x, double y,
                                        double z) {
-  const double* rc = Y.S.rcell;
+  const double* rc = Z.rcell;
   int idx[3];
   for (int d = 0; d < 3; ++d) {
     double f = rc[d] * x + rc[3 + d] * y + rc[6 + d] * z;
     f -= floor(f);
-    int i = static_cast<int>(f * Y.gdim[d]);
-    idx[d] = i >= Y.gdim[d] ? Y.gdim[d] - 1 : (i < 0 ? 0 : i);
+    int i = static_cast<int>(f * Z.gdim[d]);
+    idx[d] = i >= Z.gdim[d] ? Z.gdim[d] - 1 : (i < 0 ? 0 : i);
   }
-  return (idx[0] * Y.gdim[1] + idx[1]) * Y.gdim[2] + idx[2];
+  return (idx[0] * Z.gdim[1] + idx[1]) * Z.gdim[2] + idx[2];
 }
 
 // Mixed-precision pair: FP64 separation and minimum image, FP32 distance
 // and table interpolation.
-__device__ __forceinline__ void pair_mixed(const BSys& Y, float chgea, int ts,
+// vk = the pair's vdW table (pairtab), -1 for none.
+__device__ __forceinline__ void pair_mixed(const BSys& Y, float chgea, int vk,
                                            double dx, double dy, double dz,
-                                           float qj, int tj, float& e2,
-                                           float& ev) {
+                                           float qj, float& e2, float& ev) {
   min_image(Y.S, dx, dy, dz);
   const float fx = static_cast<float>(dx), fy = static_cast<float>(dy),
               fz = static_cast<float>(dz);
@@ -245,9 +261,8 @@ __device__ __forceinline__ void pair_mixed(const BSys& Y, float chgea, int ts,
     const float t1 = v0 + (v1 - v0) * p, t2 = v1 + (v2 - v1) * (p - 1.0f);
     if (fabsf(chgprd) > 1e-10f) e2 += (t1 + (t2 - t1) * p * 0.5f) * chgprd;
   }
-  const int k = Y.S.pairtab[ts * Y.S.ntpatm + tj];
-  if (k >= 0) {
-    const float* tab = Y.vvv32 + static_cast<size_t>(k) * (Y.S.mxgrid + 1);
+  if (vk >= 0) {
+    const float* tab = Y.vvv32 + static_cast<size_t>(vk) * (Y.S.mxgrid + 1);
     const float u = r * static_cast<float>(Y.S.rdlrpot);
     int l = static_cast<int>(u);
     l = l > imax ? imax : l;
@@ -290,31 +305,28 @@ __device__ __forceinline__ double fmat(double a, double b, double c) {
   return __builtin_fma(a, b, c);
 }
 
-// dS(k) of the trial's positions: sum_p q_p X[l]Y[m] Z[n] (conjugates for
-// negative indices), terms in T, sums in FP64.  Both kernel forms call this
-// one routine, contraction off and fused multiply-adds written out, so they
-// round identically.
-template <typename T, typename D_>
-__device__ __forceinline__ void trial_ds(const BSys& Y, const D_& D, int ig, int np,
-                                         const Cplx<T>* xy, int nlm, int lm, int n,
-                                         const Cplx<double> (*tab)[3][kBTab], double& dr,
+// dS(k) of the trial's positions: sum_p (q_p X_p[l] Y_p[m]) Z_p[n], with
+// q X Y tabulated per (l, m) and Z per signed n (conjugates folded in); four
+// multiply-adds per position, accumulated in T.  Both kernel forms call
+// this one routine (contraction off, fused multiply-adds written out), so
+// they round identically.
+constexpr int kZRow = 2 * kBTab;   // Z table row: n = -16..16
+template <typename T>
+__device__ __forceinline__ void trial_ds(int np, const Cplx<T>* xy, int nlm, int lm,
+                                         const Cplx<T>* zt, int zoff, double& dr,
                                          double& di) {
 #pragma clang fp contract(off)
-  const int an = n < 0 ? -n : n;
-  const T sg = n < 0 ? T(-1) : T(1);
-  dr = 0.0;
-  di = 0.0;
+  T ar = 0, ai = 0;
   for (int p = 0; p < np; ++p) {
-    const double q64 = Y.gq[ig][D.psite[p]];
-    const T q = static_cast<T>(D.pset[p] ? q64 : -q64);
     const Cplx<T> a = xy[p * nlm + lm];
-    const Cplx<double> c64 = tab[p][2][an];
-    const T cx = static_cast<T>(c64.x), cy = sg * static_cast<T>(c64.y);
-    const T re = fmat(a.x, cx, -(a.y * cy));
-    const T im = fmat(a.y, cx, a.x * cy);
-    dr += static_cast<double>(q * re);
-    di += static_cast<double>(q * im);
+    const Cplx<T> c = zt[p * kZRow + zoff];
+    ar = fmat(a.x, c.x, ar);
+    ar = fmat(-a.y, c.y, ar);
+    ai = fmat(a.y, c.x, ai);
+    ai = fmat(a.x, c.y, ai);
   }
+  dr = static_cast<double>(ar);
+  di = static_cast<double>(ai);
 }
 
 // a_k (|S + dS|^2 - |S|^2) = a_k (2 S.dS + |dS|^2): no cancellation of |S|^2
@@ -333,7 +345,7 @@ __device__ __forceinline__ int un(int v) { return static_cast<signed char>((v >>
 // The serial lane: move selection, adaptive steps, draws and proposal
 // (gcmc.f:745-847, 855-877, 1010-1060, 1176-1248; utility_pack.f:2545-3047)
 // ---------------------------------------------------------------------------
-__device__ void propose(const BSys& Y, BChain& C, Rng& R, MoveDesc& D) {
+__device__ void propose(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D) {
   const double delrmin = 0.1, minangle = kPi / 18.0;
   int ig = 0;
   if (C.ntpguest > 1) ig = static_cast<int>(floor(R.next() * C.ntpguest));
@@ -348,7 +360,7 @@ __device__ void propose(const BSys& Y, BChain& C, Rng& R, MoveDesc& D) {
   else if (randmov < cum[7]) { kind = FMC_DISPLACE; sub = 5; }
   else if (randmov < cum[8]) kind = 0;
   else kind = FMC_DISPLACE;
-  const int ns = Y.ns[ig];
+  const int ns = Z.ns[ig];
   const int nmols = C.nmol[ig];
   if (nmols >= 1 && C.disp_count[ig] >= 1) {
     if (C.disp_count[ig] % 100 == 0) {
@@ -384,7 +396,7 @@ __device__ void propose(const BSys& Y, BChain& C, Rng& R, MoveDesc& D) {
   D.accepted = 0;
   if (kind == FMC_INSERT) {
     int used = 0;
-    for (int g = 0; g < Y.ngt; ++g) used += C.nmol[g] * Y.ns[g];
+    for (int g = 0; g < Z.ngt; ++g) used += C.nmol[g] * Z.ns[g];
     if (used + ns >= C.cap_atoms) {
       C.fault = 1;
       D.kind = 0;
@@ -392,14 +404,14 @@ __device__ void propose(const BSys& Y, BChain& C, Rng& R, MoveDesc& D) {
     }
     const double ra = R.next(), rb = R.next(), rc = R.next();
     const double r1 = R.next(), r2 = R.next(), r3 = R.next(), r4 = R.next();
-    const double* cell = Y.S.cell;
+    const double* cell = Z.cell;
     const double xc = ra * cell[0] + rb * cell[3] + rc * cell[6];
     const double yc = ra * cell[1] + rb * cell[4] + rc * cell[7];
     const double zc = ra * cell[2] + rb * cell[5] + rc * cell[8];
     for (int s = 0; s < ns; ++s) {
-      D.nx[s] = Y.tx[ig][s];
-      D.ny[s] = Y.ty[ig][s];
-      D.nz[s] = Y.tz[ig][s];
+      D.nx[s] = Z.tx[ig][s];
+      D.ny[s] = Z.ty[ig][s];
+      D.nz[s] = Z.tz[ig][s];
     }
     rotate_sites(ns, r1 * 2.0 * kPi, r2, r3, r4, D.nx, D.ny, D.nz);
     for (int s = 0; s < ns; ++s) {
@@ -452,15 +464,15 @@ __device__ void propose(const BSys& Y, BChain& C, Rng& R, MoveDesc& D) {
       z[s] = tc + z[s];
     }
     for (int s = 0; s < ns; ++s) {
-      const double m = Y.gmass[ig][s];
+      const double m = Z.gmass[ig][s];
       w = w + m;
       cx = cx + x[s] * m;
       cy = cy + y[s] * m;
       cz = cz + z[s] * m;
     }
     cx = cx / w; cy = cy / w; cz = cz / w;
-    const double* rcl = Y.S.rcell;
-    const double* cell = Y.S.cell;
+    const double* rcl = Z.rcell;
+    const double* cell = Z.cell;
     const double sx = (rcl[0] * cx + rcl[3] * cy + rcl[6] * cz);
     const double sy = (rcl[1] * cx + rcl[4] * cy + rcl[7] * cz);
     const double sz = (rcl[2] * cx + rcl[5] * cy + rcl[8] * cz);
@@ -494,17 +506,17 @@ __device__ void propose(const BSys& Y, BChain& C, Rng& R, MoveDesc& D) {
   }
   D.np = np;
   int na = 0;
-  for (int g = 0; g < Y.ngt; ++g) {
+  for (int g = 0; g < Z.ngt; ++g) {
     D.gpre[g] = na;
-    na += C.nmol[g] * Y.ns[g];
+    na += C.nmol[g] * Z.ns[g];
   }
-  D.gpre[Y.ngt] = na;
+  D.gpre[Z.ngt] = na;
   D.nguest = na;
 }
 
 // Acceptance and bookkeeping on the serial lane (gcmc.f:938-1002, 1106-1172,
 // 1326-1378, 1814-1870).
-__device__ void decide(const BSys& Y, BChain& C, Rng& R, MoveDesc& D,
+__device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
                        double e1, double e2, double ev) {
   const int ig = D.ig;
   const int nmols = C.nmol[ig];
@@ -513,15 +525,15 @@ __device__ void decide(const BSys& Y, BChain& C, Rng& R, MoveDesc& D,
   const int slot = D.kind == 0 ? 0 : (D.sub ? D.sub : D.kind);
   if (D.kind == FMC_INSERT || D.kind == FMC_DELETE) {
     int n2[kMaxGuestTypes];
-    for (int g = 0; g < Y.ngt; ++g) n2[g] = C.nmol[g];
+    for (int g = 0; g < Z.ngt; ++g) n2[g] = C.nmol[g];
     n2[ig] += (D.kind == FMC_INSERT) ? 1 : -1;
-    delrc = tail_of(Y, n2) - C.elrc;
+    delrc = tail_of(Z, n2) - C.elrc;
     if (D.kind == FMC_INSERT) {
-      e1 = e1 + Y.engsic[ig];
-      estep = (e1 + e2 + Y.e3[ig] + ev + delrc) / C.engunit;
+      e1 = e1 + Z.engsic[ig];
+      estep = (e1 + e2 + Z.e3[ig] + ev + delrc) / C.engunit;
     } else {
-      e1 = e1 - Y.engsic[ig];
-      estep = (e1 - e2 - Y.e3[ig] - ev + delrc) / C.engunit;
+      e1 = e1 - Z.engsic[ig];
+      estep = (e1 - e2 - Z.e3[ig] - ev + delrc) / C.engunit;
     }
     const double rande = R.next();
     acc = accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp,
@@ -583,24 +595,28 @@ __device__ void decide(const BSys& Y, BChain& C, Rng& R, MoveDesc& D,
 // and dS(k) goes to the chain's scratch; for Nk beyond what registers and
 // LDS hold (kmax 16: Nk ~ 10^4).  Otherwise S(k) lives in registers, KPT
 // per thread, a_k and k indices in LDS.  Both forms give a thread the same
-// contiguous k = tid*kpt + j and reduce in the same order, so a chain's
-// results do not depend on the form its launch used.
+// k = tid + j*BT (coalesced global and conflict-free LDS access) and reduce
+// in the same order, so a chain's results do not depend on the form its
+// launch used.  The register form keeps the trial's S + dS in registers
+// too; the streaming form writes dS to the chain's scratch.
 template <int BT, int KPT, bool MIXED, bool STREAM>
 __global__ void __launch_bounds__(BT)
     batch_kernel(const BSys* __restrict__ systems, BChain* __restrict__ chains,
                  const int* __restrict__ ids, long long nmoves, int nkcap) {
   using T = typename std::conditional<MIXED, float, double>::type;
   __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
+  __shared__ Cplx<T> zt[2 * kMaxSites * kZRow];           // Z_p[n], signed n
   __shared__ MoveDesc D;
   __shared__ float rng_u[97];
   __shared__ double red[3][BT / kWave];
   __shared__ int shift_lo, shift_hi;
   __shared__ BChain C;
+  __shared__ SysQ Z;   // the system's small constants, for the whole launch
 
   const int tid = threadIdx.x;
-  // The system description and the chain's buffer pointers are read from
-  // global memory (uniform addresses: scalar loads); the chain's mutable
-  // state lives in LDS for the serial lane.
+  // The chain's mutable state and the system's small constants are copied
+  // to LDS for the launch; tables and the framework are read from global
+  // memory.
   BChain& Cg = chains[ids[blockIdx.x]];
   const BSys& Y = systems[Cg.sys];
   {
@@ -608,37 +624,41 @@ __global__ void __launch_bounds__(BT)
     int* dst = reinterpret_cast<int*>(&C);
     for (int i = tid; i < static_cast<int>(sizeof(BChain) / 4); i += BT) dst[i] = src[i];
   }
+  {
+    const int* src = reinterpret_cast<const int*>(&Y.q);
+    int* dst = reinterpret_cast<int*>(&Z);
+    for (int i = tid; i < static_cast<int>(sizeof(SysQ) / 4); i += BT) dst[i] = src[i];
+  }
+  __syncthreads();
   const int nk = Y.nk;
-  // a_k and the packed (l,m,n) of every k-vector in LDS; this thread owns
-  // the contiguous k = tid*KPT + j, whose S(k) stays in registers
-  // dynamic LDS: a_k [nkcap], packed k index [nkcap], then the XY tables
+  // dynamic LDS: a_k [nkcap], packed k index [nkcap] (register form), then
+  // the XY tables
   extern __shared__ double dyn_lds[];
   double* ak_l = dyn_lds;
   int* kv_l = reinterpret_cast<int*>(dyn_lds + (STREAM ? 0 : nkcap));
   // X[l]*Y[m] for every (l, m) of the half space, per trial position:
   // [p][l*(2*kmax2+1) + m + kmax2]
-  const int nlm = (Y.kmax[0] + 1) * (2 * Y.kmax[1] + 1);
+  const int nlm = (Z.kmax[0] + 1) * (2 * Z.kmax[1] + 1);
   Cplx<T>* xy_l =
       reinterpret_cast<Cplx<T>*>(STREAM ? reinterpret_cast<int*>(dyn_lds)
                                         : kv_l + ((nkcap + 3) & ~3));
-  // k-vectors spread evenly: this thread owns kpt = ceil(nk/BT) contiguous
-  // k = tid*kpt + j; the index (l, m) is precomputed into kv's upper bits
+  // this thread owns k = tid + j*BT, j < kpt = ceil(nk/BT); the index
+  // (l, m) is precomputed into kv's upper bits
   const int kpt = (nk + BT - 1) / BT;
   if (!STREAM) {
     for (int k = tid; k < nk; k += BT) {
       ak_l[k] = Y.ak[k];
       const int v = Y.kv[k];
-      const int lm = ul(v) * (2 * Y.kmax[1] + 1) + um(v) + Y.kmax[1];
+      const int lm = ul(v) * (2 * Z.kmax[1] + 1) + um(v) + Z.kmax[1];
       kv_l[k] = (v & 0xff0000) | lm;   // n in bits 16..23, (l,m) index below
     }
   }
-  const int k0 = tid * kpt;
   double sre[KPT], sim[KPT];
-  double nre[KPT], nim[KPT];   // S_new of the trial (FP64) or S + dS
+  double nre[KPT], nim[KPT];   // S + dS of the trial
   if (!STREAM) {
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
-      const int k = k0 + j;
+      const int k = tid + j * BT;
       const bool in = j < kpt && k < nk;
       sre[j] = in ? Cg.sre[k] : 0.0;
       sim[j] = in ? Cg.sim[k] : 0.0;
@@ -657,7 +677,7 @@ __global__ void __launch_bounds__(BT)
     // C.steps was last written before the barrier ending the previous move:
     // every thread takes the same branch
     if (C.steps >= C.stop_at) break;
-    if (tid == 0) propose(Y, C, R, D);
+    if (tid == 0) propose(Z, C, R, D);
     __syncthreads();
     const int kind = D.kind;
     const int ns = D.ns;
@@ -673,12 +693,12 @@ __global__ void __launch_bounds__(BT)
         for (int t = tid; t < np * 3 * nl; t += BT) {
           const int p = t / (3 * nl), r = t - p * 3 * nl;
           const int d = r / nl, l = r - d * nl;
-          if (l > Y.kmax[d]) continue;
+          if (l > Z.kmax[d]) continue;
           const int set = D.pset[p], s = D.psite[p];
           const double x = set ? D.nx[s] : D.ox[s];
           const double y = set ? D.ny[s] : D.oy[s];
           const double z = set ? D.nz[s] : D.oz[s];
-          double f = Y.S.rcell[d] * x + Y.S.rcell[3 + d] * y + Y.S.rcell[6 + d] * z;
+          double f = Z.rcell[d] * x + Z.rcell[3 + d] * y + Z.rcell[6 + d] * z;
           f = f - floor(f);
           double ph = l * f;
           ph = ph - rint(ph);
@@ -689,20 +709,35 @@ __global__ void __launch_bounds__(BT)
         if (tid >= BT - kWave && tid < BT - kWave + np) {
           const int p = tid - (BT - kWave);
           const int set = D.pset[p], s = D.psite[p];
-          const int c = set ? cell_of(Y, D.nx[s], D.ny[s], D.nz[s])
-                            : cell_of(Y, D.ox[s], D.oy[s], D.oz[s]);
+          const int c = set ? cell_of(Z, D.nx[s], D.ny[s], D.nz[s])
+                            : cell_of(Z, D.ox[s], D.oy[s], D.oz[s]);
           const int lo = Y.goff[c];
           D.plo[p] = lo;
           D.plen[p] = Y.goff[c + 1] - lo;
+          D.ppx[p] = set ? D.nx[s] : D.ox[s];
+          D.ppy[p] = set ? D.ny[s] : D.oy[s];
+          D.ppz[p] = set ? D.nz[s] : D.oz[s];
+          D.pce[p] = Z.gq[ig][s] / Z.epsq * kR4pie0;
+          D.pq[p] = set ? Z.gq[ig][s] : -Z.gq[ig][s];
+          D.pts[p] = Z.gtype[ig][s];
+          D.psg[p] = (kind == FMC_DISPLACE && set == 0) ? -1 : 1;
         }
       }
       __syncthreads();
+      // pairtab rows of the positions' site types
+      const int ntp = Y.S.ntpatm;
+      const bool ldsrow = ntp <= kLdsTypes;
+      if (ldsrow)
+        for (int t = tid; t < np * ntp; t += BT) {
+          const int p = t / ntp, tj = t - p * ntp;
+          D.ptab[p][tj] = static_cast<short>(Y.S.pairtab[D.pts[p] * ntp + tj]);
+        }
       // X[l] * Y[m] for every (l, m) and position (m < 0 conjugates Y)
       {
-        const int w2 = 2 * Y.kmax[1] + 1;
+        const int w2 = 2 * Z.kmax[1] + 1;
         for (int t = tid; t < np * nlm; t += BT) {
           const int p = t / nlm, r = t - p * nlm;
-          const int l = r / w2, m = r - l * w2 - Y.kmax[1];
+          const int l = r / w2, m = r - l * w2 - Z.kmax[1];
           const Cplx<double> a = tab[p][0][l];
           const Cplx<double> b = tab[p][1][m < 0 ? -m : m];
           double ox, oy;
@@ -713,20 +748,25 @@ __global__ void __launch_bounds__(BT)
             ox = a.x * b.x + a.y * b.y;
             oy = a.y * b.x - b.y * a.x;
           }
-          xy_l[t] = {static_cast<T>(ox), static_cast<T>(oy)};
+          xy_l[t] = {static_cast<T>(D.pq[p] * ox), static_cast<T>(D.pq[p] * oy)};
+        }
+        const int k3 = Z.kmax[2], nz = 2 * k3 + 1;
+        for (int t = tid; t < np * nz; t += BT) {
+          const int p = t / nz, n = t - p * nz - k3;
+          const Cplx<double> c = tab[p][2][n < 0 ? -n : n];
+          zt[p * kZRow + n + k3] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
         }
       }
       __syncthreads();
       // ---- reciprocal space (ewald1_guest, ewald.f:141-319) ----
       // dS(k) = sum_new q e^{ik.r} - sum_old q e^{ik.r}
       if (STREAM) {
-        const int w2 = 2 * Y.kmax[1] + 1;
-        const int k1 = min(k0 + kpt, nk);
-        for (int k = k0; k < k1; ++k) {
+        const int w2 = 2 * Z.kmax[1] + 1;
+        for (int k = tid; k < nk; k += BT) {
           const int v = Y.kv[k];
-          const int lm = ul(v) * w2 + um(v) + Y.kmax[1];
+          const int lm = ul(v) * w2 + um(v) + Z.kmax[1];
           double dr, di;
-          trial_ds<T>(Y, D, ig, np, xy_l, nlm, lm, un(v), tab, dr, di);
+          trial_ds<T>(np, xy_l, nlm, lm, zt, un(v) + Z.kmax[2], dr, di);
           const double s0 = Cg.sre[k], s1 = Cg.sim[k];
           Cg.dsre[k] = dr;
           Cg.dsim[k] = di;
@@ -735,15 +775,16 @@ __global__ void __launch_bounds__(BT)
       } else {
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
-          if (j >= kpt || k0 + j >= nk) break;
-          const int v = kv_l[k0 + j];
+          const int k = tid + j * BT;
+          if (j >= kpt || k >= nk) break;
+          const int v = kv_l[k];
           const int lm = v & 0xffff;
           const int n = static_cast<signed char>((v >> 16) & 0xff);
           double dr, di;
-          trial_ds<T>(Y, D, ig, np, xy_l, nlm, lm, n, tab, dr, di);
+          trial_ds<T>(np, xy_l, nlm, lm, zt, n + Z.kmax[2], dr, di);
           nre[j] = sre[j] + dr;
           nim[j] = sim[j] + di;
-          e1 += de_k(ak_l[k0 + j], sre[j], sim[j], dr, di);
+          e1 += de_k(ak_l[k], sre[j], sim[j], dr, di);
         }
       }
       if (tid == 0) {
@@ -763,49 +804,75 @@ __global__ void __launch_bounds__(BT)
         const int total = nfwp + np * ng;
         const int own0 = (kind != FMC_INSERT) ? D.gpre[ig] + D.imol * ns : -1;
         double a2 = 0, av = 0;
-        for (int idx = tid; idx < total; idx += BT) {
-          int p;
-          double ax, ay, az, aq;
-          int at;
-          if (idx < nfwp) {
-            p = 0;
-            while (p + 1 < np && D.ppre[p + 1] <= idx) ++p;
-            const int a = Y.gidx[D.plo[p] + (idx - D.ppre[p])];
-            ax = Y.S.fx[a]; ay = Y.S.fy[a]; az = Y.S.fz[a]; aq = Y.S.fq[a];
-            at = Y.S.ftype[a];
-          } else {
-            const int r = idx - nfwp;
-            p = r / ng;
-            const int a = r - p * ng;
-            if (a >= own0 && a < own0 + ns && own0 >= 0) continue;
-            int g = 0;
-            while (g + 1 < Y.ngt && D.gpre[g + 1] <= a) ++g;
-            const int la = a - D.gpre[g];
-            const int sa = la % Y.ns[g];
-            ax = Cg.gx[g][la]; ay = Cg.gy[g][la]; az = Cg.gz[g][la];
-            aq = Y.gq[g][sa];
-            at = Y.gtype[g][sa];
+        // kU items per thread per pass: their index loads, then their
+        // coordinate loads, issue together (one memory latency per pass)
+        constexpr int kU = 4;
+        for (int base = tid; base < total; base += kU * BT) {
+          int pp[kU], aa[kU];
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            const int idx = base + u * BT;
+            pp[u] = -1;
+            aa[u] = 0;
+            if (idx < nfwp) {
+              int p = 0;
+              while (p + 1 < np && D.ppre[p + 1] <= idx) ++p;
+              pp[u] = p;
+              aa[u] = Y.gidx[D.plo[p] + (idx - D.ppre[p])];
+            } else if (idx < total) {
+              const int r = idx - nfwp;
+              const int p = r / ng;
+              const int a = r - p * ng;
+              if (!(own0 >= 0 && a >= own0 && a < own0 + ns)) {
+                pp[u] = p;
+                aa[u] = -1 - a;   // guest atom a
+              }
+            }
           }
-          const int set = D.pset[p], s = D.psite[p];
-          const double px = set ? D.nx[s] : D.ox[s];
-          const double py = set ? D.ny[s] : D.oy[s];
-          const double pz = set ? D.nz[s] : D.oz[s];
-          T b2 = 0, bv = 0;
-          if constexpr (MIXED) {
-            const float ce = static_cast<float>(Y.gq[ig][s] / Y.S.epsq * kR4pie0);
-            pair_mixed(Y, ce, Y.gtype[ig][s], px - ax, py - ay, pz - az,
-                       static_cast<float>(aq), at, b2, bv);
-          } else {
-            const double ce = Y.gq[ig][s] / Y.S.epsq * kR4pie0;
-            pair_fp64(Y.S, ce, Y.gtype[ig][s], px - ax, py - ay, pz - az, aq, at,
-                      b2, bv);
+          double ax[kU], ay[kU], az[kU], aq[kU];
+          int at[kU];
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            ax[u] = ay[u] = az[u] = aq[u] = 0.0;
+            at[u] = 0;
+            if (pp[u] < 0) continue;
+            if (aa[u] >= 0) {
+              const int a = aa[u];
+              ax[u] = Y.S.fx[a]; ay[u] = Y.S.fy[a]; az[u] = Y.S.fz[a];
+              aq[u] = Y.S.fq[a];
+              at[u] = Y.S.ftype[a];
+            } else {
+              const int a = -1 - aa[u];
+              int g = 0;
+              while (g + 1 < Z.ngt && D.gpre[g + 1] <= a) ++g;
+              const int la = a - D.gpre[g];
+              const int sa = la % Z.ns[g];
+              ax[u] = Cg.gx[g][la]; ay[u] = Cg.gy[g][la]; az[u] = Cg.gz[g][la];
+              aq[u] = Z.gq[g][sa];
+              at[u] = Z.gtype[g][sa];
+            }
           }
-          if (kind == FMC_DISPLACE && set == 0) {
-            a2 -= static_cast<double>(b2);
-            av -= static_cast<double>(bv);
-          } else {
-            a2 += static_cast<double>(b2);
-            av += static_cast<double>(bv);
+#pragma unroll
+          for (int u = 0; u < kU; ++u) {
+            if (pp[u] < 0) continue;
+            const int p = pp[u];
+            T b2 = 0, bv = 0;
+            const int vk = ldsrow ? D.ptab[p][at[u]] : Y.S.pairtab[D.pts[p] * ntp + at[u]];
+            if constexpr (MIXED) {
+              pair_mixed(Y, static_cast<float>(D.pce[p]), vk, D.ppx[p] - ax[u],
+                         D.ppy[p] - ay[u], D.ppz[p] - az[u], static_cast<float>(aq[u]), b2,
+                         bv);
+            } else {
+              pair_fp64_vk(Y.S, D.pce[p], vk, D.ppx[p] - ax[u], D.ppy[p] - ay[u],
+                           D.ppz[p] - az[u], aq[u], b2, bv);
+            }
+            if (D.psg[p] < 0) {
+              a2 -= static_cast<double>(b2);
+              av -= static_cast<double>(bv);
+            } else {
+              a2 += static_cast<double>(b2);
+              av += static_cast<double>(bv);
+            }
           }
         }
         e2 = a2;
@@ -829,8 +896,8 @@ __global__ void __launch_bounds__(BT)
         b += red[1][w];
         c += red[2][w];
       }
-      a = Y.rvolm2 * kR4pie0 * a / Y.S.epsq;
-      decide(Y, C, R, D, a, b, c);
+      a = Z.rvolm2 * kR4pie0 * a / Z.epsq;
+      decide(Z, C, R, D, a, b, c);
       shift_lo = 0;
       shift_hi = 0;
       if (D.accepted && kind == FMC_DELETE) {
@@ -842,15 +909,15 @@ __global__ void __launch_bounds__(BT)
     if (D.accepted) {
       // S(k) <- S_new (ewald.f:260-293, gcmc.f:971-974)
       if (STREAM) {
-        const int k1 = min(k0 + kpt, nk);
-        for (int k = k0; k < k1; ++k) {
+        for (int k = tid; k < nk; k += BT) {
           Cg.sre[k] = Cg.sre[k] + Cg.dsre[k];
           Cg.sim[k] = Cg.sim[k] + Cg.dsim[k];
         }
       } else {
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
-          if (j < kpt) {
+          const int k = tid + j * BT;
+          if (j < kpt && k < nk) {
             sre[j] = nre[j];
             sim[j] = nim[j];
           }
@@ -894,7 +961,7 @@ __global__ void __launch_bounds__(BT)
   if (!STREAM) {
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
-      const int k = k0 + j;
+      const int k = tid + j * BT;
       if (j < kpt && k < nk) {
         Cg.sre[k] = sre[j];
         Cg.sim[k] = sim[j];
@@ -1034,14 +1101,14 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
   const BSys& Y = P.one ? *P.one : P.sys[C.sys];
   if (tid == 0) {
     int na = 0;
-    for (int g = 0; g < Y.ngt; ++g) {
+    for (int g = 0; g < Y.q.ngt; ++g) {
       gpre[g] = na;
-      na += C.nmol[g] * Y.ns[g];
+      na += C.nmol[g] * Y.q.ns[g];
     }
-    gpre[Y.ngt] = na;
+    gpre[Y.q.ngt] = na;
   }
   __syncthreads();
-  const int nga = gpre[Y.ngt];
+  const int nga = gpre[Y.q.ngt];
   const double* rc = Y.S.rcell;
   // reciprocal space
   double erec = 0.0, dev = 0.0;
@@ -1050,8 +1117,8 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
     const int l = ul(v), m = um(v), n = un(v);
     const double fr = Y.sfw_re[k], fi = Y.sfw_im[k];
     double sx = fr, sy = fi;
-    for (int g = 0; g < Y.ngt; ++g) {
-      const int nat = C.nmol[g] * Y.ns[g];
+    for (int g = 0; g < Y.q.ngt; ++g) {
+      const int nat = C.nmol[g] * Y.q.ns[g];
       for (int a = 0; a < nat; ++a) {
         const double x = C.gx[g][a], y = C.gy[g][a], z = C.gz[g][a];
         double ph = l * (rc[0] * x + rc[3] * y + rc[6] * z) +
@@ -1060,7 +1127,7 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
         ph -= rint(ph);
         double sv, cv;
         sincospi(2.0 * ph, &sv, &cv);
-        const double q = Y.gq[g][a % Y.ns[g]];
+        const double q = Y.q.gq[g][a % Y.q.ns[g]];
         sx += q * cv;
         sy += q * sv;
       }
@@ -1076,14 +1143,14 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
   double e2 = 0.0, ev = 0.0;
   for (int i = 0; i < nga; ++i) {
     int g = 0;
-    while (g + 1 < Y.ngt && gpre[g + 1] <= i) ++g;
+    while (g + 1 < Y.q.ngt && gpre[g + 1] <= i) ++g;
     const int la = i - gpre[g];
-    const int s = la % Y.ns[g];
-    const int mol = la / Y.ns[g];
+    const int s = la % Y.q.ns[g];
+    const int mol = la / Y.q.ns[g];
     const double xi = C.gx[g][la], yi = C.gy[g][la], zi = C.gz[g][la];
-    const double ce = Y.gq[g][s] / Y.S.epsq * kR4pie0;
-    const int ts = Y.gtype[g][s];
-    const int c = cell_of(Y, xi, yi, zi);
+    const double ce = Y.q.gq[g][s] / Y.S.epsq * kR4pie0;
+    const int ts = Y.q.gtype[g][s];
+    const int c = cell_of(Y.q, xi, yi, zi);
     const int lo = Y.goff[c], len = Y.goff[c + 1] - lo;
     for (int t = tid; t < len; t += kRecBT) {
       const int a = Y.gidx[lo + t];
@@ -1092,12 +1159,12 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
     }
     for (int j = i + 1 + tid; j < nga; j += kRecBT) {
       int h = 0;
-      while (h + 1 < Y.ngt && gpre[h + 1] <= j) ++h;
+      while (h + 1 < Y.q.ngt && gpre[h + 1] <= j) ++h;
       const int lb = j - gpre[h];
-      if (h == g && lb / Y.ns[h] == mol) continue;
-      const int sb = lb % Y.ns[h];
+      if (h == g && lb / Y.q.ns[h] == mol) continue;
+      const int sb = lb % Y.q.ns[h];
       pair_fp64(Y.S, ce, ts, xi - C.gx[h][lb], yi - C.gy[h][lb], zi - C.gz[h][lb],
-                Y.gq[h][sb], Y.gtype[h][sb], e2, ev);
+                Y.q.gq[h][sb], Y.q.gtype[h][sb], e2, ev);
     }
   }
   erec = wave_sum(erec);
@@ -1119,17 +1186,17 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
       cc += red[2][w];
       d = fmax(d, red[3][w]);
     }
-    double e = Y.rvolm2 * kR4pie0 * a / Y.S.epsq;
-    for (int g = 0; g < Y.ngt; ++g) e += C.nmol[g] * (Y.engsic[g] + Y.e3[g]);
+    double e = Y.q.rvolm2 * kR4pie0 * a / Y.S.epsq;
+    for (int g = 0; g < Y.q.ngt; ++g) e += C.nmol[g] * (Y.q.engsic[g] + Y.q.e3[g]);
     e += b + cc;
     // the move loop's tail bookkeeping starts from elrc = 0 and adds
     // delrc = tail(N +/- 1) - elrc (gcmc.f:891-901): compare the running
     // sum with the same tail it carries, report the true one in e_full
     const double book = (e + C.elrc) / C.engunit;
-    C.e_full = (e + tail_of(Y, C.nmol)) / C.engunit;
+    C.e_full = (e + tail_of(Y.q, C.nmol)) / C.engunit;
     if (P.base) {
       C.e_acc = book;
-      for (int g = 0; g < Y.ngt; ++g) C.energy[g] = C.e_full;
+      for (int g = 0; g < Y.q.ngt; ++g) C.energy[g] = C.e_full;
     }
     C.drift = C.e_acc - book;
     C.sk_dev = d;
@@ -1341,9 +1408,8 @@ struct KCfg {
       {BT_, 0, true, batch_kernel<BT_, 1, true, true>}
 
 const KCfg kConfigs[] = {
-    KC(256, 8),  KC(256, 16), KC(256, 28), KC(512, 4),  KC(512, 8),
-    KC(512, 14), KC(1024, 2), KC(1024, 4), KC(1024, 7), KC(1024, 10),
-    KS(256),     KS(512),
+    KC(256, 8), KC(256, 16), KC(256, 28), KC(512, 4), KC(512, 8), KC(512, 14),
+    KS(256),    KS(512),
 };
 
 const KCfg* find_cfg(int bt, int kpt, bool mixed) {
@@ -1381,6 +1447,9 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   S.imcon = s.imcon;
   std::memcpy(S.cell, s.cell, sizeof(S.cell));
   std::memcpy(S.rcell, s.rcell, sizeof(S.rcell));
+  std::memcpy(Y.q.cell, s.cell, sizeof(Y.q.cell));
+  std::memcpy(Y.q.rcell, s.rcell, sizeof(Y.q.rcell));
+  Y.q.epsq = s.epsq;
   S.inv[0] = 1.0 / s.cell[0];
   S.inv[1] = 1.0 / s.cell[4];
   S.inv[2] = 1.0 / s.cell[8];
@@ -1423,11 +1492,11 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   Y.nk = nk;
   Y.kv = kv;
   Y.ak = ak;
-  for (int d = 0; d < 3; ++d) Y.kmax[d] = s.kmax[d];
+  for (int d = 0; d < 3; ++d) Y.q.kmax[d] = s.kmax[d];
   if (s.kmax[0] >= kBTab || s.kmax[1] >= kBTab || s.kmax[2] >= kBTab)
     return FMC_ERR_UNSUPPORTED;
-  Y.volm = s.volm;
-  Y.rvolm2 = 2.0 * (2.0 * kPi / s.volm);
+  Y.q.volm = s.volm;
+  Y.q.rvolm2 = 2.0 * (2.0 * kPi / s.volm);
   // framework S(k)
   double *sre, *sim, *rc;
   if ((r = dput(K, &sre, static_cast<double*>(nullptr), nk)) ||
@@ -1442,23 +1511,23 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   // candidate grid: cells of ~1.5 A along each perpendicular width
   double p[10];
   dcell(s.cell, p);
-  for (int d = 0; d < 3; ++d) Y.gdim[d] = std::max(1, static_cast<int>(p[6 + d] / 1.5));
-  while (static_cast<long long>(Y.gdim[0]) * Y.gdim[1] * Y.gdim[2] > 250000)
-    for (int d = 0; d < 3; ++d) Y.gdim[d] = std::max(1, Y.gdim[d] * 4 / 5);
-  const int ncell = Y.gdim[0] * Y.gdim[1] * Y.gdim[2];
+  for (int d = 0; d < 3; ++d) Y.q.gdim[d] = std::max(1, static_cast<int>(p[6 + d] / 1.5));
+  while (static_cast<long long>(Y.q.gdim[0]) * Y.q.gdim[1] * Y.q.gdim[2] > 250000)
+    for (int d = 0; d < 3; ++d) Y.q.gdim[d] = std::max(1, Y.q.gdim[d] * 4 / 5);
+  const int ncell = Y.q.gdim[0] * Y.q.gdim[1] * Y.q.gdim[2];
   double hd = 0.0;
   for (int sa = -1; sa <= 1; sa += 2)
     for (int sb = -1; sb <= 1; sb += 2) {
       double v[3];
       for (int d = 0; d < 3; ++d)
-        v[d] = (s.cell[d] / Y.gdim[0] + sa * s.cell[3 + d] / Y.gdim[1] +
-                sb * s.cell[6 + d] / Y.gdim[2]) * 0.5;
+        v[d] = (s.cell[d] / Y.q.gdim[0] + sa * s.cell[3 + d] / Y.q.gdim[1] +
+                sb * s.cell[6 + d] / Y.q.gdim[2]) * 0.5;
       hd = std::max(hd, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
     }
   if (nfw >= 65536) return FMC_ERR_CAPACITY;
   GridParams G;
   G.S = S;
-  for (int d = 0; d < 3; ++d) G.gdim[d] = Y.gdim[d];
+  for (int d = 0; d < 3; ++d) G.gdim[d] = Y.q.gdim[d];
   const double reach = s.rcut + hd + 1e-6;
   if (reach >= std::min(p[6], std::min(p[7], p[8]))) return FMC_ERR_CELL;
   G.reach2 = reach * reach;
@@ -1484,31 +1553,31 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   Y.goff = off;
   Y.gidx = idx;
   // guests
-  Y.ngt = static_cast<int>(s.guests.size());
-  for (int g = 0; g < Y.ngt; ++g) {
+  Y.q.ngt = static_cast<int>(s.guests.size());
+  for (int g = 0; g < Y.q.ngt; ++g) {
     const GuestTemplate& t = s.guests[g];
-    Y.ns[g] = t.ns;
+    Y.q.ns[g] = t.ns;
     for (int i = 0; i < t.ns; ++i) {
-      Y.gq[g][i] = t.q[i];
-      Y.gtype[g][i] = t.type[i] - 1;
-      Y.gmass[g][i] = t.mass[i];
-      Y.tx[g][i] = t.x[i];
-      Y.ty[g][i] = t.y[i];
-      Y.tz[g][i] = t.z[i];
+      Y.q.gq[g][i] = t.q[i];
+      Y.q.gtype[g][i] = t.type[i] - 1;
+      Y.q.gmass[g][i] = t.mass[i];
+      Y.q.tx[g][i] = t.x[i];
+      Y.q.ty[g][i] = t.y[i];
+      Y.q.tz[g][i] = t.z[i];
     }
-    Y.e3[g] = t.e3;
-    Y.engsic[g] = t.engsic;
+    Y.q.e3[g] = t.e3;
+    Y.q.engsic[g] = t.engsic;
   }
   // tail as a quadratic form in the molecule counts
   {
     const int T = s.ntpatm;
     std::vector<double> F(T, 0.0), Z(T, 0.0);
-    std::vector<std::vector<double>> cg(Y.ngt, std::vector<double>(T, 0.0));
+    std::vector<std::vector<double>> cg(Y.q.ngt, std::vector<double>(T, 0.0));
     for (int i = 0; i < nfw; ++i) {
       F[s.ftype[i] - 1] += 1.0;
       if (s.ffrz[i]) Z[s.ftype[i] - 1] += 1.0;
     }
-    for (int g = 0; g < Y.ngt; ++g)
+    for (int g = 0; g < Y.q.ngt; ++g)
       for (int i = 0; i < s.guests[g].ns; ++i) cg[g][s.guests[g].type[i] - 1] += 1.0;
     double t0 = 0.0, tb[kMaxGuestTypes] = {0}, tA[kMaxGuestTypes][kMaxGuestTypes] = {{0}};
     const bool on = !(s.imcon == 0 || s.imcon == 6 || s.ntpvdw == 0);
@@ -1518,15 +1587,15 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
         if (!on) continue;
         const double w = 2.0 * kPi * s.steadd[iv] / s.volm;
         t0 += w * (F[i] * F[j] - Z[i] * Z[j]);
-        for (int g = 0; g < Y.ngt; ++g) {
+        for (int g = 0; g < Y.q.ngt; ++g) {
           tb[g] += w * (F[i] * cg[g][j] + F[j] * cg[g][i]);
-          for (int h = 0; h < Y.ngt; ++h) tA[g][h] += w * cg[g][i] * cg[h][j];
+          for (int h = 0; h < Y.q.ngt; ++h) tA[g][h] += w * cg[g][i] * cg[h][j];
         }
       }
-    Y.t0 = t0;
-    for (int g = 0; g < Y.ngt; ++g) {
-      Y.tb[g] = tb[g];
-      for (int h = 0; h < Y.ngt; ++h) Y.tA[g][h] = tA[g][h];
+    Y.q.t0 = t0;
+    for (int g = 0; g < Y.q.ngt; ++g) {
+      Y.q.tb[g] = tb[g];
+      for (int h = 0; h < Y.q.ngt; ++h) Y.q.tA[g][h] = tA[g][h];
     }
   }
   if ((r = dput(K, &sh->d_self, &Y, 1))) return r;
@@ -1672,7 +1741,7 @@ int snapshot(fmc_batch* b, Snapshot* s, bool restore) {
     const BSys& Y = b->systems[C.sys]->dev;
     std::vector<std::pair<double*, size_t>> bufs = {{C.sre, (size_t)Y.nk},
                                                     {C.sim, (size_t)Y.nk}};
-    for (int g = 0; g < Y.ngt; ++g) {
+    for (int g = 0; g < Y.q.ngt; ++g) {
       bufs.push_back({C.gx[g], (size_t)C.cap_atoms});
       bufs.push_back({C.gy[g], (size_t)C.cap_atoms});
       bufs.push_back({C.gz[g], (size_t)C.cap_atoms});
@@ -1695,7 +1764,7 @@ int snapshot(fmc_batch* b, Snapshot* s, bool restore) {
 // Project autotuner: time each block size (with the smallest register tile
 // that covers the largest Nk) on real moves, keep the fastest, restore state.
 int autotune(fmc_batch* b) {
-  std::vector<int> bts = {256, 512, 1024};
+  std::vector<int> bts = {256, 512};
   Snapshot snap;
   int r;
   if ((r = snapshot(b, &snap, false))) return r;
@@ -1931,8 +2000,7 @@ extern "C" int fmc_batch_set_tile(fmc_batch* b, int threads) {
   }
   const bool stream = (threads & FMC_TILE_STREAM) != 0;
   const int bt = threads & ~FMC_TILE_STREAM;
-  if (bt != 256 && bt != 512 && bt != 1024) return FMC_ERR_UNSUPPORTED;
-  if (stream && bt == 1024) return FMC_ERR_UNSUPPORTED;
+  if (bt != 256 && bt != 512) return FMC_ERR_UNSUPPORTED;
   b->bt = bt;
   b->kpt = 0;
   b->tuned = true;

@@ -87,6 +87,21 @@ __device__ __forceinline__ void pair_fp64(const DevSys& S, double chgea,
                   S.rdlrpot, imax);
 }
 
+// The same pair with its vdW table index vk (pairtab, -1 for none) given.
+__device__ __forceinline__ void pair_fp64_vk(const DevSys& S, double chgea, int vk,
+                                             double dx, double dy, double dz, double qj,
+                                             double& e2, double& ev) {
+  min_image(S, dx, dy, dz);
+  const double rsq = dx * dx + dy * dy + dz * dz;
+  if (!(S.rcut2 > rsq)) return;
+  const double r = sqrt(rsq);
+  const double chgprd = chgea * qj;
+  const int imax = S.mxgrid - 2;
+  if (fabs(chgprd) > 1e-10) e2 += interp3(S.erc, r, S.rdrewd, imax) * chgprd;
+  if (vk >= 0)
+    ev += interp3(S.vvv + static_cast<size_t>(vk) * (S.mxgrid + 1), r, S.rdlrpot, imax);
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);

@@ -230,7 +230,7 @@ int fmc_batch_poll(fmc_batch *b, int *ids, int cap, int *n);
  * with HIP events on the batch's stream (one launch = one fmc_batch_launch
  * completed by fmc_batch_sync); reset != 0 clears the totals */
 int fmc_batch_timing(fmc_batch *b, int reset, double *ms, long long *launches);
-/* tile shape: threads per chain work-group (256, 512 or 1024); 0 =
+/* tile shape: threads per chain work-group (256 or 512); 0 =
  * autotuned.  S(k) is register-resident when the largest Nk fits the tile,
  * else the streaming form runs (S(k) in HBM/L2, any Nk); threads |
  * FMC_TILE_STREAM forces the streaming form (256 or 512).  Results do not
