@@ -205,6 +205,17 @@ def serving_leg(n_req=400, rate=200.0, moves=5000):
             "makespan_s": wall, "slices": srv.slices}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the batch kernel from the committed
+    FETCH_SIZE/WRITE_SIZE counter runs (scaled by a plain copy kernel's
+    reading), or None when they are not there."""
+    p = os.path.join(ROOT, "profiles", "batch_kernel_256chains_co2_mixed_pmc.md")
+    if not os.path.exists(p):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(p).read())
+    return float(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +227,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run a plain 256 MiB device copy (the reference "
+                         "kernel for scaling FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one MI355X; chains shard by chain "
@@ -224,6 +238,16 @@ def main():
         sys.exit(2)
 
     from fastmc_amd.capi import PREC_MIXED
+
+    if args.copy_calibration:
+        import torch
+        n = 256 << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda:0").fill_(1)
+        c = torch.empty_like(a)
+        for _ in range(3):
+            c.copy_(a)
+        torch.cuda.synchronize()
+        del a, c
 
     cpu = None if args.no_cpu else cpu_baseline()
 
@@ -306,7 +330,7 @@ def main():
         "kernel_ms_per_launch": kernel_ms / max(1, launches),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None},
+                     "traffic": measured_traffic()},
         "valu_fp32": {"achieved": valu, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                       "frac": valu / PEAK_FP32_TFLOPS},
         "cpu_baseline": cpu,

@@ -66,6 +66,10 @@ struct SysQ {
   double t0, tb[kMaxGuestTypes], tA[kMaxGuestTypes][kMaxGuestTypes];
 };
 
+struct alignas(16) Rec {
+  double x, y, z, q;
+};
+
 struct BSys {
   DevSys S;
   int nk;
@@ -73,8 +77,13 @@ struct BSys {
   const double* ak;
   const double* sfw_re;   // framework S(k) (constant)
   const double* sfw_im;
-  const int* goff;        // [ncell+1]
-  const unsigned short* gidx;
+  // candidate cells: cell c lists, in ascending atom order, every framework
+  // atom that can lie within rcut of a point in it, as records
+  // rec[c*gcap + i] (x, y, z, q) and rtype[c*gcap + i]; unused slots hold
+  // NaN coordinates, which every pair test skips
+  const Rec* grec;
+  const int* grec_t;
+  int gcap;
   const float* erc32;
   const float* vvv32;
   SysQ q;
@@ -335,6 +344,16 @@ __device__ __forceinline__ double de_k(double ak, double s0, double s1, double d
   const double sd = fmat(s0, dr, s1 * di);
   const double dd = fmat(dr, dr, di * di);
   return ak * fmat(2.0, sd, dd);
+}
+
+// a / d for 0 <= a < 2^22 and 0 < d < 2^11 from rd = 1/d: the float
+// quotient is within one of the true one, and one correction step fixes it
+// (a runtime integer division costs tens of instructions)
+__device__ __forceinline__ int div_rc(int a, int d, float rd) {
+  int q = static_cast<int>(static_cast<float>(a) * rd);
+  const int r = a - q * d;
+  q += (r >= d) - (r < 0);
+  return q;
 }
 
 __device__ __forceinline__ int ul(int v) { return static_cast<signed char>(v & 0xff); }
@@ -660,8 +679,8 @@ __global__ void __launch_bounds__(BT)
     for (int j = 0; j < KPT; ++j) {
       const int k = tid + j * BT;
       const bool in = j < kpt && k < nk;
-      sre[j] = in ? Cg.sre[k] : 0.0;
-      sim[j] = in ? Cg.sim[k] : 0.0;
+      sre[j] = in ? C.sre[k] : 0.0;
+      sim[j] = in ? C.sim[k] : 0.0;
     }
   }
   Rng R;
@@ -711,9 +730,8 @@ __global__ void __launch_bounds__(BT)
           const int set = D.pset[p], s = D.psite[p];
           const int c = set ? cell_of(Z, D.nx[s], D.ny[s], D.nz[s])
                             : cell_of(Z, D.ox[s], D.oy[s], D.oz[s]);
-          const int lo = Y.goff[c];
-          D.plo[p] = lo;
-          D.plen[p] = Y.goff[c + 1] - lo;
+          D.plo[p] = c * Y.gcap;
+          D.plen[p] = Y.gcap;
           D.ppx[p] = set ? D.nx[s] : D.ox[s];
           D.ppy[p] = set ? D.ny[s] : D.oy[s];
           D.ppz[p] = set ? D.nz[s] : D.oz[s];
@@ -726,18 +744,20 @@ __global__ void __launch_bounds__(BT)
       __syncthreads();
       // pairtab rows of the positions' site types
       const int ntp = Y.S.ntpatm;
+      const float rntp = 1.0f / ntp;
       const bool ldsrow = ntp <= kLdsTypes;
       if (ldsrow)
         for (int t = tid; t < np * ntp; t += BT) {
-          const int p = t / ntp, tj = t - p * ntp;
+          const int p = div_rc(t, ntp, rntp), tj = t - p * ntp;
           D.ptab[p][tj] = static_cast<short>(Y.S.pairtab[D.pts[p] * ntp + tj]);
         }
       // X[l] * Y[m] for every (l, m) and position (m < 0 conjugates Y)
       {
         const int w2 = 2 * Z.kmax[1] + 1;
+        const float rnlm = 1.0f / nlm, rw2 = 1.0f / w2;
         for (int t = tid; t < np * nlm; t += BT) {
-          const int p = t / nlm, r = t - p * nlm;
-          const int l = r / w2, m = r - l * w2 - Z.kmax[1];
+          const int p = div_rc(t, nlm, rnlm), r = t - p * nlm;
+          const int l = div_rc(r, w2, rw2), m = r - l * w2 - Z.kmax[1];
           const Cplx<double> a = tab[p][0][l];
           const Cplx<double> b = tab[p][1][m < 0 ? -m : m];
           double ox, oy;
@@ -752,7 +772,7 @@ __global__ void __launch_bounds__(BT)
         }
         const int k3 = Z.kmax[2], nz = 2 * k3 + 1;
         for (int t = tid; t < np * nz; t += BT) {
-          const int p = t / nz, n = t - p * nz - k3;
+          const int p = div_rc(t, nz, 1.0f / nz), n = t - p * nz - k3;
           const Cplx<double> c = tab[p][2][n < 0 ? -n : n];
           zt[p * kZRow + n + k3] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
         }
@@ -767,9 +787,9 @@ __global__ void __launch_bounds__(BT)
           const int lm = ul(v) * w2 + um(v) + Z.kmax[1];
           double dr, di;
           trial_ds<T>(np, xy_l, nlm, lm, zt, un(v) + Z.kmax[2], dr, di);
-          const double s0 = Cg.sre[k], s1 = Cg.sim[k];
-          Cg.dsre[k] = dr;
-          Cg.dsim[k] = di;
+          const double s0 = C.sre[k], s1 = C.sim[k];
+          C.dsre[k] = dr;
+          C.dsim[k] = di;
           e1 += de_k(Y.ak[k], s0, s1, dr, di);
         }
       } else {
@@ -799,8 +819,10 @@ __global__ void __launch_bounds__(BT)
       // ---- real space: framework candidates of every position's cell and
       //      every guest atom, one flattened loop (ewald2 + srfrce) ----
       {
-        const int nfwp = D.ppre[np];
+        const int gcap = Y.gcap;
+        const int nfwp = np * gcap;
         const int ng = D.nguest;
+        const float rgcap = 1.0f / gcap, rng = 1.0f / max(ng, 1);
         const int total = nfwp + np * ng;
         const int own0 = (kind != FMC_INSERT) ? D.gpre[ig] + D.imol * ns : -1;
         double a2 = 0, av = 0;
@@ -815,13 +837,12 @@ __global__ void __launch_bounds__(BT)
             pp[u] = -1;
             aa[u] = 0;
             if (idx < nfwp) {
-              int p = 0;
-              while (p + 1 < np && D.ppre[p + 1] <= idx) ++p;
+              const int p = div_rc(idx, gcap, rgcap);
               pp[u] = p;
-              aa[u] = Y.gidx[D.plo[p] + (idx - D.ppre[p])];
+              aa[u] = D.plo[p] + (idx - p * gcap);   // record slot
             } else if (idx < total) {
               const int r = idx - nfwp;
-              const int p = r / ng;
+              const int p = div_rc(r, ng, rng);
               const int a = r - p * ng;
               if (!(own0 >= 0 && a >= own0 && a < own0 + ns)) {
                 pp[u] = p;
@@ -837,17 +858,19 @@ __global__ void __launch_bounds__(BT)
             at[u] = 0;
             if (pp[u] < 0) continue;
             if (aa[u] >= 0) {
-              const int a = aa[u];
-              ax[u] = Y.S.fx[a]; ay[u] = Y.S.fy[a]; az[u] = Y.S.fz[a];
-              aq[u] = Y.S.fq[a];
-              at[u] = Y.S.ftype[a];
+              const Rec rr = Y.grec[aa[u]];
+              ax[u] = rr.x; ay[u] = rr.y; az[u] = rr.z;
+              aq[u] = rr.q;
+              at[u] = Y.grec_t[aa[u]];
             } else {
               const int a = -1 - aa[u];
               int g = 0;
               while (g + 1 < Z.ngt && D.gpre[g + 1] <= a) ++g;
               const int la = a - D.gpre[g];
-              const int sa = la % Z.ns[g];
-              ax[u] = Cg.gx[g][la]; ay[u] = Cg.gy[g][la]; az[u] = Cg.gz[g][la];
+              const int sa =
+                  la - div_rc(la, Z.ns[g], __builtin_amdgcn_rcpf(static_cast<float>(Z.ns[g]))) *
+                           Z.ns[g];
+              ax[u] = C.gx[g][la]; ay[u] = C.gy[g][la]; az[u] = C.gz[g][la];
               aq[u] = Z.gq[g][sa];
               at[u] = Z.gtype[g][sa];
             }
@@ -910,8 +933,8 @@ __global__ void __launch_bounds__(BT)
       // S(k) <- S_new (ewald.f:260-293, gcmc.f:971-974)
       if (STREAM) {
         for (int k = tid; k < nk; k += BT) {
-          Cg.sre[k] = Cg.sre[k] + Cg.dsre[k];
-          Cg.sim[k] = Cg.sim[k] + Cg.dsim[k];
+          C.sre[k] = C.sre[k] + C.dsre[k];
+          C.sim[k] = C.sim[k] + C.dsim[k];
         }
       } else {
 #pragma unroll
@@ -925,15 +948,15 @@ __global__ void __launch_bounds__(BT)
       }
       if (kind == FMC_INSERT) {
         if (tid < ns) {
-          Cg.gx[ig][D.slot + tid] = D.nx[tid];
-          Cg.gy[ig][D.slot + tid] = D.ny[tid];
-          Cg.gz[ig][D.slot + tid] = D.nz[tid];
+          C.gx[ig][D.slot + tid] = D.nx[tid];
+          C.gy[ig][D.slot + tid] = D.ny[tid];
+          C.gz[ig][D.slot + tid] = D.nz[tid];
         }
       } else if (kind == FMC_DISPLACE) {
         if (tid < ns) {
-          Cg.gx[ig][D.imol * ns + tid] = D.nx[tid];
-          Cg.gy[ig][D.imol * ns + tid] = D.ny[tid];
-          Cg.gz[ig][D.imol * ns + tid] = D.nz[tid];
+          C.gx[ig][D.imol * ns + tid] = D.nx[tid];
+          C.gy[ig][D.imol * ns + tid] = D.ny[tid];
+          C.gz[ig][D.imol * ns + tid] = D.nz[tid];
         }
       } else if (kind == FMC_DELETE) {
         // successors shift down by one molecule (gcmc.f:1127-1131)
@@ -941,15 +964,15 @@ __global__ void __launch_bounds__(BT)
           const int i = base + tid;
           double x = 0, y = 0, z = 0;
           if (i < shift_hi) {
-            x = Cg.gx[ig][i + ns];
-            y = Cg.gy[ig][i + ns];
-            z = Cg.gz[ig][i + ns];
+            x = C.gx[ig][i + ns];
+            y = C.gy[ig][i + ns];
+            z = C.gz[ig][i + ns];
           }
           __syncthreads();
           if (i < shift_hi) {
-            Cg.gx[ig][i] = x;
-            Cg.gy[ig][i] = y;
-            Cg.gz[ig][i] = z;
+            C.gx[ig][i] = x;
+            C.gy[ig][i] = y;
+            C.gz[ig][i] = z;
           }
           __syncthreads();
         }
@@ -963,8 +986,8 @@ __global__ void __launch_bounds__(BT)
     for (int j = 0; j < KPT; ++j) {
       const int k = tid + j * BT;
       if (j < kpt && k < nk) {
-        Cg.sre[k] = sre[j];
-        Cg.sim[k] = sim[j];
+        C.sre[k] = sre[j];
+        C.sim[k] = sim[j];
       }
     }
   }
@@ -988,8 +1011,9 @@ struct GridParams {
   int gdim[3];
   double reach2;   // (rcut + half the cell diagonal)^2
   int* count;
-  int* off;
-  unsigned short* idx;
+  Rec* rec;
+  int* rtype;
+  int cap;
 };
 
 __device__ __forceinline__ void grid_center(const GridParams& P, int c,
@@ -1042,14 +1066,17 @@ __global__ void __launch_bounds__(256) grid_count_kernel(GridParams P) {
   if (threadIdx.x == 0) P.count[c] = cnt;
 }
 
-// ordered fill: candidates of a cell in increasing atom index
+// ordered fill: candidates of a cell in increasing atom index, then NaN
+// records up to the capacity
 __global__ void __launch_bounds__(256) grid_fill_kernel(GridParams P) {
   __shared__ int scan[256];
   __shared__ int base;
   const int c = blockIdx.x;
   double cx, cy, cz;
   grid_center(P, c, cx, cy, cz);
-  if (threadIdx.x == 0) base = P.off[c];
+  Rec* rec = P.rec + static_cast<size_t>(c) * P.cap;
+  int* rt = P.rtype + static_cast<size_t>(c) * P.cap;
+  if (threadIdx.x == 0) base = 0;
   __syncthreads();
   for (int a0 = 0; a0 < P.S.nfw; a0 += 256) {
     const int a = a0 + threadIdx.x;
@@ -1064,10 +1091,19 @@ __global__ void __launch_bounds__(256) grid_fill_kernel(GridParams P) {
       scan[threadIdx.x] += v;
       __syncthreads();
     }
-    if (f) P.idx[base + scan[threadIdx.x] - 1] = static_cast<unsigned short>(a);
+    const int slot = base + scan[threadIdx.x] - 1;
+    if (f && slot < P.cap) {
+      rec[slot] = {P.S.fx[a], P.S.fy[a], P.S.fz[a], P.S.fq[a]};
+      rt[slot] = P.S.ftype[a];
+    }
     __syncthreads();
     if (threadIdx.x == 255) base += scan[255];
     __syncthreads();
+  }
+  const double nan = __builtin_nan("");
+  for (int i = base + threadIdx.x; i < P.cap; i += 256) {
+    rec[i] = {nan, nan, nan, 0.0};
+    rt[i] = 0;
   }
 }
 
@@ -1151,11 +1187,11 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
     const double ce = Y.q.gq[g][s] / Y.S.epsq * kR4pie0;
     const int ts = Y.q.gtype[g][s];
     const int c = cell_of(Y.q, xi, yi, zi);
-    const int lo = Y.goff[c], len = Y.goff[c + 1] - lo;
-    for (int t = tid; t < len; t += kRecBT) {
-      const int a = Y.gidx[lo + t];
-      pair_fp64(Y.S, ce, ts, xi - Y.S.fx[a], yi - Y.S.fy[a], zi - Y.S.fz[a], Y.S.fq[a],
-                Y.S.ftype[a], e2, ev);
+    const int lo = c * Y.gcap;
+    for (int t = tid; t < Y.gcap; t += kRecBT) {
+      const Rec rr = Y.grec[lo + t];
+      pair_fp64(Y.S, ce, ts, xi - rr.x, yi - rr.y, zi - rr.z, rr.q, Y.grec_t[lo + t], e2,
+                ev);
     }
     for (int j = i + 1 + tid; j < nga; j += kRecBT) {
       int h = 0;
@@ -1408,7 +1444,8 @@ struct KCfg {
       {BT_, 0, true, batch_kernel<BT_, 1, true, true>}
 
 const KCfg kConfigs[] = {
-    KC(256, 8), KC(256, 16), KC(256, 28), KC(512, 4), KC(512, 8), KC(512, 14),
+    KC(256, 8), KC(256, 16), KC(256, 28), KC(512, 4), KC(512, 8), KC(512, 12),
+    KC(512, 14),
     KS(256),    KS(512),
 };
 
@@ -1508,50 +1545,60 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   BCHECK(hipGetLastError());
   Y.sfw_re = sre;
   Y.sfw_im = sim;
-  // candidate grid: cells of ~1.5 A along each perpendicular width
+  // candidate grid: cells of >= 1.5 A along each perpendicular width, each
+  // with a fixed-capacity list of framework atom records (one read per
+  // candidate, no index indirection); the cell edge grows until the lists
+  // fit in kGridBytes
+  if (nfw >= (1 << 24)) return FMC_ERR_CAPACITY;
   double p[10];
   dcell(s.cell, p);
-  for (int d = 0; d < 3; ++d) Y.q.gdim[d] = std::max(1, static_cast<int>(p[6 + d] / 1.5));
-  while (static_cast<long long>(Y.q.gdim[0]) * Y.q.gdim[1] * Y.q.gdim[2] > 250000)
-    for (int d = 0; d < 3; ++d) Y.q.gdim[d] = std::max(1, Y.q.gdim[d] * 4 / 5);
-  const int ncell = Y.q.gdim[0] * Y.q.gdim[1] * Y.q.gdim[2];
-  double hd = 0.0;
-  for (int sa = -1; sa <= 1; sa += 2)
-    for (int sb = -1; sb <= 1; sb += 2) {
-      double v[3];
-      for (int d = 0; d < 3; ++d)
-        v[d] = (s.cell[d] / Y.q.gdim[0] + sa * s.cell[3 + d] / Y.q.gdim[1] +
-                sb * s.cell[6 + d] / Y.q.gdim[2]) * 0.5;
-      hd = std::max(hd, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
-    }
-  if (nfw >= 65536) return FMC_ERR_CAPACITY;
-  GridParams G;
-  G.S = S;
-  for (int d = 0; d < 3; ++d) G.gdim[d] = Y.q.gdim[d];
-  const double reach = s.rcut + hd + 1e-6;
-  if (reach >= std::min(p[6], std::min(p[7], p[8]))) return FMC_ERR_CELL;
-  G.reach2 = reach * reach;
-  int *cnt, *off;
-  if ((r = dput(K, &cnt, static_cast<int*>(nullptr), ncell)) ||
-      (r = dput(K, &off, static_cast<int*>(nullptr), ncell + 1)))
-    return r;
-  G.count = cnt;
-  hipLaunchKernelGGL(grid_count_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
-  BCHECK(hipGetLastError());
-  std::vector<int> hc(ncell), ho(ncell + 1, 0);
-  BCHECK(hipMemcpyAsync(hc.data(), cnt, ncell * sizeof(int), hipMemcpyDeviceToHost, b->setup));
-  BCHECK(hipStreamSynchronize(b->setup));
-  for (int c = 0; c < ncell; ++c) ho[c + 1] = ho[c] + hc[c];
-  BCHECK(hipMemcpy(off, ho.data(), (ncell + 1) * sizeof(int), hipMemcpyHostToDevice));
-  unsigned short* idx;
-  if ((r = dput(K, &idx, static_cast<unsigned short*>(nullptr), ho[ncell]))) return r;
-  G.off = off;
-  G.idx = idx;
-  hipLaunchKernelGGL(grid_fill_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
-  BCHECK(hipGetLastError());
-  BCHECK(hipStreamSynchronize(b->setup));
-  Y.goff = off;
-  Y.gidx = idx;
+  const size_t kGridBytes = size_t(512) << 20;
+  for (double edge = 1.5;; edge *= 1.25) {
+    for (int d = 0; d < 3; ++d) Y.q.gdim[d] = std::max(1, static_cast<int>(p[6 + d] / edge));
+    const int ncell = Y.q.gdim[0] * Y.q.gdim[1] * Y.q.gdim[2];
+    double hd = 0.0;
+    for (int sa = -1; sa <= 1; sa += 2)
+      for (int sb = -1; sb <= 1; sb += 2) {
+        double v[3];
+        for (int d = 0; d < 3; ++d)
+          v[d] = (s.cell[d] / Y.q.gdim[0] + sa * s.cell[3 + d] / Y.q.gdim[1] +
+                  sb * s.cell[6 + d] / Y.q.gdim[2]) * 0.5;
+        hd = std::max(hd, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+      }
+    GridParams G;
+    G.S = S;
+    for (int d = 0; d < 3; ++d) G.gdim[d] = Y.q.gdim[d];
+    const double reach = s.rcut + hd + 1e-6;
+    if (reach >= std::min(p[6], std::min(p[7], p[8]))) return FMC_ERR_CELL;
+    G.reach2 = reach * reach;
+    int* cnt;   // kept with the system (no free while a launch may run)
+    if ((r = dput(K, &cnt, static_cast<int*>(nullptr), ncell))) return r;
+    G.count = cnt;
+    hipLaunchKernelGGL(grid_count_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
+    BCHECK(hipGetLastError());
+    std::vector<int> hc(ncell);
+    BCHECK(hipMemcpyAsync(hc.data(), cnt, ncell * sizeof(int), hipMemcpyDeviceToHost,
+                          b->setup));
+    BCHECK(hipStreamSynchronize(b->setup));
+    const int cap = std::max(1, *std::max_element(hc.begin(), hc.end()));
+    const size_t bytes = static_cast<size_t>(ncell) * cap * (sizeof(Rec) + sizeof(int));
+    if (bytes > kGridBytes && ncell > 1) continue;
+    Rec* rec;
+    int* rt;
+    if ((r = dput(K, &rec, static_cast<Rec*>(nullptr), static_cast<size_t>(ncell) * cap)) ||
+        (r = dput(K, &rt, static_cast<int*>(nullptr), static_cast<size_t>(ncell) * cap)))
+      return r;
+    G.rec = rec;
+    G.rtype = rt;
+    G.cap = cap;
+    hipLaunchKernelGGL(grid_fill_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
+    BCHECK(hipGetLastError());
+    BCHECK(hipStreamSynchronize(b->setup));
+    Y.grec = rec;
+    Y.grec_t = rt;
+    Y.gcap = cap;
+    break;
+  }
   // guests
   Y.q.ngt = static_cast<int>(s.guests.size());
   for (int g = 0; g < Y.q.ngt; ++g) {
@@ -1874,7 +1921,15 @@ extern "C" int fmc_chain_footprint(const fmc_ctx* ctx, int precision, long long*
   b += static_cast<long long>(s.fx.size()) * (4 * 8 + 4);
   b += static_cast<long long>(s.vvv.size() + s.erc.size()) * 12;
   b += nk * (4 + 8 + 16);
-  b += static_cast<long long>(s.fx.size()) * 2 * 60;   // candidate lists (~60 cells/atom)
+  // candidate records: ~(V / 1.5^3) cells x (atoms within rcut + 1.3 A),
+  // at most 512 MiB
+  {
+    const double nfw = static_cast<double>(s.fx.size());
+    const double reach = s.rcut + 1.3;
+    const double cap = 1.2 * nfw / s.volm * 4.18879 * reach * reach * reach + 8.0;
+    const double cells = s.volm / 3.375;
+    b += static_cast<long long>(std::min(cells * cap * 36.0, 536870912.0));
+  }
   *bytes = b;
   return FMC_OK;
 }
