@@ -1552,7 +1552,7 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   if (nfw >= (1 << 24)) return FMC_ERR_CAPACITY;
   double p[10];
   dcell(s.cell, p);
-  const size_t kGridBytes = size_t(512) << 20;
+  const size_t kGridBytes = size_t(1) << 30;
   for (double edge = 1.5;; edge *= 1.25) {
     for (int d = 0; d < 3; ++d) Y.q.gdim[d] = std::max(1, static_cast<int>(p[6 + d] / edge));
     const int ncell = Y.q.gdim[0] * Y.q.gdim[1] * Y.q.gdim[2];
@@ -1922,13 +1922,13 @@ extern "C" int fmc_chain_footprint(const fmc_ctx* ctx, int precision, long long*
   b += static_cast<long long>(s.vvv.size() + s.erc.size()) * 12;
   b += nk * (4 + 8 + 16);
   // candidate records: ~(V / 1.5^3) cells x (atoms within rcut + 1.3 A),
-  // at most 512 MiB
+  // at most 1 GiB
   {
     const double nfw = static_cast<double>(s.fx.size());
     const double reach = s.rcut + 1.3;
     const double cap = 1.2 * nfw / s.volm * 4.18879 * reach * reach * reach + 8.0;
     const double cells = s.volm / 3.375;
-    b += static_cast<long long>(std::min(cells * cap * 36.0, 536870912.0));
+    b += static_cast<long long>(std::min(cells * cap * 36.0, 1073741824.0));
   }
   *bytes = b;
   return FMC_OK;
