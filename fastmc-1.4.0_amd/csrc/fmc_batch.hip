@@ -322,8 +322,7 @@ __device__ __forceinline__ double fmat(double a, double b, double c) {
 constexpr int kZRow = 2 * kBTab;   // Z table row: n = -16..16
 template <typename T>
 __device__ __forceinline__ void trial_ds(int np, const Cplx<T>* xy, int nlm, int lm,
-                                         const Cplx<T>* zt, int zoff, double& dr,
-                                         double& di) {
+                                         const Cplx<T>* zt, int zoff, T& dr, T& di) {
 #pragma clang fp contract(off)
   T ar = 0, ai = 0;
   for (int p = 0; p < np; ++p) {
@@ -334,8 +333,8 @@ __device__ __forceinline__ void trial_ds(int np, const Cplx<T>* xy, int nlm, int
     ai = fmat(a.y, c.x, ai);
     ai = fmat(a.x, c.y, ai);
   }
-  dr = static_cast<double>(ar);
-  di = static_cast<double>(ai);
+  dr = ar;
+  di = ai;
 }
 
 // a_k (|S + dS|^2 - |S|^2) = a_k (2 S.dS + |dS|^2): no cancellation of |S|^2
@@ -673,7 +672,7 @@ __global__ void __launch_bounds__(BT)
     }
   }
   double sre[KPT], sim[KPT];
-  double nre[KPT], nim[KPT];   // S + dS of the trial
+  T dre[KPT], dim[KPT];   // the trial's dS, as trial_ds summed it (exact)
   if (!STREAM) {
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
@@ -785,26 +784,30 @@ __global__ void __launch_bounds__(BT)
         for (int k = tid; k < nk; k += BT) {
           const int v = Y.kv[k];
           const int lm = ul(v) * w2 + um(v) + Z.kmax[1];
-          double dr, di;
-          trial_ds<T>(np, xy_l, nlm, lm, zt, un(v) + Z.kmax[2], dr, di);
+          T dr32, di32;
+          trial_ds<T>(np, xy_l, nlm, lm, zt, un(v) + Z.kmax[2], dr32, di32);
+          const double dr = dr32, di = di32;
           const double s0 = C.sre[k], s1 = C.sim[k];
           C.dsre[k] = dr;
           C.dsim[k] = di;
           e1 += de_k(Y.ak[k], s0, s1, dr, di);
         }
       } else {
+        // fully unrolled (predicated, no early exit) so that sre/sim/dre/dim
+        // keep static register indices
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
           const int k = tid + j * BT;
-          if (j >= kpt || k >= nk) break;
-          const int v = kv_l[k];
-          const int lm = v & 0xffff;
-          const int n = static_cast<signed char>((v >> 16) & 0xff);
-          double dr, di;
-          trial_ds<T>(np, xy_l, nlm, lm, zt, n + Z.kmax[2], dr, di);
-          nre[j] = sre[j] + dr;
-          nim[j] = sim[j] + di;
-          e1 += de_k(ak_l[k], sre[j], sim[j], dr, di);
+          dre[j] = 0;
+          dim[j] = 0;
+          if (j < kpt && k < nk) {
+            const int v = kv_l[k];
+            const int lm = v & 0xffff;
+            const int n = static_cast<signed char>((v >> 16) & 0xff);
+            trial_ds<T>(np, xy_l, nlm, lm, zt, n + Z.kmax[2], dre[j], dim[j]);
+            const double dr = dre[j], di = dim[j];
+            e1 += de_k(ak_l[k], sre[j], sim[j], dr, di);
+          }
         }
       }
       if (tid == 0) {
@@ -941,8 +944,8 @@ __global__ void __launch_bounds__(BT)
         for (int j = 0; j < KPT; ++j) {
           const int k = tid + j * BT;
           if (j < kpt && k < nk) {
-            sre[j] = nre[j];
-            sim[j] = nim[j];
+            sre[j] = sre[j] + static_cast<double>(dre[j]);
+            sim[j] = sim[j] + static_cast<double>(dim[j]);
           }
         }
       }
