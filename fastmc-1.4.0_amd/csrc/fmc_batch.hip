@@ -28,6 +28,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -1406,6 +1407,12 @@ struct fmc_batch {
   bool dirty_sys = true;
   hipStream_t stream = nullptr;
   hipStream_t setup = nullptr;   // table/grid building while a launch runs
+  // One thread drives launches; another may admit chains meanwhile.  mu
+  // guards the host-side tables; admit_mu serialises admissions (the set-up
+  // stream); device waits happen outside both.
+  std::mutex mu;
+  std::mutex admit_mu;
+  std::vector<char> reserved;   // slots being admitted
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // graph replays: one event pair around each replay of the move graph
   std::vector<std::pair<hipEvent_t, hipEvent_t>> cyc_ev;
@@ -1468,14 +1475,11 @@ int kpt_for(int bt, int nk) {
   return best;
 }
 
-int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
+// Device data of a new system (tables, k-vectors, framework S(k), candidate
+// records) built on the set-up stream; touches no shared batch state.
+int build_system(fmc_batch* b, const fmc_ctx* ctx, uint64_t key,
+                 std::unique_ptr<SysHost>* out) {
   const SystemDesc& s = ctx->sys;
-  const uint64_t key = system_key(s);
-  for (size_t i = 0; i < b->systems.size(); ++i)
-    if (b->systems[i] && b->systems[i]->key == key) {
-      *out = static_cast<int>(i);
-      return FMC_OK;
-    }
   auto sh = std::make_unique<SysHost>();
   sh->key = key;
   BSys& Y = sh->dev;
@@ -1652,16 +1656,35 @@ int upload_system(fmc_batch* b, const fmc_ctx* ctx, int* out) {
   for (auto& kb : K) sh->bytes += kb->n;
   sh->nlm = (s.kmax[0] + 1) * (2 * s.kmax[1] + 1);
   for (const auto& g : s.guests) sh->npos = std::max(sh->npos, 2 * g.ns);
+  *out = std::move(sh);
+  return FMC_OK;
+}
+
+// (batch lock held) index of a system, -1 if it is not there
+int find_system(fmc_batch* b, uint64_t key) {
+  for (size_t i = 0; i < b->systems.size(); ++i)
+    if (b->systems[i] && b->systems[i]->key == key) return static_cast<int>(i);
+  return -1;
+}
+
+// (batch lock held) takes a built system into the table
+int insert_system(fmc_batch* b, std::unique_ptr<SysHost> sh) {
   b->dirty_sys = true;
   for (size_t i = 0; i < b->systems.size(); ++i)
     if (!b->systems[i]) {
       b->systems[i] = std::move(sh);
-      *out = static_cast<int>(i);
-      return FMC_OK;
+      return static_cast<int>(i);
     }
   b->systems.push_back(std::move(sh));
-  *out = static_cast<int>(b->systems.size()) - 1;
-  return FMC_OK;
+  return static_cast<int>(b->systems.size()) - 1;
+}
+
+// (batch lock held) one reference less; a system nobody uses is freed
+void release_system(fmc_batch* b, int sys) {
+  if (sys >= 0 && b->systems[sys] && --b->systems[sys]->refs <= 0) {
+    b->systems[sys].reset();
+    b->dirty_sys = true;
+  }
 }
 
 void invalidate_graph(fmc_batch* b) {
@@ -1775,15 +1798,18 @@ struct Snapshot {
 };
 
 int snapshot(fmc_batch* b, Snapshot* s, bool restore) {
+  // active chains only: a slot being admitted meanwhile is not touched
   const int nmax = b->max_chains;
   if (!restore) {
     s->chains.resize(nmax);
-    BCHECK(hipMemcpy(s->chains.data(), b->d_chains, nmax * sizeof(BChain),
-                     hipMemcpyDeviceToHost));
+    for (int id : b->active_ids)
+      BCHECK(hipMemcpy(&s->chains[id], b->d_chains + id, sizeof(BChain),
+                       hipMemcpyDeviceToHost));
     s->data.clear();
   } else {
-    BCHECK(hipMemcpy(b->d_chains, s->chains.data(), nmax * sizeof(BChain),
-                     hipMemcpyHostToDevice));
+    for (int id : b->active_ids)
+      BCHECK(hipMemcpy(b->d_chains + id, &s->chains[id], sizeof(BChain),
+                       hipMemcpyHostToDevice));
   }
   size_t slot = 0;
   for (int id : b->active_ids) {
@@ -1872,6 +1898,7 @@ extern "C" int fmc_batch_create(fmc_batch** out, int max_chains, int precision) 
   b->max_chains = max_chains;
   b->precision = precision;
   b->chains.resize(max_chains);
+  b->reserved.assign(max_chains, 0);
   b->h_chains.resize(max_chains);
   std::memset(b->h_chains.data(), 0, max_chains * sizeof(BChain));
   BCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -1945,111 +1972,147 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
     return FMC_ERR_ARG;
   for (int g = 0; g < p->ntpguest; ++g)
     if (p->freq[g][3] > 0.0 || p->freq[g][8] > 0.0) return FMC_ERR_UNSUPPORTED;
-  int slot = -1;
-  for (int i = 0; i < b->max_chains; ++i)
-    if (!b->chains[i] || !b->chains[i]->active) {
-      slot = i;
+  std::lock_guard<std::mutex> admit(b->admit_mu);
+  const uint64_t key = system_key(ctx->sys);
+  int slot = -1, sys = -1;
+  {
+    // reserve a slot and pin the system if the batch already holds it
+    std::lock_guard<std::mutex> g(b->mu);
+    for (int i = 0; i < b->max_chains; ++i)
+      if (!b->chains[i] && !b->reserved[i]) {
+        slot = i;
+        break;
+      }
+    if (slot < 0) return FMC_ERR_CAPACITY;
+    b->reserved[slot] = 1;
+    sys = find_system(b, key);
+    if (sys >= 0) b->systems[sys]->refs++;
+  }
+  int r = FMC_OK;
+  BSys* d_self = nullptr;
+  std::unique_ptr<ChainHost> ch;
+  BChain C;
+  do {
+    if (sys < 0) {
+      std::unique_ptr<SysHost> sh;
+      if ((r = build_system(b, ctx, key, &sh))) break;
+      std::lock_guard<std::mutex> g(b->mu);
+      sh->refs = 1;
+      sys = insert_system(b, std::move(sh));
+      refresh_limits(b);
+    }
+    {
+      std::lock_guard<std::mutex> g(b->mu);
+      d_self = b->systems[sys]->d_self;
+    }
+    ch = std::make_unique<ChainHost>();
+    ch->sys = sys;
+    std::memset(&C, 0, sizeof(C));
+    C.sys = sys;
+    C.ntpguest = p->ntpguest;
+    C.cap_atoms = ctx->cap_atoms;
+    const int nk = ctx->nk;
+    auto& K = ch->keep;
+    // S(k) is set by the admission recompute below
+    if ((r = dput(K, &C.sre, static_cast<double*>(nullptr), nk)) ||
+        (r = dput(K, &C.sim, static_cast<double*>(nullptr), nk)) ||
+        (r = dput(K, &C.dsre, static_cast<double*>(nullptr), nk)) ||
+        (r = dput(K, &C.dsim, static_cast<double*>(nullptr), nk)))
+      break;
+    if (hipMemsetAsync(C.sre, 0, nk * sizeof(double), b->setup) != hipSuccess ||
+        hipMemsetAsync(C.sim, 0, nk * sizeof(double), b->setup) != hipSuccess) {
+      r = FMC_ERR_HIP;
       break;
     }
-  if (slot < 0) return FMC_ERR_CAPACITY;
-  int r, sys;
-  if ((r = upload_system(b, ctx, &sys))) return r;
-  auto ch = std::make_unique<ChainHost>();
-  ch->sys = sys;
-  BChain C;
-  std::memset(&C, 0, sizeof(C));
-  C.sys = sys;
-  C.ntpguest = p->ntpguest;
-  C.cap_atoms = ctx->cap_atoms;
-  const int nk = ctx->nk;
-  auto& K = ch->keep;
-  if ((r = dput(K, &C.sre, static_cast<double*>(nullptr), nk)) ||
-      (r = dput(K, &C.sim, static_cast<double*>(nullptr), nk)) ||
-      (r = dput(K, &C.dsre, static_cast<double*>(nullptr), nk)) ||
-      (r = dput(K, &C.dsim, static_cast<double*>(nullptr), nk)))
+    for (int g = 0; g < p->ntpguest && !r; ++g) {
+      if ((r = dput(K, &C.gx[g], ctx->hx[g].data(), C.cap_atoms)) ||
+          (r = dput(K, &C.gy[g], ctx->hy[g].data(), C.cap_atoms)) ||
+          (r = dput(K, &C.gz[g], ctx->hz[g].data(), C.cap_atoms)))
+        break;
+      C.nmol[g] = ctx->nmol[g];
+    }
+    if (r) break;
+    C.temp = p->temp;
+    C.beta = 1.0 / (kKboltz * p->temp);
+    C.statvolm = ctx->sys.volm * 1e-30;
+    C.engunit = p->engunit > 0 ? p->engunit : 418.4;
+    C.eq_steps = p->eq_steps;
+    for (int g = 0; g < p->ntpguest; ++g) {
+      C.press[g] = p->press[g];
+      double f[9], norm = 0.0;
+      for (int i = 0; i < 9; ++i) {
+        f[i] = p->freq[g][i];
+        norm += f[i];
+      }
+      if (norm == 0.0) {
+        for (int i = 0; i < 9; ++i) f[i] = 0.0;
+        f[0] = f[1] = f[2] = 1.0 / 3.0;
+        norm = 1.0;
+      }
+      C.cum[g][0] = f[0] / norm;
+      for (int i = 1; i < 9; ++i) C.cum[g][i] = C.cum[g][i - 1] + (f[i] / norm);
+      C.disp_ratio[g] = p->disp_ratio[g] > 0 ? p->disp_ratio[g] : 0.5;
+      C.tran_ratio[g] = p->tran_ratio[g] > 0 ? p->tran_ratio[g] : 0.5;
+      C.delrdisp[g] = p->delr[g];
+      C.tran_delr[g] = p->delr[g];
+    }
+    C.rota_ratio = p->rota_ratio > 0 ? p->rota_ratio : 0.5;
+    C.rotangle = kPi / 3.0;
+    C.rota_rotangle = C.rotangle;
+    C.elrc = 0.0;
+    C.stop_at = p->max_moves > 0 ? p->max_moves : std::numeric_limits<long long>::max();
+    duni_setup(p->seed[0], p->seed[1], p->seed[2], p->seed[3], &C);
+    for (auto& kb : K) ch->bytes += kb->n;
+    // the slot is outside any launch in flight: its struct and the
+    // admission recompute (S(k), the full FP64 energy the accumulators
+    // start from) go through the set-up stream
+    if (hipMemcpyAsync(b->d_chains + slot, &C, sizeof(BChain), hipMemcpyHostToDevice,
+                       b->setup) != hipSuccess) {
+      r = FMC_ERR_HIP;
+      break;
+    }
+    RecomputeParams P;
+    P.sys = nullptr;
+    P.chains = b->d_chains;
+    P.ids = nullptr;
+    P.maxdev = b->d_dev1;
+    P.base = 1;
+    P.one_id = slot;
+    P.one = d_self;
+    hipLaunchKernelGGL(recompute_kernel, dim3(1), dim3(kRecBT), 0, b->setup, P);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->setup) != hipSuccess)
+      r = FMC_ERR_HIP;
+  } while (false);
+  std::lock_guard<std::mutex> g(b->mu);
+  b->reserved[slot] = 0;
+  if (r) {
+    ch.reset();
+    release_system(b, sys);
+    refresh_limits(b);
     return r;
-  BCHECK(hipMemcpy(C.sre, ctx->d_sre[ctx->cur], nk * sizeof(double), hipMemcpyDeviceToDevice));
-  BCHECK(hipMemcpy(C.sim, ctx->d_sim[ctx->cur], nk * sizeof(double), hipMemcpyDeviceToDevice));
-  for (int g = 0; g < p->ntpguest; ++g) {
-    if ((r = dput(K, &C.gx[g], ctx->hx[g].data(), C.cap_atoms)) ||
-        (r = dput(K, &C.gy[g], ctx->hy[g].data(), C.cap_atoms)) ||
-        (r = dput(K, &C.gz[g], ctx->hz[g].data(), C.cap_atoms)))
-      return r;
-    C.nmol[g] = ctx->nmol[g];
   }
-  C.temp = p->temp;
-  C.beta = 1.0 / (kKboltz * p->temp);
-  C.statvolm = ctx->sys.volm * 1e-30;
-  C.engunit = p->engunit > 0 ? p->engunit : 418.4;
-  C.eq_steps = p->eq_steps;
-  for (int g = 0; g < p->ntpguest; ++g) {
-    C.press[g] = p->press[g];
-    double f[9], norm = 0.0;
-    for (int i = 0; i < 9; ++i) {
-      f[i] = p->freq[g][i];
-      norm += f[i];
-    }
-    if (norm == 0.0) {
-      for (int i = 0; i < 9; ++i) f[i] = 0.0;
-      f[0] = f[1] = f[2] = 1.0 / 3.0;
-      norm = 1.0;
-    }
-    C.cum[g][0] = f[0] / norm;
-    for (int i = 1; i < 9; ++i) C.cum[g][i] = C.cum[g][i - 1] + (f[i] / norm);
-    C.disp_ratio[g] = p->disp_ratio[g] > 0 ? p->disp_ratio[g] : 0.5;
-    C.tran_ratio[g] = p->tran_ratio[g] > 0 ? p->tran_ratio[g] : 0.5;
-    C.delrdisp[g] = p->delr[g];
-    C.tran_delr[g] = p->delr[g];
-  }
-  C.rota_ratio = p->rota_ratio > 0 ? p->rota_ratio : 0.5;
-  C.rotangle = kPi / 3.0;
-  C.rota_rotangle = C.rotangle;
-  C.elrc = 0.0;
-  C.stop_at = p->max_moves > 0 ? p->max_moves : std::numeric_limits<long long>::max();
-  duni_setup(p->seed[0], p->seed[1], p->seed[2], p->seed[3], &C);
-  for (auto& kb : K) ch->bytes += kb->n;
   ch->active = true;
-  SysHost& sh = *b->systems[sys];
-  sh.refs++;
-  refresh_limits(b);
   b->chains[slot] = std::move(ch);
   b->h_chains[slot] = C;
-  // the slot is outside any launch in flight: its struct and the admission
-  // recompute (full FP64 energy, the accumulators' starting point) go
-  // through the set-up stream
-  BCHECK(hipMemcpyAsync(b->d_chains + slot, &C, sizeof(BChain), hipMemcpyHostToDevice,
-                        b->setup));
-  RecomputeParams P;
-  P.sys = nullptr;
-  P.chains = b->d_chains;
-  P.ids = nullptr;
-  P.maxdev = b->d_dev1;
-  P.base = 1;
-  P.one_id = slot;
-  P.one = sh.d_self;
-  hipLaunchKernelGGL(recompute_kernel, dim3(1), dim3(kRecBT), 0, b->setup, P);
-  BCHECK(hipGetLastError());
-  BCHECK(hipStreamSynchronize(b->setup));
   if (chain_id) *chain_id = slot;
   return FMC_OK;
 }
 
 extern "C" int fmc_batch_remove_chain(fmc_batch* b, int id) {
-  if (!b || id < 0 || id >= b->max_chains || !b->chains[id] || !b->chains[id]->active)
-    return FMC_ERR_ARG;
+  if (!b || id < 0 || id >= b->max_chains) return FMC_ERR_ARG;
   BCHECK(hipStreamSynchronize(b->stream));
+  std::lock_guard<std::mutex> g(b->mu);
+  if (!b->chains[id] || !b->chains[id]->active) return FMC_ERR_ARG;
   const int sys = b->chains[id]->sys;
   b->chains[id].reset();
-  if (sys >= 0 && b->systems[sys] && --b->systems[sys]->refs <= 0) {
-    b->systems[sys].reset();   // nobody references it any more
-    b->dirty_sys = true;
-  }
+  release_system(b, sys);
   refresh_limits(b);
   return FMC_OK;
 }
 
 extern "C" int fmc_batch_set_tile(fmc_batch* b, int threads) {
   if (!b) return FMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(b->mu);
   if (threads == 0) {
     b->bt = b->kpt = 0;
     b->tuned = false;
@@ -2068,6 +2131,7 @@ extern "C" int fmc_batch_set_tile(fmc_batch* b, int threads) {
 
 extern "C" int fmc_batch_get_tile(const fmc_batch* b, int* threads) {
   if (!b || !threads) return FMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(const_cast<fmc_batch*>(b)->mu);
   *threads = b->bt | ((b->tuned && b->kpt == 0 && b->bt) ? FMC_TILE_STREAM : 0);
   return FMC_OK;
 }
@@ -2076,6 +2140,7 @@ extern "C" int fmc_batch_launch(fmc_batch* b, long long n) {
   if (!b || n < 0) return FMC_ERR_ARG;
   int r;
   if ((r = fmc_batch_sync(b))) return r;
+  std::lock_guard<std::mutex> g(b->mu);
   if ((r = sync_tables(b))) return r;
   if (b->active_ids.empty()) return FMC_OK;
   if ((r = ensure_tile(b))) return r;
@@ -2090,7 +2155,8 @@ extern "C" int fmc_batch_launch(fmc_batch* b, long long n) {
 
 extern "C" int fmc_batch_sync(fmc_batch* b) {
   if (!b) return FMC_ERR_ARG;
-  BCHECK(hipStreamSynchronize(b->stream));
+  BCHECK(hipStreamSynchronize(b->stream));   // the wait holds no lock
+  std::lock_guard<std::mutex> g(b->mu);
   if (b->timing_pending) {
     // mode 2: one batch-kernel graph replay per event pair
     float ms = 0.0f;
@@ -2129,6 +2195,7 @@ extern "C" int fmc_batch_query(fmc_batch* b, int* done) {
 extern "C" int fmc_batch_timing(fmc_batch* b, int reset, double* ms,
                                 long long* launches) {
   if (!b) return FMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(b->mu);
   if (ms) *ms = b->kernel_ms;
   if (launches) *launches = b->launches;
   if (reset) {
@@ -2145,7 +2212,9 @@ extern "C" int fmc_batch_run(fmc_batch* b, long long n) {
 }
 
 extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
-  if (!b || !st || id < 0 || id >= b->max_chains || !b->chains[id]) return FMC_ERR_ARG;
+  if (!b || !st || id < 0 || id >= b->max_chains) return FMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(b->mu);
+  if (!b->chains[id]) return FMC_ERR_ARG;
   BChain C;
   BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
   std::memset(st, 0, sizeof(*st));
@@ -2175,7 +2244,9 @@ extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
 }
 
 extern "C" int fmc_batch_enable_log(fmc_batch* b, int id, long long cap) {
-  if (!b || id < 0 || id >= b->max_chains || !b->chains[id] || cap < 0) return FMC_ERR_ARG;
+  if (!b || id < 0 || id >= b->max_chains || cap < 0) return FMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(b->mu);
+  if (!b->chains[id]) return FMC_ERR_ARG;
   BChain C;
   BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
   double* lg = nullptr;
@@ -2191,7 +2262,9 @@ extern "C" int fmc_batch_enable_log(fmc_batch* b, int id, long long cap) {
 }
 
 extern "C" int fmc_batch_get_log(fmc_batch* b, int id, long long n, double* rec) {
-  if (!b || !rec || id < 0 || id >= b->max_chains || !b->chains[id]) return FMC_ERR_ARG;
+  if (!b || !rec || id < 0 || id >= b->max_chains) return FMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(b->mu);
+  if (!b->chains[id]) return FMC_ERR_ARG;
   BChain C;
   BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
   if (!C.log || n > C.log_n) return FMC_ERR_ARG;
@@ -2224,6 +2297,7 @@ extern "C" int fmc_batch_recompute(fmc_batch* b, double* max_sk_dev, double* max
   if (!b) return FMC_ERR_ARG;
   int r;
   if ((r = fmc_batch_sync(b))) return r;
+  std::lock_guard<std::mutex> g(b->mu);
   if ((r = sync_tables(b))) return r;
   if (max_sk_dev) *max_sk_dev = 0.0;
   if (max_drift) *max_drift = 0.0;
@@ -2254,6 +2328,7 @@ extern "C" int fmc_batch_launch_cycles(fmc_batch* b, long long n_cycles,
   if (!b || n_cycles < 0 || moves_per_cycle < 0) return FMC_ERR_ARG;
   int r;
   if ((r = fmc_batch_sync(b))) return r;
+  std::lock_guard<std::mutex> g(b->mu);
   if ((r = sync_tables(b))) return r;
   if (b->active_ids.empty() || n_cycles == 0) return FMC_OK;
   if ((r = ensure_tile(b))) return r;
@@ -2311,6 +2386,7 @@ extern "C" int fmc_batch_poll(fmc_batch* b, int* ids, int cap, int* n) {
   if (!b || !n || (cap > 0 && !ids)) return FMC_ERR_ARG;
   int r;
   if ((r = fmc_batch_sync(b))) return r;
+  std::lock_guard<std::mutex> g(b->mu);
   *n = 0;
   BCHECK(hipMemcpy(b->h_chains.data(), b->d_chains, b->max_chains * sizeof(BChain),
                    hipMemcpyDeviceToHost));

@@ -42,6 +42,9 @@ struct fmc_ctx {
   int pending = 0, pend_gt = 0, pend_mol = 0;
   double pend_x[fmc::kMaxSites], pend_y[fmc::kMaxSites], pend_z[fmc::kMaxSites];
 
+  bool dev_ready = false;   // device state built (first single-chain call)
+
+  int host_state();
   int upload_state();
   void release();
 };

@@ -599,6 +599,37 @@ extern "C" int fmc_set_capacity(fmc_ctx* c, int max_guest_atoms) {
   return FMC_OK;
 }
 
+// Host side of finalize: k-vector count, capacity and the host mirror of
+// the guests read from CONFIG/REVIVE.  Nothing touches the device, so a
+// context that only feeds batched chains never allocates device memory.
+int fmc_ctx::host_state() {
+  const SystemDesc& s = sys;
+  const int ng = static_cast<int>(s.guests.size());
+  nk = static_cast<int>(s.kvecs.size());
+  cap_atoms = s.max_guest_atoms;
+  nmol.assign(ng, 0);
+  int guest_atoms = 0;
+  for (int t = 0; t < ng; ++t) {
+    const int ns = s.guests[t].ns;
+    hx[t].assign(cap_atoms, 0.0);
+    hy[t].assign(cap_atoms, 0.0);
+    hz[t].assign(cap_atoms, 0.0);
+    const int n0 = static_cast<int>(s.gx0[t].size());
+    if (n0 > cap_atoms) return FMC_ERR_CAPACITY;
+    for (int i = 0; i < n0; ++i) {
+      hx[t][i] = s.gx0[t][i];
+      hy[t][i] = s.gy0[t][i];
+      hz[t][i] = s.gz0[t][i];
+    }
+    nmol[t] = n0 / ns;
+    guest_atoms += n0;
+  }
+  if (guest_atoms > cap_atoms) return FMC_ERR_CAPACITY;
+  return FMC_OK;
+}
+
+// Device side, on the first single-chain call: tables, framework, guests,
+// S(k) buffers and the all-atom S(k) of gcmc.f:522.
 int fmc_ctx::upload_state() {
   const SystemDesc& s = sys;
   const int ng = static_cast<int>(s.guests.size());
@@ -618,7 +649,6 @@ int fmc_ctx::upload_state() {
   if ((r = dalloc(&d_fz, nfw)) || (r = upload(d_fz, s.fz.data(), nfw))) return r;
   if ((r = dalloc(&d_fq, nfw)) || (r = upload(d_fq, s.fq.data(), nfw))) return r;
   if ((r = dalloc(&d_ftype, nfw)) || (r = upload(d_ftype, ft0.data(), nfw))) return r;
-  nk = static_cast<int>(s.kvecs.size());
   std::vector<int> kv(nk);
   std::vector<double> ak(nk);
   for (int k = 0; k < nk; ++k) {
@@ -631,30 +661,15 @@ int fmc_ctx::upload_state() {
   for (int b = 0; b < 2; ++b) {
     if ((r = dalloc(&d_sre[b], nk)) || (r = dalloc(&d_sim[b], nk))) return r;
   }
-  cap_atoms = s.max_guest_atoms;
-  nmol.assign(ng, 0);
   for (int t = 0; t < ng; ++t) {
-    const int ns = s.guests[t].ns;
+    const int n0 = nmol[t] * s.guests[t].ns;
     if ((r = dalloc(&d_gx[t], cap_atoms)) || (r = dalloc(&d_gy[t], cap_atoms)) ||
         (r = dalloc(&d_gz[t], cap_atoms)))
       return r;
-    hx[t].assign(cap_atoms, 0.0);
-    hy[t].assign(cap_atoms, 0.0);
-    hz[t].assign(cap_atoms, 0.0);
-    const int n0 = static_cast<int>(s.gx0[t].size());
-    for (int i = 0; i < n0; ++i) {
-      hx[t][i] = s.gx0[t][i];
-      hy[t][i] = s.gy0[t][i];
-      hz[t][i] = s.gz0[t][i];
-    }
-    nmol[t] = n0 / ns;
     if ((r = upload(d_gx[t], hx[t].data(), n0)) || (r = upload(d_gy[t], hy[t].data(), n0)) ||
         (r = upload(d_gz[t], hz[t].data(), n0)))
       return r;
   }
-  int guest_atoms = 0;
-  for (int t = 0; t < ng; ++t) guest_atoms += nmol[t] * s.guests[t].ns;
-  if (guest_atoms > cap_atoms) return FMC_ERR_CAPACITY;
   recip_chunk = 1024;
   recip_nchunk_cap = static_cast<int>((nfw + cap_atoms) / recip_chunk) + 2;
   if ((r = dalloc(&d_rpart, static_cast<size_t>(recip_nchunk_cap) * nk))) return r;
@@ -696,9 +711,19 @@ extern "C" int fmc_ctx_finalize(fmc_ctx* c) {
     std::fprintf(stderr, "fastmc_amd: %s\n", err.c_str());
     return r;
   }
+  return c->host_state();
+}
+
+// device state on the first call that needs it
+static int ensure_device(fmc_ctx* c) {
+  if (!c->sys.finalized) return FMC_ERR_STATE;
+  if (c->dev_ready) return FMC_OK;
+  int r;
   if ((r = c->upload_state())) return r;
   double e;
-  return recip_full_impl(c, true, true, &e);
+  if ((r = recip_full_impl(c, true, true, &e))) return r;
+  c->dev_ready = true;
+  return FMC_OK;
 }
 
 extern "C" int fmc_get_info(const fmc_ctx* c, int* nk, double* volm,
@@ -756,6 +781,9 @@ extern "C" int fmc_get_table(const fmc_ctx* c, int which, int t, double* out) {
 
 extern "C" int fmc_get_sk(const fmc_ctx* c, double* re, double* im) {
   if (!c || !c->sys.finalized) return FMC_ERR_STATE;
+  // building the device state on first use changes nothing observable
+  int r;
+  if ((r = ensure_device(const_cast<fmc_ctx*>(c)))) return r;
   HIPCHECK(hipMemcpy(re, c->d_sre[c->cur], c->nk * sizeof(double), hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(im, c->d_sim[c->cur], c->nk * sizeof(double), hipMemcpyDeviceToHost));
   return FMC_OK;
@@ -767,6 +795,8 @@ extern "C" int fmc_get_sk(const fmc_ctx* c, double* re, double* im) {
 
 extern "C" int fmc_recip_full(fmc_ctx* c, int all_atoms, double* energy) {
   if (!c || !c->sys.finalized || !energy) return FMC_ERR_STATE;
+  int r;
+  if ((r = ensure_device(c))) return r;
   c->pending = 0;
   return recip_full_impl(c, all_atoms != 0, all_atoms != 0, energy);
 }
@@ -775,6 +805,10 @@ extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
                          const double* x, const double* y, const double* z,
                          double* out) {
   if (!c || !c->sys.finalized) return FMC_ERR_STATE;
+  {
+    const int r0 = ensure_device(c);
+    if (r0) return r0;
+  }
   if (gt < 0 || gt >= static_cast<int>(c->sys.guests.size()) || !out)
     return FMC_ERR_ARG;
   if (kind != FMC_INSERT && kind != FMC_DELETE && kind != FMC_DISPLACE)
@@ -910,9 +944,10 @@ extern "C" int fmc_tail_delta(fmc_ctx* c, int gt, int sign, double elrc,
 
 extern "C" int fmc_energy_full(fmc_ctx* c, double* out) {
   if (!c || !c->sys.finalized || !out) return FMC_ERR_STATE;
+  int r;
+  if ((r = ensure_device(c))) return r;
   c->pending = 0;
   double efw = 0.0, eall = 0.0;
-  int r;
   if ((r = recip_full_impl(c, false, false, &efw))) return r;
   if ((r = recip_full_impl(c, true, true, &eall))) return r;
   PairAllParams P;

@@ -12,10 +12,13 @@ the next slice, and finished chains leave at slice boundaries.
 
 A chain's arithmetic does not depend on which other chains share its
 launch (one work-group per chain, a fixed tile), so a request's result is
-the same alone and in the mix.
+the same alone and in the mix.  Admission runs on its own thread (the C
+library guards its tables and serialises admissions), so building a new
+system overlaps the slices on the GPU.
 """
 from __future__ import annotations
 
+import threading
 import time
 from dataclasses import dataclass, field
 from typing import Callable, Dict, List, Optional, Sequence
@@ -65,79 +68,87 @@ class Server:
         self.batch = Batch(self.slots, self.precision)
         self.batch.set_tile(self.tile)
         self._live: Dict[int, tuple] = {}   # chain id -> (request, admitted, bytes)
-        self._used = 0
-        self._ctx: Dict[int, tuple] = {}    # rid -> (context, footprint) built
-        self._spent: List[Context] = []     # freed at the next slice boundary
+        self._used = 0                      # bytes admitted or being admitted
+        self._inflight = 0                  # admissions under way
         self.slices = 0
         self.moves_done = 0
 
     def close(self):
         self.batch.close()
 
-    def _admit(self, req: Request, now: float) -> bool:
-        if req.rid not in self._ctx:
-            ctx = Context.from_arrays(req.arrays)
-            self._ctx[req.rid] = (ctx, footprint(ctx, self.precision))
-        ctx, need = self._ctx[req.rid]
-        if len(self._live) >= self.slots or self._used + need > self.hbm_budget:
-            return False
-        cid = self.batch.add_chain(ctx, req.temp, req.press_pa, req.seed,
-                                   delr=req.delr, eq_steps=req.eq_steps,
-                                   max_moves=req.moves)
-        # the chain holds its own copy; the context's device buffers are
-        # released at the next slice boundary, not under a running slice
-        self._spent.append(self._ctx.pop(req.rid)[0])
-        self._live[cid] = (req, now, need)
-        self._used += need
-        return True
-
-    def _retire(self, now: float):
+    def _retire(self, now: float, cv: threading.Condition):
         for cid in self.batch.poll():
-            req, t_adm, need = self._live.pop(cid)
+            with cv:
+                req, t_adm, need = self._live.pop(cid)
             st = self.batch.stats(cid)
             self.batch.remove_chain(cid)
-            self._used -= need
-            self.moves_done += st["steps"]
-            self.results[req.rid] = Result(req.rid, req.arrival, t_adm, now, st, need)
+            with cv:
+                self._used -= need
+                self.moves_done += st["steps"]
+                self.results[req.rid] = Result(req.rid, req.arrival, t_adm, now, st, need)
+                cv.notify_all()
 
     def run(self, requests: Sequence[Request], realtime: bool = True) -> List[Result]:
-        """Serve the requests; with realtime, each becomes visible at its
-        arrival time (seconds after the start), else all at once."""
+        """Serve the requests; with realtime, each arrives at its arrival
+        time (seconds after the start), else all at once.  An admission
+        thread builds each request's system and adds its chain while the
+        GPU runs slices; this thread launches slices back to back and
+        retires finished chains."""
         pending = sorted(requests, key=lambda r: r.arrival)
+        cv = threading.Condition()
         t0 = self.clock()
-        nxt = 0
-        waiting: List[Request] = []
-        in_flight = False
-        while nxt < len(pending) or waiting or self._live:
-            now = self.clock() - t0
-            while nxt < len(pending) and (not realtime or pending[nxt].arrival <= now):
-                waiting.append(pending[nxt])
-                nxt += 1
-            # admission (host work overlaps the slice in flight)
-            while waiting and self._admit(waiting[0], self.clock() - t0):
-                waiting.pop(0)
-            if in_flight:
-                if (realtime and nxt < len(pending) and not self.batch.done()
-                        and pending[nxt].arrival <= self.clock() - t0 + 2e-4):
-                    continue   # an arrival is due: admit it under this slice
-                if realtime and not self.batch.done():
-                    time.sleep(1e-4)
-                    continue
-                self.batch.sync()
-                in_flight = False
-                self._retire(self.clock() - t0)
-                for c in self._spent:
-                    c.close()
-                self._spent.clear()
-                continue
-            if self._live:
+        state = {"done": False, "error": None}
+
+        def admitter():
+            try:
+                for req in pending:
+                    if realtime:
+                        delay = req.arrival - (self.clock() - t0)
+                        if delay > 0:
+                            time.sleep(delay)
+                    ctx = Context.from_arrays(req.arrays)
+                    need = footprint(ctx, self.precision)
+                    with cv:
+                        while (len(self._live) + self._inflight >= self.slots or
+                               self._used + need > self.hbm_budget):
+                            cv.wait()
+                        self._inflight += 1
+                        self._used += need
+                    cid = self.batch.add_chain(ctx, req.temp, req.press_pa, req.seed,
+                                               delr=req.delr, eq_steps=req.eq_steps,
+                                               max_moves=req.moves)
+                    ctx.close()
+                    with cv:
+                        self._inflight -= 1
+                        self._live[cid] = (req, self.clock() - t0, need)
+                        cv.notify_all()
+            except BaseException as e:   # surfaced by run()
+                state["error"] = e
+            finally:
+                with cv:
+                    state["done"] = True
+                    cv.notify_all()
+
+        th = threading.Thread(target=admitter, daemon=True)
+        th.start()
+        try:
+            while True:
+                with cv:
+                    if state["error"] is not None:
+                        break
+                    if state["done"] and not self._live and not self._inflight:
+                        break
+                    if not self._live:
+                        cv.wait(timeout=0.01)
+                        continue
                 self.batch.launch(self.slice_moves)
                 self.slices += 1
-                in_flight = True
-            elif nxt < len(pending):
-                delay = pending[nxt].arrival - (self.clock() - t0)
-                if delay > 0:
-                    time.sleep(min(delay, 0.01))
+                self.batch.sync()
+                self._retire(self.clock() - t0, cv)
+        finally:
+            th.join()
+        if state["error"] is not None:
+            raise state["error"]
         return [self.results[r.rid] for r in requests]
 
 
