@@ -703,8 +703,6 @@ void fmc_ctx::release() {
 extern "C" int fmc_ctx_finalize(fmc_ctx* c) {
   if (!c) return FMC_ERR_ARG;
   if (c->sys.finalized) return FMC_ERR_STATE;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FMC_ERR_NODEVICE;
   std::string err;
   int r = c->sys.finalize(&err);
   if (r) {
@@ -718,6 +716,8 @@ extern "C" int fmc_ctx_finalize(fmc_ctx* c) {
 static int ensure_device(fmc_ctx* c) {
   if (!c->sys.finalized) return FMC_ERR_STATE;
   if (c->dev_ready) return FMC_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return FMC_ERR_NODEVICE;
   int r;
   if ((r = c->upload_state())) return r;
   double e;

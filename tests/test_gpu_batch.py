@@ -248,3 +248,21 @@ def test_batch_fp64_triclinic_revive(fmc, oracle, restart_example):
     b.run(n)
     worst = _compare(b.get_log(cid, n), ref)
     print(f"triclinic revive: {int(ref[:, 2].sum())} accepted, worst estep rel {worst:.2e}")
+
+
+def test_capacity_fault_reported(fmc):
+    """Guest capacity exhausted (the reference's error 75, gcmc.f:870): the
+    chain stops inserting, flags the fault, and poll() hands it back."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd.capi import PREC_MIXED
+    A = _porous(seed=4, guests=("co2",), press=(20.0,)).system_arrays()
+    ctx = fmc.Context.from_arrays(A, max_guest_atoms=10)
+    b = Batch(2, PREC_MIXED)
+    full = b.add_chain(ctx, 298.0, [20.0e5], (1, 2, 3, 4), delr=A["delr"])
+    ok = b.add_chain(fmc.Context.from_arrays(A), 298.0, [20.0e5], (1, 2, 3, 4),
+                     delr=A["delr"], max_moves=10 ** 9)
+    b.run(3000)
+    st = b.stats(full)
+    assert st["capacity_fault"] and st["nmol"][0] <= 3
+    assert not b.stats(ok)["capacity_fault"] and b.stats(ok)["nmol"][0] > 3
+    assert b.poll() == [full]
