@@ -224,8 +224,7 @@ struct MoveDesc {
   int nguest;                   // guest atoms of every type
   int gpre[kMaxGuestTypes + 1]; // guest-type prefix over atoms
   int pset[2 * kMaxSites], psite[2 * kMaxSites];
-  int plo[2 * kMaxSites], plen[2 * kMaxSites];
-  int ppre[2 * kMaxSites + 1];  // prefix of candidate-list lengths
+  int plo[2 * kMaxSites];        // first candidate record of the cell
   // per position: coordinates, q/epsq*r4pie0, site type, sign of its terms
   double ppx[2 * kMaxSites], ppy[2 * kMaxSites], ppz[2 * kMaxSites];
   double pce[2 * kMaxSites];
@@ -731,7 +730,6 @@ __global__ void __launch_bounds__(BT)
           const int c = set ? cell_of(Z, D.nx[s], D.ny[s], D.nz[s])
                             : cell_of(Z, D.ox[s], D.oy[s], D.oz[s]);
           D.plo[p] = c * Y.gcap;
-          D.plen[p] = Y.gcap;
           D.ppx[p] = set ? D.nx[s] : D.ox[s];
           D.ppy[p] = set ? D.ny[s] : D.oy[s];
           D.ppz[p] = set ? D.nz[s] : D.oz[s];
@@ -811,17 +809,11 @@ __global__ void __launch_bounds__(BT)
           }
         }
       }
-      if (tid == 0) {
-        int acc = 0;
-        for (int p = 0; p < np; ++p) {
-          D.ppre[p] = acc;
-          acc += D.plen[p];
-        }
-        D.ppre[np] = acc;
-      }
-      __syncthreads();
       // ---- real space: framework candidates of every position's cell and
-      //      every guest atom, one flattened loop (ewald2 + srfrce) ----
+      //      every guest atom, one flattened loop (ewald2 + srfrce).  It
+      //      reads only what the set-up phases wrote, so no barrier
+      //      separates it from the reciprocal part: a wave that finishes
+      //      its k-vectors starts its pairs at once ----
       {
         const int gcap = Y.gcap;
         const int nfwp = np * gcap;
@@ -906,11 +898,12 @@ __global__ void __launch_bounds__(BT)
         ev = av;
       }
     }
-    // block reductions (fixed tree)
-    e1 = wave_sum(e1);
-    e2 = wave_sum(e2);
-    ev = wave_sum(ev);
-    if ((tid % kWave) == 0) {
+    // block reductions (fixed tree): DPP within each wave, then the waves'
+    // totals in order
+    e1 = wave_sum_dpp(e1);
+    e2 = wave_sum_dpp(e2);
+    ev = wave_sum_dpp(ev);
+    if ((tid % kWave) == kWave - 1) {
       red[0][tid / kWave] = e1;
       red[1][tid / kWave] = e2;
       red[2][tid / kWave] = ev;

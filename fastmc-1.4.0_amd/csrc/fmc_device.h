@@ -108,6 +108,31 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Cross-lane move of a double by a DPP pattern (lanes the row mask leaves
+// out read 0).
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_take(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, ROWS, 0xf, false);
+  const int hi =
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+// Wave sum through DPP (no LDS traffic); the total is valid in lane 63.
+// Fixed order: quad pairs, quads, half rows, rows, then rows 0+1 into row
+// 1 and 2+3 into row 3, then row 1 into row 3.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_take<0xB1>(v);         // quad_perm [1,0,3,2]
+  v += dpp_take<0x4E>(v);         // quad_perm [2,3,0,1]
+  v += dpp_take<0x141>(v);        // row_half_mirror
+  v += dpp_take<0x140>(v);        // row_mirror
+  v += dpp_take<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+  v += dpp_take<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
