@@ -266,7 +266,8 @@ __device__ __forceinline__ void pair_mixed(const BSys& Y, float chgea, int vk,
     int l = static_cast<int>(u);
     l = l > imax ? imax : l;
     const float p = u - static_cast<float>(l);
-    const float v0 = Y.erc32[l], v1 = Y.erc32[l + 1], v2 = Y.erc32[l + 2];
+    const float v0 = gload(Y.erc32 + l), v1 = gload(Y.erc32 + l + 1),
+                v2 = gload(Y.erc32 + l + 2);
     const float t1 = v0 + (v1 - v0) * p, t2 = v1 + (v2 - v1) * (p - 1.0f);
     if (fabsf(chgprd) > 1e-10f) e2 += (t1 + (t2 - t1) * p * 0.5f) * chgprd;
   }
@@ -276,7 +277,7 @@ __device__ __forceinline__ void pair_mixed(const BSys& Y, float chgea, int vk,
     int l = static_cast<int>(u);
     l = l > imax ? imax : l;
     const float p = u - static_cast<float>(l);
-    const float v0 = tab[l], v1 = tab[l + 1], v2 = tab[l + 2];
+    const float v0 = gload(tab + l), v1 = gload(tab + l + 1), v2 = gload(tab + l + 2);
     const float t1 = v0 + (v1 - v0) * p, t2 = v1 + (v2 - v1) * (p - 1.0f);
     ev += t1 + (t2 - t1) * p * 0.5f;
   }
@@ -786,9 +787,9 @@ __global__ void __launch_bounds__(BT)
           T dr32, di32;
           trial_ds<T>(np, xy_l, nlm, lm, zt, un(v) + Z.kmax[2], dr32, di32);
           const double dr = dr32, di = di32;
-          const double s0 = C.sre[k], s1 = C.sim[k];
-          C.dsre[k] = dr;
-          C.dsim[k] = di;
+          const double s0 = gload(C.sre + k), s1 = gload(C.sim + k);
+          gstore(C.dsre + k, dr);
+          gstore(C.dsim + k, di);
           e1 += de_k(Y.ak[k], s0, s1, dr, di);
         }
       } else {
@@ -854,10 +855,12 @@ __global__ void __launch_bounds__(BT)
             at[u] = 0;
             if (pp[u] < 0) continue;
             if (aa[u] >= 0) {
-              const Rec rr = Y.grec[aa[u]];
-              ax[u] = rr.x; ay[u] = rr.y; az[u] = rr.z;
-              aq[u] = rr.q;
-              at[u] = Y.grec_t[aa[u]];
+              const double* rr = &Y.grec[aa[u]].x;
+              ax[u] = gload(rr);
+              ay[u] = gload(rr + 1);
+              az[u] = gload(rr + 2);
+              aq[u] = gload(rr + 3);
+              at[u] = gload(Y.grec_t + aa[u]);
             } else {
               const int a = -1 - aa[u];
               int g = 0;
@@ -866,7 +869,9 @@ __global__ void __launch_bounds__(BT)
               const int sa =
                   la - div_rc(la, Z.ns[g], __builtin_amdgcn_rcpf(static_cast<float>(Z.ns[g]))) *
                            Z.ns[g];
-              ax[u] = C.gx[g][la]; ay[u] = C.gy[g][la]; az[u] = C.gz[g][la];
+              ax[u] = gload(C.gx[g] + la);
+              ay[u] = gload(C.gy[g] + la);
+              az[u] = gload(C.gz[g] + la);
               aq[u] = Z.gq[g][sa];
               at[u] = Z.gtype[g][sa];
             }
@@ -876,7 +881,8 @@ __global__ void __launch_bounds__(BT)
             if (pp[u] < 0) continue;
             const int p = pp[u];
             T b2 = 0, bv = 0;
-            const int vk = ldsrow ? D.ptab[p][at[u]] : Y.S.pairtab[D.pts[p] * ntp + at[u]];
+            const int vk =
+                ldsrow ? D.ptab[p][at[u]] : gload(Y.S.pairtab + D.pts[p] * ntp + at[u]);
             if constexpr (MIXED) {
               pair_mixed(Y, static_cast<float>(D.pce[p]), vk, D.ppx[p] - ax[u],
                          D.ppy[p] - ay[u], D.ppz[p] - az[u], static_cast<float>(aq[u]), b2,
@@ -930,8 +936,8 @@ __global__ void __launch_bounds__(BT)
       // S(k) <- S_new (ewald.f:260-293, gcmc.f:971-974)
       if (STREAM) {
         for (int k = tid; k < nk; k += BT) {
-          C.sre[k] = C.sre[k] + C.dsre[k];
-          C.sim[k] = C.sim[k] + C.dsim[k];
+          gstore(C.sre + k, gload(C.sre + k) + gload(C.dsre + k));
+          gstore(C.sim + k, gload(C.sim + k) + gload(C.dsim + k));
         }
       } else {
 #pragma unroll

@@ -31,6 +31,18 @@ struct DevSys {
   const int* ftype;      // 0-based
 };
 
+// Loads and stores through pointers known to be in global memory (the
+// compiler cannot prove it for pointers read from structures, and flat
+// accesses share the LDS wait counter).
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, const T& v) {
+  *(__attribute__((address_space(1))) T*)(p) = v;
+}
+
 // Minimum image (utility_pack.f:1138-1238) for imcon 1, 2, 3.  Fortran nint
 // rounds half away from zero, which is round().
 __device__ __forceinline__ void min_image(const DevSys& S, double& x,
