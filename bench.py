@@ -4,11 +4,14 @@ Headline (``value``): whole-job GCMC trial moves per second of the batched
 workload -- 256 independent chains (32 synthetic frameworks of 300-6000
 atoms, orthorhombic 20-60 A cells replicated to >= 2 rcut, x 8 CO2
 pressures 0.01-10 bar, 298 K, kmax 8-14), mixed precision (FP32 pair terms
-and phases, FP64 S(k) and totals).  A "step" is one replay of the captured
-move cycle: one persistent launch that advances every chain by ``--moves``
-trial moves, then the FP64 full recompute of every chain (S(k) resync and
-energy drift).  Inputs are resident in HBM before timing starts; the
-warmup steps equilibrate the chains.
+and phases, FP64 S(k) and totals), with the FP64 full recompute (S(k)
+resync, energy drift) every 1000 moves of each chain, inside the launch.
+A "step" is one persistent launch with a GPU-time budget (``--budget-ms``):
+every chain moves until the budget has passed, so fast chains do not idle
+behind slow ones; each chain's trajectory does not depend on where the
+launches split it.  Moves are counted per chain from its own counters.
+Inputs are resident in HBM before timing starts; the warmup steps
+equilibrate the chains.
 
 Reported beside it: the plain single-chain FP64 drop-in rate (the Fortran
 driver over the C ABI on the 3392-atom CO2/N2 supercell), the ratio, the
@@ -135,7 +138,7 @@ def plain_fp64_rate(moves=1600):
 # batched workload
 # ---------------------------------------------------------------------------
 
-def build_batch(precision, n_chains=256):
+def build_batch(precision, n_chains=256, recompute_every=0):
     import fastmc_amd
     from fastmc_amd import synth
     from fastmc_amd.batch import Batch
@@ -148,7 +151,7 @@ def build_batch(precision, n_chains=256):
             ctxs[key] = fastmc_amd.Context.from_arrays(w.system_arrays())
         ctx = ctxs[key]
         b.add_chain(ctx, w.temp, [p * 1e5 for p in w.press_bar], w.seed,
-                    delr=w.delr)
+                    delr=w.delr, recompute_every=recompute_every)
         inf = ctx.info()
         info.append(dict(nk=inf["nk"], ns=ctx.nsites[0], nfw=w.fw.n,
                          volm=inf["volm"], rcut=w.rcut))
@@ -221,9 +224,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--moves", type=int, default=1000,
-                    help="trial moves per chain per step (one cycle: moves, "
-                         "then the FP64 recompute)")
+    ap.add_argument("--budget-ms", type=float, default=15.0,
+                    help="GPU time per step (one launch); every chain moves "
+                         "until it has passed")
+    ap.add_argument("--recompute-every", type=int, default=1000,
+                    help="moves between FP64 recomputes of a chain")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -251,17 +256,19 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline()
 
-    b, ws, info = build_batch(PREC_MIXED)
+    b, ws, info = build_batch(PREC_MIXED, recompute_every=args.recompute_every)
     n_chains = len(ws)
+    budget_us = args.budget_ms * 1e3
+    never = 10 ** 12
     for _ in range(args.warmup):
-        b.launch_cycles(1, args.moves)
+        b.launch_budget(never, budget_us)
         b.sync()
     before = [b.stats(c) for c in range(n_chains)]
-    # timed region: every chain resident in HBM, one graph replay per step
+    # timed region: every chain resident in HBM, one budgeted launch per step
     b.timing(reset=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        b.launch_cycles(1, args.moves)
+        b.launch_budget(never, budget_us)
     b.sync()
     dt = time.perf_counter() - t0
     kernel_ms, launches = b.timing()
@@ -287,11 +294,12 @@ def main():
         byt += bb * moves
         flo += ff * moves
     # per launch: algorithmic bytes / the batch kernel's average duration
-    # (HIP events on the batch stream, recorded inside the captured cycle)
+    # (HIP events on the batch stream around each launch)
     k_s = kernel_ms * 1e-3
     achieved = byt / k_s / 1e9
     valu = flo / k_s / 1e12
     drift = max(abs(a["drift"]) for a in after)
+    recomputes = sum(a["recomputes"] - z["recomputes"] for a, z in zip(after, before))
     nmol = [a["nmol"][0] for a in after]
     tile = b.tile()
     b.close()
@@ -316,8 +324,10 @@ def main():
         "config": {"workload": "256 batched CO2 chains: 32 frameworks 300-6000 "
                                "atoms x 8 pressures 0.01-10 bar, 298 K, kmax 8-14, "
                                "rcut 12.5 A, mixed precision, FP64 recompute "
-                               "every cycle",
-                   "chains": n_chains, "moves_per_cycle": args.moves,
+                               "every 1000 moves of each chain",
+                   "chains": n_chains, "step_budget_ms": args.budget_ms,
+                   "moves_per_chain_per_step_mean": total_moves / n_chains / args.steps,
+                   "recomputes_in_timed_region": recomputes,
                    "tile_threads": tile,
                    "guest_molecules_mean": sum(nmol) / len(nmol),
                    "guest_molecules_max": max(nmol)},

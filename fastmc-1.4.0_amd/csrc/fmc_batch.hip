@@ -126,6 +126,7 @@ struct BChain {
   long long log_cap, log_n;
   int fault;   // 1: guest capacity reached (reference error 75)
   long long stop_at;   // the chain idles once steps reaches this
+  long long recomp_every;   // > 0: FP64 recompute inside the launch every this many moves
   // FP64 full-energy bookkeeping: e_acc is the running sum of accepted
   // estep from the full energy at admission; e_full the last full
   // recompute (kcal/mol); drift = e_acc - e_full at that recompute
@@ -607,6 +608,95 @@ __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
   }
 }
 
+// Exact S(k) of a chain: framework part plus every guest atom, FP64 with
+// the phase reduced to [-1/2, 1/2] before sincospi (ewald.f:368-630 for the
+// guest atoms).  One code path for the recompute kernel and the batch
+// kernel's in-launch recompute, so both give the same bits.
+__device__ __forceinline__ void exact_sk(const BSys& Y, const SysQ& Z, const BChain& C,
+                                         int k, double& sx, double& sy) {
+  const int v = Y.kv[k];
+  const int l = ul(v), m = um(v), n = un(v);
+  const double* rc = Z.rcell;
+  sx = Y.sfw_re[k];
+  sy = Y.sfw_im[k];
+  for (int g = 0; g < Z.ngt; ++g) {
+    const int nat = C.nmol[g] * Z.ns[g];
+    for (int a = 0; a < nat; ++a) {
+      const double x = C.gx[g][a], y = C.gy[g][a], z = C.gz[g][a];
+      double ph = l * (rc[0] * x + rc[3] * y + rc[6] * z) +
+                  m * (rc[1] * x + rc[4] * y + rc[7] * z) +
+                  n * (rc[2] * x + rc[5] * y + rc[8] * z);
+      ph -= rint(ph);
+      double sv, cv;
+      sincospi(2.0 * ph, &sv, &cv);
+      const double q = Z.gq[g][a % Z.ns[g]];
+      sx += q * cv;
+      sy += q * sv;
+    }
+  }
+}
+
+// Real-space energy of all guests (single_point's pair part): guest atom i
+// against its cell's framework candidates and every later guest atom of
+// another molecule; threads tid, tid+nt, ... share the pairs of each i.
+__device__ __forceinline__ void guest_pairs(const BSys& Y, const SysQ& Z, const BChain& C,
+                                            int tid, int nt, double& e2, double& ev) {
+  int gpre[kMaxGuestTypes + 1];
+  int na = 0;
+  for (int g = 0; g < kMaxGuestTypes; ++g) {
+    gpre[g] = na;
+    if (g < Z.ngt) na += C.nmol[g] * Z.ns[g];
+  }
+  gpre[kMaxGuestTypes] = na;
+  for (int i = 0; i < na; ++i) {
+    int g = 0;
+    while (g + 1 < Z.ngt && gpre[g + 1] <= i) ++g;
+    const int la = i - gpre[g];
+    const int s = la % Z.ns[g];
+    const int mol = la / Z.ns[g];
+    const double xi = C.gx[g][la], yi = C.gy[g][la], zi = C.gz[g][la];
+    const double ce = Z.gq[g][s] / Z.epsq * kR4pie0;
+    const int ts = Z.gtype[g][s];
+    const int c = cell_of(Z, xi, yi, zi);
+    const int lo = c * Y.gcap;
+    for (int t = tid; t < Y.gcap; t += nt) {
+      const double* rr = &Y.grec[lo + t].x;
+      pair_fp64(Y.S, ce, ts, xi - gload(rr), yi - gload(rr + 1), zi - gload(rr + 2),
+                gload(rr + 3), gload(Y.grec_t + lo + t), e2, ev);
+    }
+    for (int j = i + 1 + tid; j < na; j += nt) {
+      int h = 0;
+      while (h + 1 < Z.ngt && gpre[h + 1] <= j) ++h;
+      const int lb = j - gpre[h];
+      if (h == g && lb / Z.ns[h] == mol) continue;
+      const int sb = lb % Z.ns[h];
+      pair_fp64(Y.S, ce, ts, xi - C.gx[h][lb], yi - C.gy[h][lb], zi - C.gz[h][lb],
+                Z.gq[h][sb], Z.gtype[h][sb], e2, ev);
+    }
+  }
+}
+
+// Thread 0's part of a recompute: the FP64 totals into the chain's
+// bookkeeping (see recompute_kernel).
+__device__ __forceinline__ void recompute_finish(const SysQ& Z, BChain& C, double erec,
+                                                 double e2, double ev, double dev, int base) {
+  double e = Z.rvolm2 * kR4pie0 * erec / Z.epsq;
+  for (int g = 0; g < Z.ngt; ++g) e += C.nmol[g] * (Z.engsic[g] + Z.e3[g]);
+  e += e2 + ev;
+  // the move loop's tail bookkeeping starts from elrc = 0 and adds
+  // delrc = tail(N +/- 1) - elrc (gcmc.f:891-901): compare the running
+  // sum with the same tail it carries, report the true one in e_full
+  const double book = (e + C.elrc) / C.engunit;
+  C.e_full = (e + tail_of(Z, C.nmol)) / C.engunit;
+  if (base) {
+    C.e_acc = book;
+    for (int g = 0; g < Z.ngt; ++g) C.energy[g] = C.e_full;
+  }
+  C.drift = C.e_acc - book;
+  C.sk_dev = dev;
+  C.n_recompute++;
+}
+
 // ---------------------------------------------------------------------------
 // The persistent kernel
 // ---------------------------------------------------------------------------
@@ -621,13 +711,15 @@ __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
 template <int BT, int KPT, bool MIXED, bool STREAM>
 __global__ void __launch_bounds__(BT)
     batch_kernel(const BSys* __restrict__ systems, BChain* __restrict__ chains,
-                 const int* __restrict__ ids, long long nmoves, int nkcap) {
+                 const int* __restrict__ ids, long long nmoves, int nkcap,
+                 long long budget_ticks) {
   using T = typename std::conditional<MIXED, float, double>::type;
   __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
   __shared__ Cplx<T> zt[2 * kMaxSites * kZRow];           // Z_p[n], signed n
   __shared__ MoveDesc D;
   __shared__ float rng_u[97];
-  __shared__ double red[3][BT / kWave];
+  __shared__ double red[4][BT / kWave];
+  __shared__ int stop_flag;
   __shared__ int shift_lo, shift_hi;
   __shared__ BChain C;
   __shared__ SysQ Z;   // the system's small constants, for the whole launch
@@ -684,6 +776,12 @@ __global__ void __launch_bounds__(BT)
     }
   }
   Rng R;
+  // time budget (constant-rate counter): thread 0 raises stop_flag after
+  // the move that crosses it; the flag is read after a barrier
+  const long long t_start = (tid == 0 && budget_ticks > 0)
+                                ? static_cast<long long>(__builtin_amdgcn_s_memrealtime())
+                                : 0;
+  if (tid == 0) stop_flag = 0;
   __syncthreads();
   if (tid == 0) {
     for (int i = 0; i < 97; ++i) rng_u[i] = C.u[i];
@@ -693,9 +791,9 @@ __global__ void __launch_bounds__(BT)
   __syncthreads();
 
   for (long long mv = 0; mv < nmoves; ++mv) {
-    // C.steps was last written before the barrier ending the previous move:
-    // every thread takes the same branch
-    if (C.steps >= C.stop_at) break;
+    // C.steps and stop_flag were last written before the barrier ending the
+    // previous move: every thread takes the same branch
+    if (C.steps >= C.stop_at || stop_flag) break;
     if (tid == 0) propose(Z, C, R, D);
     __syncthreads();
     const int kind = D.kind;
@@ -924,6 +1022,9 @@ __global__ void __launch_bounds__(BT)
       }
       a = Z.rvolm2 * kR4pie0 * a / Z.epsq;
       decide(Z, C, R, D, a, b, c);
+      if (budget_ticks > 0 &&
+          static_cast<long long>(__builtin_amdgcn_s_memrealtime()) - t_start >= budget_ticks)
+        stop_flag = 1;
       shift_lo = 0;
       shift_hi = 0;
       if (D.accepted && kind == FMC_DELETE) {
@@ -982,6 +1083,64 @@ __global__ void __launch_bounds__(BT)
       }
     }
     __syncthreads();
+    // FP64 recompute every recomp_every moves of this chain, inside the
+    // launch: S(k) <- exact, full energy, drift (as recompute_kernel; the
+    // same exact_sk, so the same S bits).  Uniform branch (LDS values
+    // written before the barrier above).
+    if (C.recomp_every > 0 && C.steps % C.recomp_every == 0) {
+      double erec = 0.0, dev = 0.0;
+      if (STREAM) {
+        for (int k = tid; k < nk; k += BT) {
+          double sx, sy;
+          exact_sk(Y, Z, C, k, sx, sy);
+          const double fr = Y.sfw_re[k], fi = Y.sfw_im[k];
+          dev = fmax(dev, fmax(fabs(sx - gload(C.sre + k)), fabs(sy - gload(C.sim + k))));
+          gstore(C.sre + k, sx);
+          gstore(C.sim + k, sy);
+          erec += Y.ak[k] * ((sx - fr) * (sx + fr) + (sy - fi) * (sy + fi));
+        }
+      } else {
+        for (int j = 0; j < kpt; ++j) {   // not unrolled: one copy of exact_sk
+          const int k = tid + j * BT;
+          if (k >= nk) break;
+          double sx, sy;
+          exact_sk(Y, Z, C, k, sx, sy);
+          const double fr = Y.sfw_re[k], fi = Y.sfw_im[k];
+#pragma unroll
+          for (int jj = 0; jj < KPT; ++jj)
+            if (jj == j) {
+              dev = fmax(dev, fmax(fabs(sx - sre[jj]), fabs(sy - sim[jj])));
+              sre[jj] = sx;
+              sim[jj] = sy;
+            }
+          erec += ak_l[k] * ((sx - fr) * (sx + fr) + (sy - fi) * (sy + fi));
+        }
+      }
+      double e2r = 0.0, evr = 0.0;
+      guest_pairs(Y, Z, C, tid, BT, e2r, evr);
+      erec = wave_sum_dpp(erec);
+      e2r = wave_sum_dpp(e2r);
+      evr = wave_sum_dpp(evr);
+      for (int off = kWave / 2; off > 0; off >>= 1) dev = fmax(dev, __shfl_xor(dev, off, kWave));
+      if ((tid % kWave) == kWave - 1) {
+        red[0][tid / kWave] = erec;
+        red[1][tid / kWave] = e2r;
+        red[2][tid / kWave] = evr;
+        red[3][tid / kWave] = dev;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double a = 0.0, b = 0.0, cc = 0.0, d = 0.0;
+        for (int w = 0; w < BT / kWave; ++w) {
+          a += red[0][w];
+          b += red[1][w];
+          cc += red[2][w];
+          d = fmax(d, red[3][w]);
+        }
+        recompute_finish(Z, C, a, b, cc, d, 0);
+      }
+      __syncthreads();
+    }
   }
   // write the state back
   if (!STREAM) {
@@ -1134,78 +1293,22 @@ constexpr int kRecBT = 256;
 
 __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
   __shared__ double red[4][kRecBT / kWave];
-  __shared__ int gpre[kMaxGuestTypes + 1];
   const int tid = threadIdx.x;
   BChain& C = P.chains[P.one_id >= 0 ? P.one_id : P.ids[blockIdx.x]];
   const BSys& Y = P.one ? *P.one : P.sys[C.sys];
-  if (tid == 0) {
-    int na = 0;
-    for (int g = 0; g < Y.q.ngt; ++g) {
-      gpre[g] = na;
-      na += C.nmol[g] * Y.q.ns[g];
-    }
-    gpre[Y.q.ngt] = na;
-  }
-  __syncthreads();
-  const int nga = gpre[Y.q.ngt];
-  const double* rc = Y.S.rcell;
-  // reciprocal space
   double erec = 0.0, dev = 0.0;
   for (int k = tid; k < Y.nk; k += kRecBT) {
-    const int v = Y.kv[k];
-    const int l = ul(v), m = um(v), n = un(v);
+    double sx, sy;
+    exact_sk(Y, Y.q, C, k, sx, sy);
     const double fr = Y.sfw_re[k], fi = Y.sfw_im[k];
-    double sx = fr, sy = fi;
-    for (int g = 0; g < Y.q.ngt; ++g) {
-      const int nat = C.nmol[g] * Y.q.ns[g];
-      for (int a = 0; a < nat; ++a) {
-        const double x = C.gx[g][a], y = C.gy[g][a], z = C.gz[g][a];
-        double ph = l * (rc[0] * x + rc[3] * y + rc[6] * z) +
-                    m * (rc[1] * x + rc[4] * y + rc[7] * z) +
-                    n * (rc[2] * x + rc[5] * y + rc[8] * z);
-        ph -= rint(ph);
-        double sv, cv;
-        sincospi(2.0 * ph, &sv, &cv);
-        const double q = Y.q.gq[g][a % Y.q.ns[g]];
-        sx += q * cv;
-        sy += q * sv;
-      }
-    }
     dev = fmax(dev, fmax(fabs(sx - C.sre[k]), fabs(sy - C.sim[k])));
     C.sre[k] = sx;
     C.sim[k] = sy;
     // |S|^2 - |S_fw|^2 = (S - S_fw).(S + S_fw)
     erec += Y.ak[k] * ((sx - fr) * (sx + fr) + (sy - fi) * (sy + fi));
   }
-  // real space: guest atom i against its cell's framework candidates and
-  // against every later guest atom of another molecule
   double e2 = 0.0, ev = 0.0;
-  for (int i = 0; i < nga; ++i) {
-    int g = 0;
-    while (g + 1 < Y.q.ngt && gpre[g + 1] <= i) ++g;
-    const int la = i - gpre[g];
-    const int s = la % Y.q.ns[g];
-    const int mol = la / Y.q.ns[g];
-    const double xi = C.gx[g][la], yi = C.gy[g][la], zi = C.gz[g][la];
-    const double ce = Y.q.gq[g][s] / Y.S.epsq * kR4pie0;
-    const int ts = Y.q.gtype[g][s];
-    const int c = cell_of(Y.q, xi, yi, zi);
-    const int lo = c * Y.gcap;
-    for (int t = tid; t < Y.gcap; t += kRecBT) {
-      const Rec rr = Y.grec[lo + t];
-      pair_fp64(Y.S, ce, ts, xi - rr.x, yi - rr.y, zi - rr.z, rr.q, Y.grec_t[lo + t], e2,
-                ev);
-    }
-    for (int j = i + 1 + tid; j < nga; j += kRecBT) {
-      int h = 0;
-      while (h + 1 < Y.q.ngt && gpre[h + 1] <= j) ++h;
-      const int lb = j - gpre[h];
-      if (h == g && lb / Y.q.ns[h] == mol) continue;
-      const int sb = lb % Y.q.ns[h];
-      pair_fp64(Y.S, ce, ts, xi - C.gx[h][lb], yi - C.gy[h][lb], zi - C.gz[h][lb],
-                Y.q.gq[h][sb], Y.q.gtype[h][sb], e2, ev);
-    }
-  }
+  guest_pairs(Y, Y.q, C, tid, kRecBT, e2, ev);
   erec = wave_sum(erec);
   e2 = wave_sum(e2);
   ev = wave_sum(ev);
@@ -1225,21 +1328,7 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
       cc += red[2][w];
       d = fmax(d, red[3][w]);
     }
-    double e = Y.q.rvolm2 * kR4pie0 * a / Y.S.epsq;
-    for (int g = 0; g < Y.q.ngt; ++g) e += C.nmol[g] * (Y.q.engsic[g] + Y.q.e3[g]);
-    e += b + cc;
-    // the move loop's tail bookkeeping starts from elrc = 0 and adds
-    // delrc = tail(N +/- 1) - elrc (gcmc.f:891-901): compare the running
-    // sum with the same tail it carries, report the true one in e_full
-    const double book = (e + C.elrc) / C.engunit;
-    C.e_full = (e + tail_of(Y.q, C.nmol)) / C.engunit;
-    if (P.base) {
-      C.e_acc = book;
-      for (int g = 0; g < Y.q.ngt; ++g) C.energy[g] = C.e_full;
-    }
-    C.drift = C.e_acc - book;
-    C.sk_dev = d;
-    C.n_recompute++;
+    recompute_finish(Y.q, C, a, b, cc, d, P.base);
     P.maxdev[blockIdx.x] = d;
   }
 }
@@ -1437,7 +1526,7 @@ struct fmc_batch {
 
 namespace {
 
-using KernelFn = void (*)(const BSys*, BChain*, const int*, long long, int);
+using KernelFn = void (*)(const BSys*, BChain*, const int*, long long, int, long long);
 
 struct KCfg {
   int bt, kpt;   // kpt 0: the streaming form
@@ -1756,7 +1845,7 @@ int lds_need(fmc_batch* b, const KCfg* cfg, size_t* out) {
   return FMC_OK;
 }
 
-int launch(fmc_batch* b, long long n, int bt, int kpt) {
+int launch(fmc_batch* b, long long n, int bt, int kpt, long long budget_ticks = 0) {
   const KCfg* cfg = find_cfg(bt, kpt, b->precision == FMC_PREC_MIXED);
   if (!cfg) return FMC_ERR_UNSUPPORTED;
   if (b->active_ids.empty()) return FMC_OK;
@@ -1766,7 +1855,7 @@ int launch(fmc_batch* b, long long n, int bt, int kpt) {
   if ((r = lds_need(b, cfg, &lds))) return r;
   hipLaunchKernelGGL(cfg->fn, dim3(static_cast<unsigned>(b->active_ids.size())),
                      dim3(cfg->bt), lds, b->stream, b->d_sys, b->d_chains, b->d_ids, n,
-                     b->max_nk);
+                     b->max_nk, budget_ticks);
   BCHECK(hipGetLastError());
   return FMC_OK;
 }
@@ -2060,6 +2149,7 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
     C.rota_rotangle = C.rotangle;
     C.elrc = 0.0;
     C.stop_at = p->max_moves > 0 ? p->max_moves : std::numeric_limits<long long>::max();
+    C.recomp_every = p->recompute_every > 0 ? p->recompute_every : 0;
     duni_setup(p->seed[0], p->seed[1], p->seed[2], p->seed[3], &C);
     for (auto& kb : K) ch->bytes += kb->n;
     // the slot is outside any launch in flight: its struct and the
@@ -2146,6 +2236,28 @@ extern "C" int fmc_batch_launch(fmc_batch* b, long long n) {
   // kernel time on the batch's own stream
   BCHECK(hipEventRecord(b->ev0, b->stream));
   if ((r = launch(b, n, b->bt, b->kpt))) return r;
+  BCHECK(hipEventRecord(b->ev1, b->stream));
+  b->timing_pending = true;
+  b->timing_mode = 1;
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_launch_budget(fmc_batch* b, long long max_moves,
+                                       double budget_us) {
+  if (!b || max_moves < 0 || budget_us <= 0.0) return FMC_ERR_ARG;
+  int r;
+  if ((r = fmc_batch_sync(b))) return r;
+  std::lock_guard<std::mutex> g(b->mu);
+  if ((r = sync_tables(b))) return r;
+  if (b->active_ids.empty()) return FMC_OK;
+  if ((r = ensure_tile(b))) return r;
+  int dev = 0, khz = 0;
+  BCHECK(hipGetDevice(&dev));
+  BCHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+  if (khz <= 0) return FMC_ERR_UNSUPPORTED;
+  const long long ticks = std::max(1LL, static_cast<long long>(budget_us * 1e-3 * khz));
+  BCHECK(hipEventRecord(b->ev0, b->stream));
+  if ((r = launch(b, max_moves, b->bt, b->kpt, ticks))) return r;
   BCHECK(hipEventRecord(b->ev1, b->stream));
   b->timing_pending = true;
   b->timing_mode = 1;

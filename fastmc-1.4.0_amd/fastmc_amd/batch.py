@@ -37,7 +37,8 @@ class Batch:
     def add_chain(self, ctx, temp: float, press_pa: Sequence[float],
                   seed: Sequence[int], freq: Optional[Sequence[Sequence[float]]] = None,
                   delr: float = 1.0, eq_steps: int = 0, engunit: float = 418.4,
-                  ratios=(0.5, 0.5, 0.5), max_moves: int = 0) -> int:
+                  ratios=(0.5, 0.5, 0.5), max_moves: int = 0,
+                  recompute_every: int = 0) -> int:
         p = ChainParams()
         p.temp = temp
         p.ntpguest = len(press_pa)
@@ -55,6 +56,7 @@ class Batch:
         p.engunit = engunit
         p.eq_steps = eq_steps
         p.max_moves = max_moves
+        p.recompute_every = recompute_every
         cid = C.c_int()
         check(self.L.fmc_batch_add_chain(self.h, ctx.h, C.byref(p), C.byref(cid)),
               "fmc_batch_add_chain")
@@ -70,6 +72,12 @@ class Batch:
 
     def launch(self, n: int):
         check(self.L.fmc_batch_launch(self.h, n), "fmc_batch_launch")
+
+    def launch_budget(self, max_moves: int, budget_us: float):
+        """Enqueue a launch in which every chain moves until budget_us of
+        GPU time has passed (or max_moves)."""
+        check(self.L.fmc_batch_launch_budget(self.h, max_moves, budget_us),
+              "fmc_batch_launch_budget")
 
     def sync(self):
         check(self.L.fmc_batch_sync(self.h), "fmc_batch_sync")

@@ -35,7 +35,8 @@ class ChainParams(C.Structure):
                 ("freq", (C.c_double * 9) * 4), ("delr", C.c_double * 4),
                 ("disp_ratio", C.c_double * 4), ("tran_ratio", C.c_double * 4),
                 ("rota_ratio", C.c_double), ("engunit", C.c_double),
-                ("eq_steps", C.c_longlong), ("max_moves", C.c_longlong)]
+                ("eq_steps", C.c_longlong), ("max_moves", C.c_longlong),
+                ("recompute_every", C.c_longlong)]
 
 
 class ChainStats(C.Structure):
@@ -83,6 +84,7 @@ _SIGS = {
     "fmc_batch_run": [C.c_void_p, C.c_longlong],
     "fmc_batch_launch": [C.c_void_p, C.c_longlong],
     "fmc_batch_sync": [C.c_void_p],
+    "fmc_batch_launch_budget": [C.c_void_p, C.c_longlong, C.c_double],
     "fmc_batch_query": [C.c_void_p, IP],
     "fmc_batch_stats": [C.c_void_p, C.c_int, C.POINTER(ChainStats)],
     "fmc_batch_enable_log": [C.c_void_p, C.c_int, C.c_longlong],

@@ -162,6 +162,10 @@ typedef struct fmc_chain_params {
   long long eq_steps;   /* moves before production averaging starts */
   long long max_moves;  /* the chain stops after this many moves (0 = no
                          * limit); a stopped chain idles in later launches */
+  long long recompute_every; /* > 0: FP64 recompute (as fmc_batch_recompute)
+                         * inside the launches, every this many moves of
+                         * the chain, so its cadence does not depend on how
+                         * the moves are split into launches */
 } fmc_chain_params;
 
 typedef struct fmc_chain_stats {
@@ -197,6 +201,11 @@ int fmc_batch_run(fmc_batch *b, long long n_moves);
 /* enqueue without waiting / wait for completion */
 int fmc_batch_launch(fmc_batch *b, long long n_moves);
 int fmc_batch_sync(fmc_batch *b);
+/* enqueue a launch in which each chain moves until budget_us of GPU time
+ * has passed (checked after every move) or it has made max_moves moves;
+ * the per-chain move counts differ, the chains' trajectories do not depend
+ * on where the launches split them */
+int fmc_batch_launch_budget(fmc_batch *b, long long max_moves, double budget_us);
 /* *done = 1 when nothing enqueued on the batch is still running (does not
  * wait) */
 int fmc_batch_query(fmc_batch *b, int *done);
