@@ -189,11 +189,11 @@ def serving_leg(n_req=400, rate=200.0, moves=5000):
     """Isotherm-point serving: Poisson arrivals at `rate`/s, each request a
     framework of 300-10000 atoms with 1-3 of CO2/N2/CH4, 0.01-30 bar,
     273-348 K, `moves` trial moves, mixed precision; continuous batching
-    (admission by HBM footprint, slices of 250 moves).  Latency = arrival to
-    result."""
+    (admission by HBM footprint on its own thread, 4 ms budgeted slices).
+    Latency = arrival to result."""
     from fastmc_amd import serve
     reqs = serve.poisson_requests(n_req, rate, seed=2024, moves=moves)
-    srv = serve.Server(slots=256, slice_moves=250, tile=512)
+    srv = serve.Server(slots=256, slice_ms=4.0, tile=512)
     t0 = time.perf_counter()
     res = srv.run(reqs, realtime=True)
     wall = time.perf_counter() - t0

@@ -6,7 +6,8 @@ are admitted as chain slots and device memory free up (admission by the
 chain's projected HBM footprint, ``fmc_chain_footprint``); an admitted
 request becomes one device-resident chain (coordinates, S(k), RNG and
 statistics stay in HBM until it finishes).  The GPU advances every
-admitted chain in slices of ``slice_moves`` moves; requests that arrive
+admitted chain in slices of ``slice_moves`` moves (or of ``slice_ms`` of
+GPU time, every chain moving at its own speed); requests that arrive
 while a slice runs are built and admitted on the host meanwhile and join
 the next slice, and finished chains leave at slice boundaries.
 
@@ -59,6 +60,7 @@ class Server:
     slots: int = 256
     precision: int = PREC_MIXED
     slice_moves: int = 250
+    slice_ms: float = 0.0          # > 0: time-budgeted slices instead
     hbm_budget: int = 64 << 30     # bytes of chain state admitted at once
     tile: int = 256
     clock: Callable[[], float] = time.perf_counter
@@ -141,7 +143,10 @@ class Server:
                     if not self._live:
                         cv.wait(timeout=0.01)
                         continue
-                self.batch.launch(self.slice_moves)
+                if self.slice_ms > 0:
+                    self.batch.launch_budget(10 ** 12, self.slice_ms * 1e3)
+                else:
+                    self.batch.launch(self.slice_moves)
                 self.slices += 1
                 self.batch.sync()
                 self._retire(self.clock() - t0, cv)

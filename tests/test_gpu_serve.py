@@ -29,3 +29,21 @@ def test_serve_same_alone_and_in_mix():
         for key in KEYS:
             assert alone.stats[key] == res[k].stats[key], (k, key)
     print("latency ms:", [round(1e3 * r.latency, 1) for r in res])
+
+
+def test_serve_budget_slices_same_results():
+    """Time-budgeted slices split each request's moves at other points; the
+    results equal those of fixed slices, request by request."""
+    import fastmc_amd
+    from fastmc_amd import serve
+    fastmc_amd.lib()
+    reqs = serve.poisson_requests(5, rate=2000.0, seed=12, moves=600)
+    a = serve.Server(slots=8, slice_moves=200)
+    ra = a.run(reqs, realtime=False)
+    a.close()
+    b = serve.Server(slots=8, slice_ms=1.5)
+    rb = b.run(reqs, realtime=True)
+    b.close()
+    for x, y in zip(ra, rb):
+        for key in KEYS:
+            assert x.stats[key] == y.stats[key], (x.rid, key)
