@@ -529,3 +529,175 @@ def write_control(path: str, title: str, temp: float, guests_bar,
     out.append("finish")
     with open(path, "w") as f:
         f.write("\n".join(out) + "\n")
+
+
+# ---------------------------------------------------------------------------
+# REVIVE / REVCON (restart files, utility_pack.f:1772-1862)
+# ---------------------------------------------------------------------------
+
+def read_fortran_records(path: str) -> List[bytes]:
+    """Records of a Fortran sequential unformatted file (4-byte little-endian
+    length markers before and after each record)."""
+    import struct
+    raw = open(path, "rb").read()
+    recs, p = [], 0
+    while p < len(raw):
+        if p + 4 > len(raw):
+            raise ValueError(f"{path}: truncated record marker at byte {p}")
+        n = struct.unpack_from("<i", raw, p)[0]
+        if n < 0 or p + 8 + n > len(raw):
+            raise ValueError(f"{path}: bad record length {n} at byte {p}")
+        tail = struct.unpack_from("<i", raw, p + 4 + n)[0]
+        if tail != n:
+            raise ValueError(f"{path}: record markers {n} and {tail} differ at byte {p}")
+        recs.append(raw[p + 4:p + 4 + n])
+        p += 8 + n
+    return recs
+
+
+def write_fortran_records(path: str, recs: List[bytes]) -> None:
+    import struct
+    with open(path, "wb") as f:
+        for r in recs:
+            m = struct.pack("<i", len(r))
+            f.write(m + r + m)
+
+
+@dataclasses.dataclass
+class Revive:
+    """The restart state the reference writes every 1000 moves (``revive``,
+    utility_pack.f:1800-1814) and reads back (``revread``, :1835-1862)."""
+    production: int                 # the LOGICAL as stored (non-zero = true)
+    nummols: List[int]              # per FIELD molecule type
+    chainstats: List[float]         # 1 + 14 per guest type (gcmc.f:216-230)
+    natms: int                      # guest + framework atoms
+    coords: List[Tuple["object", "object", "object"]]   # per molecule type,
+                                    # sites molecule-major (framwkxxx(mol, :))
+    energy: List[float]             # per guest type (kcal/mol)
+
+    @property
+    def is_production(self) -> bool:
+        return self.production != 0
+
+
+def read_revive(path: str, field: Field) -> Revive:
+    """revread (utility_pack.f:1835-1862): the record sizes are checked
+    against the FIELD's molecule types."""
+    import numpy as np
+    recs = read_fortran_records(path)
+    nt = len(field.molecules)
+    ng = len(field.guests)
+    if len(recs) != 5 + 3 * nt:
+        raise ValueError(f"{path}: {len(recs)} records, expected {5 + 3 * nt} "
+                         f"for {nt} molecule types")
+    production = int(np.frombuffer(recs[0], "<i4")[0])
+    nummols = [int(v) for v in np.frombuffer(recs[1], "<i4")]
+    chainstats = [float(v) for v in np.frombuffer(recs[2], "<f8")]
+    natms = int(np.frombuffer(recs[3], "<i4")[0])
+    if len(nummols) != nt or len(chainstats) != 14 * ng + 1:
+        raise ValueError(f"{path}: {len(nummols)} molecule counts and "
+                         f"{len(chainstats)} statistics do not fit the FIELD")
+    coords = []
+    total = 0
+    for k, mol in enumerate(field.molecules):
+        n = nummols[k] * len(mol.names)
+        xyz = tuple(np.frombuffer(recs[4 + 3 * k + c], "<f8").copy() for c in range(3))
+        if any(len(a) != n for a in xyz):
+            raise ValueError(f"{path}: molecule type {k + 1} holds "
+                             f"{len(xyz[0])} sites, expected {n}")
+        coords.append(xyz)
+        total += n
+    if total != natms:
+        raise ValueError(f"{path}: {total} sites listed, natms = {natms}")
+    energy = [float(v) for v in np.frombuffer(recs[-1], "<f8")]
+    if len(energy) != ng:
+        raise ValueError(f"{path}: {len(energy)} energies for {ng} guest types")
+    return Revive(production, nummols, chainstats, natms, coords, energy)
+
+
+def write_revive(path: str, rev: Revive) -> None:
+    """revive (utility_pack.f:1800-1814)."""
+    import numpy as np
+    recs = [np.array([rev.production], "<i4").tobytes(),
+            np.array(rev.nummols, "<i4").tobytes(),
+            np.array(rev.chainstats, "<f8").tobytes(),
+            np.array([rev.natms], "<i4").tobytes()]
+    for xyz in rev.coords:
+        recs += [np.asarray(a, "<f8").tobytes() for a in xyz]
+    recs.append(np.array(rev.energy, "<f8").tobytes())
+    write_fortran_records(path, recs)
+
+
+def apply_revive(field: Field, config: Config, rev: Revive) -> Tuple[Field, Config]:
+    """Copies of field and config holding the REVIVE's molecule counts and
+    coordinates, as revread leaves the module arrays for the restart
+    (gcmc.f:361-386)."""
+    import copy
+    f = copy.deepcopy(field)
+    c = copy.deepcopy(config)
+    for k, mol in enumerate(f.molecules):
+        mol.nummols = rev.nummols[k]
+        x, y, z = rev.coords[k]
+        c.coords[k] = [(float(a), float(b), float(d)) for a, b, d in zip(x, y, z)]
+    return f, c
+
+
+def fortran_g(x: float, w: int = 20, d: int = 10) -> str:
+    """Fortran Gw.d output editing: F(w-4).(d-e) and four blanks when the
+    value rounded to d digits has decimal exponent 0 <= e <= d, else Ew.d."""
+    if x == 0.0:
+        return f"{0.0:.{d - 1}f}".rjust(w - 4) + "    "
+    m = f"{abs(x):.{d - 1}e}"
+    e = int(m.split("e")[1]) + 1
+    if 0 <= e <= d:
+        f = f"{x:.{d - e}f}" + ("." if e == d else "")
+        return f.rjust(w - 4) + "    "
+    return fortran_e(x, w, d)
+
+
+def fortran_e(x: float, w: int = 20, d: int = 10) -> str:
+    """Fortran Ew.d output editing: 0.ddd...E+ee, or 0.ddd...+eee when the
+    exponent needs three digits."""
+    if x == 0.0:
+        return ("0." + "0" * d + "E+00").rjust(w)
+    m, ex = f"{abs(x):.{d - 1}e}".split("e")
+    e = int(ex) + 1
+    tail = f"E{e:+03d}" if abs(e) <= 99 else f"{e:+04d}"
+    s = ("-" if x < 0 else "") + "0." + m.replace(".", "") + tail
+    return s.rjust(w)
+
+
+def revcon_atoms(field: Field, coords) -> List[Tuple[str, float, float, float]]:
+    """(name, x, y, z) in the order ``condense`` builds the atom arrays
+    (utility_pack.f:3337-3380): guest types first, then the frameworks, each
+    in FIELD order.  coords maps the FIELD molecule index to its sites
+    (Config.coords, or the (x, y, z) arrays of a Revive)."""
+    out = []
+    order = ([k for k, m in enumerate(field.molecules) if m.guest] +
+             [k for k, m in enumerate(field.molecules) if not m.guest])
+    for k in order:
+        mol = field.molecules[k]
+        c = coords[k]
+        if isinstance(c, tuple) and len(c) == 3 and not isinstance(c[0], tuple):
+            c = list(zip(*c))
+        ns = len(mol.names)
+        for i, (x, y, z) in enumerate(c):
+            out.append((mol.names[i % ns], float(x), float(y), float(z)))
+    return out
+
+
+def write_revcon(path: str, title: str, imcon: int, cell, atoms,
+                 eng: float = 0.0, levcfg: int = 0) -> None:
+    """revive's REVCON (utility_pack.f:1788-1798): a CONFIG-format file,
+    '(80a1)' title, '(2i10,e20.10)' levcfg imcon energy, '(3f20.12)' cell,
+    then per atom '(a8,i10)' name index and '(3g20.10)' x y z.  The file can
+    be read back as a CONFIG."""
+    out = [title[:80].ljust(80), f"{levcfg:10d}{imcon:10d}" + fortran_e(eng)]
+    if imcon > 0:
+        for row in range(3):
+            out.append("".join(f"{cell[3 * row + c]:20.12f}" for c in range(3)))
+    for i, (n, x, y, z) in enumerate(atoms):
+        out.append(f"{n[:8]:<8s}{i + 1:10d}")
+        out.append(fortran_g(x) + fortran_g(y) + fortran_g(z))
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")

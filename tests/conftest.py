@@ -1,5 +1,4 @@
 import os
-import struct
 import sys
 
 import numpy as np
@@ -28,16 +27,6 @@ def oracle():
     return pyoracle
 
 
-def read_fortran_unformatted(path):
-    raw = open(path, "rb").read()
-    recs, p = [], 0
-    while p < len(raw):
-        n = struct.unpack_from("<i", raw, p)[0]
-        recs.append(raw[p + 4:p + 4 + n])
-        p += 8 + n
-    return recs
-
-
 @pytest.fixture(scope="session")
 def restart_example():
     """The reference's completed example run (data fixtures)."""
@@ -48,20 +37,14 @@ def restart_example():
     config = inputs.read_config(os.path.join(d, "CONFIG"), field)
     control = inputs.read_control(os.path.join(d, "CONTROL"))
     kat = json.load(open(os.path.join(d, "known_answers.json")))
-    recs = read_fortran_unformatted(os.path.join(d, "REVIVE"))
+    rev = inputs.read_revive(os.path.join(d, "REVIVE"), field)
     return dict(field=field, config=config, control=control, kat=kat,
-                revive=recs, dir=d)
+                revive=rev, dir=d)
 
 
 def revive_config(ex):
-    """CONFIG of the example with the REVIVE guest coordinates loaded."""
-    import copy
-    field = copy.deepcopy(ex["field"])
-    config = copy.deepcopy(ex["config"])
-    recs = ex["revive"]
-    nummols = np.frombuffer(recs[1], "<i4")
-    gx, gy, gz = [np.frombuffer(recs[4 + i], "<f8") for i in range(3)]
-    field.molecules[0].nummols = int(nummols[0])
-    config.coords[0] = list(zip(gx, gy, gz))
-    energy = float(np.frombuffer(recs[-1], "<f8")[0])
-    return field, config, energy
+    """CONFIG of the example with the REVIVE guest coordinates loaded, and
+    the REVIVE's accumulated energy."""
+    from fastmc_amd import inputs
+    field, config = inputs.apply_revive(ex["field"], ex["config"], ex["revive"])
+    return field, config, ex["revive"].energy[0]
