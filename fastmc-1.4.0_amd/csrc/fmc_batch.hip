@@ -122,6 +122,7 @@ struct BChain {
   long long steps, prod_steps;
   long long accepted[6], attempted[6];
   double sum_n[kMaxGuestTypes], sum_n2[kMaxGuestTypes], sum_e[kMaxGuestTypes];
+  double sum_en[kMaxGuestTypes], sum_e2[kMaxGuestTypes];
   double* log;
   long long log_cap, log_n;
   int fault;   // 1: guest capacity reached (reference error 75)
@@ -589,9 +590,12 @@ __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
     C.prod_steps++;
     for (int g = 0; g < C.ntpguest; ++g) {
       const double n = C.nmol[g];
+      const double e = C.energy[g];
       C.sum_n[g] += n;
       C.sum_n2[g] += n * n;
-      C.sum_e[g] += C.energy[g];
+      C.sum_e[g] += e;
+      C.sum_en[g] += e * n;
+      C.sum_e2[g] += e * e;
     }
   }
   if (C.log && C.log_n < C.log_cap) {
@@ -1425,7 +1429,14 @@ uint64_t system_key(const SystemDesc& s) {
   h = fnv(h, s.vdw_key.data(), s.vdw_key.size() * sizeof(int));
   h = fnv(h, s.vdw_a.data(), s.vdw_a.size() * sizeof(int));
   h = fnv(h, s.vdw_b.data(), s.vdw_b.size() * sizeof(int));
-  for (const auto& g : s.guests) h = fnv(h, &g, sizeof(GuestTemplate));
+  // template fields one by one: the struct's padding and unused sites are
+  // not part of the system
+  for (const auto& g : s.guests) {
+    const size_t n = static_cast<size_t>(g.ns);
+    h = fnv(h, &g.ns, sizeof(int));
+    for (const double* a : {g.x, g.y, g.z, g.q, g.mass}) h = fnv(h, a, n * sizeof(double));
+    h = fnv(h, g.type, n * sizeof(int));
+  }
   return h;
 }
 
@@ -2052,6 +2063,64 @@ extern "C" int fmc_chain_footprint(const fmc_ctx* ctx, int precision, long long*
   return FMC_OK;
 }
 
+namespace {
+
+// Admission, first step (under the batch lock): a free slot, and the
+// context's system pinned if the batch already holds it (*sys = -1 if not).
+int reserve_slot(fmc_batch* b, uint64_t key, int* slot, int* sys) {
+  std::lock_guard<std::mutex> g(b->mu);
+  *slot = -1;
+  for (int i = 0; i < b->max_chains; ++i)
+    if (!b->chains[i] && !b->reserved[i]) {
+      *slot = i;
+      break;
+    }
+  if (*slot < 0) return FMC_ERR_CAPACITY;
+  b->reserved[*slot] = 1;
+  *sys = find_system(b, key);
+  if (*sys >= 0) b->systems[*sys]->refs++;
+  return FMC_OK;
+}
+
+// Admission, second step (no lock held): build the system's tables and
+// candidate records if the batch does not hold them yet.
+int ensure_system(fmc_batch* b, const fmc_ctx* ctx, uint64_t key, int* sys,
+                  BSys** d_self) {
+  if (*sys < 0) {
+    std::unique_ptr<SysHost> sh;
+    int r = build_system(b, ctx, key, &sh);
+    if (r) return r;
+    std::lock_guard<std::mutex> g(b->mu);
+    sh->refs = 1;
+    *sys = insert_system(b, std::move(sh));
+    refresh_limits(b);
+  }
+  std::lock_guard<std::mutex> g(b->mu);
+  if (d_self) *d_self = b->systems[*sys]->d_self;
+  return FMC_OK;
+}
+
+// Admission, last step: publish the chain in its slot, or give the slot
+// and the system reference back when r reports a failure.
+int publish_chain(fmc_batch* b, int slot, int sys, int r, std::unique_ptr<ChainHost> ch,
+                  const BChain& C, int* chain_id) {
+  std::lock_guard<std::mutex> g(b->mu);
+  b->reserved[slot] = 0;
+  if (r) {
+    ch.reset();
+    release_system(b, sys);
+    refresh_limits(b);
+    return r;
+  }
+  ch->active = true;
+  b->chains[slot] = std::move(ch);
+  b->h_chains[slot] = C;
+  if (chain_id) *chain_id = slot;
+  return FMC_OK;
+}
+
+}  // namespace
+
 extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
                                    const fmc_chain_params* p, int* chain_id) {
   if (!b || !ctx || !p || !ctx->sys.finalized) return FMC_ERR_ARG;
@@ -2063,36 +2132,13 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
   std::lock_guard<std::mutex> admit(b->admit_mu);
   const uint64_t key = system_key(ctx->sys);
   int slot = -1, sys = -1;
-  {
-    // reserve a slot and pin the system if the batch already holds it
-    std::lock_guard<std::mutex> g(b->mu);
-    for (int i = 0; i < b->max_chains; ++i)
-      if (!b->chains[i] && !b->reserved[i]) {
-        slot = i;
-        break;
-      }
-    if (slot < 0) return FMC_ERR_CAPACITY;
-    b->reserved[slot] = 1;
-    sys = find_system(b, key);
-    if (sys >= 0) b->systems[sys]->refs++;
-  }
-  int r = FMC_OK;
+  int r = reserve_slot(b, key, &slot, &sys);
+  if (r) return r;
   BSys* d_self = nullptr;
   std::unique_ptr<ChainHost> ch;
   BChain C;
   do {
-    if (sys < 0) {
-      std::unique_ptr<SysHost> sh;
-      if ((r = build_system(b, ctx, key, &sh))) break;
-      std::lock_guard<std::mutex> g(b->mu);
-      sh->refs = 1;
-      sys = insert_system(b, std::move(sh));
-      refresh_limits(b);
-    }
-    {
-      std::lock_guard<std::mutex> g(b->mu);
-      d_self = b->systems[sys]->d_self;
-    }
+    if ((r = ensure_system(b, ctx, key, &sys, &d_self))) break;
     ch = std::make_unique<ChainHost>();
     ch->sys = sys;
     std::memset(&C, 0, sizeof(C));
@@ -2172,19 +2218,7 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->setup) != hipSuccess)
       r = FMC_ERR_HIP;
   } while (false);
-  std::lock_guard<std::mutex> g(b->mu);
-  b->reserved[slot] = 0;
-  if (r) {
-    ch.reset();
-    release_system(b, sys);
-    refresh_limits(b);
-    return r;
-  }
-  ch->active = true;
-  b->chains[slot] = std::move(ch);
-  b->h_chains[slot] = C;
-  if (chain_id) *chain_id = slot;
-  return FMC_OK;
+  return publish_chain(b, slot, sys, r, std::move(ch), C, chain_id);
 }
 
 extern "C" int fmc_batch_remove_chain(fmc_batch* b, int id) {
@@ -2196,6 +2230,174 @@ extern "C" int fmc_batch_remove_chain(fmc_batch* b, int id) {
   b->chains[id].reset();
   release_system(b, sys);
   refresh_limits(b);
+  return FMC_OK;
+}
+
+namespace {
+
+// Chain checkpoint layout: this header, the BChain (device pointers
+// cleared), the running S(k) (re, im), then per guest type the live sites'
+// x, y, z.  Valid for the library build that wrote it (struct sizes and
+// version are checked).
+struct CkHeader {
+  char magic[8];
+  uint64_t key;          // system_key of the chain's system
+  int version;
+  int precision;         // the writing batch's (informational)
+  int nk;
+  int ntpguest;
+  int cap_atoms;
+  int ns[kMaxGuestTypes];
+  int chain_bytes;       // sizeof(BChain)
+};
+constexpr char kCkMagic[8] = {'F', 'M', 'C', 'C', 'H', 'K', 0, 0};
+constexpr int kCkVersion = 1;
+
+long long ck_size(const BChain& C, int nk, const int* ns) {
+  long long n = sizeof(CkHeader) + sizeof(BChain) + 2LL * nk * sizeof(double);
+  for (int g = 0; g < C.ntpguest; ++g)
+    n += 3LL * C.nmol[g] * ns[g] * sizeof(double);
+  return n;
+}
+
+void clear_pointers(BChain* C) {
+  C->sre = C->sim = C->dsre = C->dsim = nullptr;
+  for (int g = 0; g < kMaxGuestTypes; ++g) C->gx[g] = C->gy[g] = C->gz[g] = nullptr;
+  C->log = nullptr;
+  C->log_cap = C->log_n = 0;
+}
+
+}  // namespace
+
+extern "C" int fmc_batch_checkpoint(fmc_batch* b, int id, void* buf, long long cap,
+                                    long long* len) {
+  if (!b || !len || id < 0 || id >= b->max_chains) return FMC_ERR_ARG;
+  BCHECK(hipStreamSynchronize(b->stream));
+  std::lock_guard<std::mutex> g(b->mu);
+  if (!b->chains[id] || !b->chains[id]->active) return FMC_ERR_ARG;
+  BChain C;
+  BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
+  const SysHost& sh = *b->systems[C.sys];
+  const int nk = sh.dev.nk;
+  const int* ns = sh.dev.q.ns;
+  const long long need = ck_size(C, nk, ns);
+  *len = need;
+  if (!buf) return FMC_OK;
+  if (cap < need) return FMC_ERR_ARG;
+  char* out = static_cast<char*>(buf);
+  CkHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, kCkMagic, sizeof(h.magic));
+  h.key = sh.key;
+  h.version = kCkVersion;
+  h.precision = b->precision;
+  h.nk = nk;
+  h.ntpguest = C.ntpguest;
+  h.cap_atoms = C.cap_atoms;
+  for (int t = 0; t < kMaxGuestTypes; ++t) h.ns[t] = t < C.ntpguest ? ns[t] : 0;
+  h.chain_bytes = static_cast<int>(sizeof(BChain));
+  std::memcpy(out, &h, sizeof(h));
+  out += sizeof(h);
+  BChain D = C;
+  clear_pointers(&D);
+  std::memcpy(out, &D, sizeof(D));
+  out += sizeof(D);
+  BCHECK(hipMemcpy(out, C.sre, nk * sizeof(double), hipMemcpyDeviceToHost));
+  out += nk * sizeof(double);
+  BCHECK(hipMemcpy(out, C.sim, nk * sizeof(double), hipMemcpyDeviceToHost));
+  out += nk * sizeof(double);
+  for (int t = 0; t < C.ntpguest; ++t) {
+    const size_t n = static_cast<size_t>(C.nmol[t]) * ns[t];
+    for (double* src : {C.gx[t], C.gy[t], C.gz[t]}) {
+      if (n) BCHECK(hipMemcpy(out, src, n * sizeof(double), hipMemcpyDeviceToHost));
+      out += n * sizeof(double);
+    }
+  }
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_restore_chain(fmc_batch* b, const fmc_ctx* ctx, const void* buf,
+                                       long long len, int* chain_id) {
+  if (!b || !ctx || !buf || !ctx->sys.finalized ||
+      len < static_cast<long long>(sizeof(CkHeader) + sizeof(BChain)))
+    return FMC_ERR_ARG;
+  const char* in = static_cast<const char*>(buf);
+  CkHeader h;
+  std::memcpy(&h, in, sizeof(h));
+  in += sizeof(h);
+  BChain C;
+  std::memcpy(&C, in, sizeof(C));
+  in += sizeof(C);
+  const uint64_t key = system_key(ctx->sys);
+  if (std::memcmp(h.magic, kCkMagic, sizeof(h.magic)) != 0 || h.version != kCkVersion ||
+      h.chain_bytes != static_cast<int>(sizeof(BChain)) || h.key != key ||
+      h.nk != ctx->nk || h.ntpguest != static_cast<int>(ctx->sys.guests.size()) ||
+      h.ntpguest != C.ntpguest || h.cap_atoms != C.cap_atoms || C.cap_atoms < 1)
+    return FMC_ERR_ARG;
+  for (int t = 0; t < h.ntpguest; ++t)
+    if (h.ns[t] != ctx->sys.guests[t].ns || C.nmol[t] < 0 ||
+        static_cast<long long>(C.nmol[t]) * h.ns[t] > C.cap_atoms)
+      return FMC_ERR_ARG;
+  if (len != ck_size(C, h.nk, h.ns)) return FMC_ERR_ARG;
+  std::lock_guard<std::mutex> admit(b->admit_mu);
+  int slot = -1, sys = -1;
+  int r = reserve_slot(b, key, &slot, &sys);
+  if (r) return r;
+  std::unique_ptr<ChainHost> ch;
+  do {
+    if ((r = ensure_system(b, ctx, key, &sys, nullptr))) break;
+    ch = std::make_unique<ChainHost>();
+    ch->sys = sys;
+    C.sys = sys;
+    clear_pointers(&C);
+    auto& K = ch->keep;
+    const double* sre = reinterpret_cast<const double*>(in);
+    const double* sim = sre + h.nk;
+    if ((r = dput(K, &C.sre, sre, h.nk)) || (r = dput(K, &C.sim, sim, h.nk)) ||
+        (r = dput(K, &C.dsre, static_cast<double*>(nullptr), h.nk)) ||
+        (r = dput(K, &C.dsim, static_cast<double*>(nullptr), h.nk)))
+      break;
+    const double* site = sim + h.nk;
+    std::vector<double> stage(C.cap_atoms);
+    for (int t = 0; t < h.ntpguest && !r; ++t) {
+      const size_t n = static_cast<size_t>(C.nmol[t]) * h.ns[t];
+      double** dst[3] = {&C.gx[t], &C.gy[t], &C.gz[t]};
+      for (int c = 0; c < 3 && !r; ++c) {
+        std::fill(stage.begin(), stage.end(), 0.0);
+        std::memcpy(stage.data(), site, n * sizeof(double));
+        site += n;
+        r = dput(K, dst[c], stage.data(), C.cap_atoms);
+      }
+    }
+    if (r) break;
+    for (auto& kb : K) ch->bytes += kb->n;
+    if (hipMemcpyAsync(b->d_chains + slot, &C, sizeof(BChain), hipMemcpyHostToDevice,
+                       b->setup) != hipSuccess ||
+        hipStreamSynchronize(b->setup) != hipSuccess)
+      r = FMC_ERR_HIP;
+  } while (false);
+  return publish_chain(b, slot, sys, r, std::move(ch), C, chain_id);
+}
+
+extern "C" int fmc_batch_get_molecules(fmc_batch* b, int id, int gtype, double* x, double* y,
+                                       double* z, int cap_sites, int* nmol) {
+  if (!b || !nmol || id < 0 || id >= b->max_chains) return FMC_ERR_ARG;
+  BCHECK(hipStreamSynchronize(b->stream));
+  std::lock_guard<std::mutex> g(b->mu);
+  if (!b->chains[id] || !b->chains[id]->active) return FMC_ERR_ARG;
+  BChain C;
+  BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
+  if (gtype < 0 || gtype >= C.ntpguest) return FMC_ERR_ARG;
+  const int ns = b->systems[C.sys]->dev.q.ns[gtype];
+  *nmol = C.nmol[gtype];
+  if (!x && !y && !z) return FMC_OK;
+  const long long n = static_cast<long long>(C.nmol[gtype]) * ns;
+  if (!x || !y || !z || cap_sites < n) return FMC_ERR_ARG;
+  if (n) {
+    BCHECK(hipMemcpy(x, C.gx[gtype], n * sizeof(double), hipMemcpyDeviceToHost));
+    BCHECK(hipMemcpy(y, C.gy[gtype], n * sizeof(double), hipMemcpyDeviceToHost));
+    BCHECK(hipMemcpy(z, C.gz[gtype], n * sizeof(double), hipMemcpyDeviceToHost));
+  }
   return FMC_OK;
 }
 
@@ -2341,6 +2543,8 @@ extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
     st->sum_n[g] = C.sum_n[g];
     st->sum_n2[g] = C.sum_n2[g];
     st->sum_e[g] = C.sum_e[g];
+    st->sum_en[g] = C.sum_en[g];
+    st->sum_e2[g] = C.sum_e2[g];
     st->delrdisp[g] = C.delrdisp[g];
     st->tran_delr[g] = C.tran_delr[g];
   }

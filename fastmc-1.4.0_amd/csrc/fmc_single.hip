@@ -562,6 +562,7 @@ extern "C" int fmc_add_guest_type(fmc_ctx* c, int ns, const double* x,
   if (static_cast<int>(c->sys.guests.size()) >= kMaxGuestTypes)
     return FMC_ERR_CAPACITY;
   GuestTemplate g;
+  std::memset(&g, 0, sizeof(g));
   g.ns = ns;
   for (int i = 0; i < ns; ++i) {
     g.x[i] = x[i];

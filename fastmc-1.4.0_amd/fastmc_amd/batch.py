@@ -67,6 +67,72 @@ class Batch:
         check(self.L.fmc_batch_remove_chain(self.h, cid), "fmc_batch_remove_chain")
         self.ns.pop(cid, None)
 
+    def checkpoint(self, cid: int) -> bytes:
+        """The chain's whole state (duni table, counters, statistics, step
+        sizes, running S(k), guest coordinates) as opaque bytes; a chain
+        restored from them continues the identical trajectory."""
+        n = C.c_longlong()
+        check(self.L.fmc_batch_checkpoint(self.h, cid, None, 0, C.byref(n)),
+              "fmc_batch_checkpoint")
+        buf = C.create_string_buffer(n.value)
+        check(self.L.fmc_batch_checkpoint(self.h, cid, buf, n.value, C.byref(n)),
+              "fmc_batch_checkpoint")
+        return buf.raw[:n.value]
+
+    def restore_chain(self, ctx, blob: bytes) -> int:
+        """Add a chain from checkpoint bytes; ctx describes the chain's
+        system (its own guest molecules are not used)."""
+        cid = C.c_int()
+        check(self.L.fmc_batch_restore_chain(self.h, ctx.h, blob, len(blob), C.byref(cid)),
+              "fmc_batch_restore_chain")
+        self.ns[cid.value] = list(ctx.nsites)
+        return cid.value
+
+    def molecules(self, cid: int, gtype: int):
+        """(x, y, z) of the chain's guest molecules of type gtype, sites
+        molecule-major."""
+        n = C.c_int()
+        check(self.L.fmc_batch_get_molecules(self.h, cid, gtype, None, None, None, 0,
+                                             C.byref(n)), "fmc_batch_get_molecules")
+        ns = self.ns[cid][gtype]
+        m = n.value * ns
+        x, y, z = (np.zeros(max(1, m)) for _ in range(3))
+        check(self.L.fmc_batch_get_molecules(self.h, cid, gtype, x.ctypes.data_as(DP),
+                                             y.ctypes.data_as(DP), z.ctypes.data_as(DP),
+                                             m, C.byref(n)), "fmc_batch_get_molecules")
+        return x[:m], y[:m], z[:m]
+
+    def revive(self, cid: int, field, config):
+        """The chain's state as the reference's REVIVE record set
+        (utility_pack.f:1800-1814): guest molecule counts and coordinates
+        from the chain, framework molecules from config, the rolling
+        production averages <N>, <E>, <EN>, <N^2>, <E^2> (gcmc.f:1818-1848)
+        in chainstats, and the accumulated energies.  Entries the chain does
+        not track (<exp(-E/kT)>, the window deviations) are zero."""
+        from .inputs import Revive
+        st = self.stats(cid)
+        prod = st["prod_steps"]
+        guests = [k for k, m in enumerate(field.molecules) if m.guest]
+        nummols, coords = [], []
+        gi = {k: i for i, k in enumerate(guests)}
+        for k, mol in enumerate(field.molecules):
+            if k in gi:
+                x, y, z = self.molecules(cid, gi[k])
+                nummols.append(len(x) // len(mol.names))
+                coords.append((x.copy(), y.copy(), z.copy()))
+            else:
+                c = np.asarray(config.coords[k], dtype=float).reshape(-1, 3)
+                nummols.append(mol.nummols)
+                coords.append((c[:, 0].copy(), c[:, 1].copy(), c[:, 2].copy()))
+        stats = [float(prod)]
+        for i in range(len(guests)):
+            p = max(prod, 1)
+            stats += [st["sum_n"][i] / p, st["sum_e"][i] / p, st["sum_en"][i] / p,
+                      st["sum_n2"][i] / p, st["sum_e2"][i] / p] + [0.0] * 9
+        natms = sum(len(c[0]) for c in coords)
+        return Revive(-1 if prod > 0 else 0, nummols, stats, natms, coords,
+                      list(st["energy"]))
+
     def run(self, n: int):
         check(self.L.fmc_batch_run(self.h, n), "fmc_batch_run")
 
@@ -98,7 +164,8 @@ class Batch:
                     accepted=list(st.accepted), attempted=list(st.attempted),
                     nmol=list(st.nmol)[:ng], energy=list(st.energy)[:ng],
                     sum_n=list(st.sum_n)[:ng], sum_n2=list(st.sum_n2)[:ng],
-                    sum_e=list(st.sum_e)[:ng], delrdisp=list(st.delrdisp)[:ng],
+                    sum_e=list(st.sum_e)[:ng], sum_en=list(st.sum_en)[:ng],
+                    sum_e2=list(st.sum_e2)[:ng], delrdisp=list(st.delrdisp)[:ng],
                     rotangle=st.rotangle, e_acc=st.e_acc, e_full=st.e_full,
                     drift=st.drift, sk_dev=st.sk_dev, recomputes=st.recomputes,
                     capacity_fault=(r == -4))

@@ -48,7 +48,8 @@ class ChainStats(C.Structure):
                 ("rotangle", C.c_double), ("tran_delr", C.c_double * 4),
                 ("rota_rotangle", C.c_double), ("e_acc", C.c_double),
                 ("e_full", C.c_double), ("drift", C.c_double),
-                ("sk_dev", C.c_double), ("recomputes", C.c_longlong)]
+                ("sk_dev", C.c_double), ("recomputes", C.c_longlong),
+                ("sum_en", C.c_double * 4), ("sum_e2", C.c_double * 4)]
 
 
 _lib = None
@@ -81,6 +82,10 @@ _SIGS = {
     "fmc_batch_destroy": [C.c_void_p],
     "fmc_batch_add_chain": [C.c_void_p, C.c_void_p, C.POINTER(ChainParams), IP],
     "fmc_batch_remove_chain": [C.c_void_p, C.c_int],
+    "fmc_batch_checkpoint": [C.c_void_p, C.c_int, C.c_void_p, C.c_longlong,
+                             C.POINTER(C.c_longlong)],
+    "fmc_batch_restore_chain": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, IP],
+    "fmc_batch_get_molecules": [C.c_void_p, C.c_int, C.c_int, DP, DP, DP, C.c_int, IP],
     "fmc_batch_run": [C.c_void_p, C.c_longlong],
     "fmc_batch_launch": [C.c_void_p, C.c_longlong],
     "fmc_batch_sync": [C.c_void_p],

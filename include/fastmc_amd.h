@@ -185,6 +185,8 @@ typedef struct fmc_chain_stats {
    * sk_dev = largest |S_running(k) - S_exact(k)| the recompute replaced */
   double e_acc, e_full, drift, sk_dev;
   long long recomputes;
+  double sum_en[4], sum_e2[4]; /* production sums of E*N and E^2
+                                * (gcmc.f:1838-1848; isosteric heat) */
 } fmc_chain_stats;
 
 /* max_chains chains; precision FMC_PREC_*.  */
@@ -196,6 +198,27 @@ int fmc_batch_add_chain(fmc_batch *b, const fmc_ctx *ctx,
                         const fmc_chain_params *p, int *chain_id);
 /* remove a chain and free its slot (serving) */
 int fmc_batch_remove_chain(fmc_batch *b, int chain_id);
+/* Chain checkpoint, the batched form of the reference's restart state
+ * (REVIVE, written by revive utility_pack.f:1800-1814 every 1000 moves,
+ * gcmc.f:737-740, and read by revread utility_pack.f:1835-1862).  Beyond
+ * REVIVE's molecule counts, coordinates, statistics and energies it holds
+ * the duni table, the adaptive step sizes, the counters and the running
+ * S(k), so a restored chain continues the identical trajectory (same
+ * precision).  The bytes are opaque and valid for this library build.
+ * buf == NULL: *len gets the size.  Waits for the launch in flight. */
+int fmc_batch_checkpoint(fmc_batch *b, int chain_id, void *buf, long long cap,
+                         long long *len);
+/* add a chain from a checkpoint; ctx must describe the system the chain was
+ * admitted with (framework, cell, Ewald and force-field set-up; its guest
+ * molecules are ignored), else FMC_ERR_ARG */
+int fmc_batch_restore_chain(fmc_batch *b, const fmc_ctx *ctx, const void *buf,
+                            long long len, int *chain_id);
+/* a chain's guest molecules of type gtype: *nmol molecules, sites
+ * molecule-major as framwkxxx(mol, 1:nummols*numatoms) (utility_pack.f:
+ * 1805-1810); x = y = z = NULL: count only.  Waits for the launch in
+ * flight. */
+int fmc_batch_get_molecules(fmc_batch *b, int chain_id, int gtype, double *x,
+                            double *y, double *z, int cap_sites, int *nmol);
 /* run n_moves moves on every active chain (one persistent launch) */
 int fmc_batch_run(fmc_batch *b, long long n_moves);
 /* enqueue without waiting / wait for completion */
