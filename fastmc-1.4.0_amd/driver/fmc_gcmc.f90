@@ -220,6 +220,10 @@ program fmc_gcmc
   real(8), allocatable :: gasmol(:,:), gaseng(:,:)
   real(8) :: st_n(maxg), st_e(maxg), st_en(maxg), st_n2(maxg), st_e2(maxg)
   integer :: totalguests
+  ! restart files (gcmc.f:349-410,737-740; utility_pack.f:1772-1862)
+  logical :: lrestart, have_rev, rev_prod
+  character(len=80) :: cfgname
+  real(8) :: chainstats(1 + 14 * maxg), dele_last
   real(8) :: t0, t1
   integer(8) :: c0, c1, crate
 
@@ -228,7 +232,7 @@ program fmc_gcmc
   ! ------------------------------------------------------------------
   temp = 0.d0; rcut = 0.d0; delr_in = 1.d0; rvdw = 0.d0; have_sum = .false.
   have_prec = .false.; mcsteps = 1; eqsteps = 0; nnumg = 1000; maxguest = 3000
-  do_trace = .false.; have_seed = .false.; rota_ratio = 0.5d0
+  do_trace = .false.; have_seed = .false.; rota_ratio = 0.5d0; lrestart = .false.
   gfreq = -1.d0; gdelr = -1.d0; gdisp = 0.5d0; gtran = 0.5d0
   press = 0.d0; freq = 0.d0; gfreq_set = .false.; gdelr_set = .false.
   gdisp_set = .false.; gtran_set = .false.; ntpguest = 0
@@ -247,6 +251,8 @@ program fmc_gcmc
       end do
     else if (has(rec, 'trace')) then
       do_trace = .true.
+    else if (has(rec, 'restart')) then
+      lrestart = .true.
     else if (has(rec, 'temp')) then
       temp = dblstr(rec)
     else if (has(rec, '&guest')) then
@@ -470,6 +476,7 @@ program fmc_gcmc
   ! ------------------------------------------------------------------
   open(12, file='CONFIG', status='old', action='read')
   read(12, '(a)') rec
+  cfgname = rec(1:80)
   read(12, '(a)') rec
   levcfg = intstr(rec)
   imcon = intstr(rec)
@@ -516,6 +523,15 @@ program fmc_gcmc
     end do
   end do
   close(12)
+
+  ! restart: the REVIVE of a previous run replaces the molecule counts and
+  ! coordinates (gcmc.f:349-410, revread utility_pack.f:1835-1862)
+  chainstats = 0.d0
+  have_rev = .false.; rev_prod = .false.
+  if (lrestart) then
+    inquire(file='branch01/REVIVE', exist=have_rev)
+    if (have_rev) call read_revive()
+  end if
 
   ! guest templates centred on their COM (readinputs.f:226-253)
   do n = 1, ntpmls
@@ -638,6 +654,10 @@ program fmc_gcmc
   accept_ins = 0; accept_del = 0
   gcmccount = 0; prodcount = 0; lgchk = .true.; tick = .false.
   production = (eqsteps == 0)
+  ! a restarted production run goes on averaging unless equilibration
+  ! steps are asked for again (gcmc.f:398-410,573)
+  if (have_rev) production = (rev_prod .and. eqsteps <= 0) .or. eqsteps == 0
+  dele_last = 0.d0
   allocate(gasmol(maxsamp, ntpguest), gaseng(maxsamp, ntpguest))
   st_n = 0.d0; st_e = 0.d0; st_en = 0.d0; st_n2 = 0.d0; st_e2 = 0.d0
   call system_clock(c0, crate)
@@ -666,6 +686,9 @@ program fmc_gcmc
         end if
       end if
     end if
+
+    ! restart files every 1000 moves (gcmc.f:737-740)
+    if (mod(gcmccount, 1000_8) == 0) call write_restart()
 
     ig = 1
     if (ntpguest > 1) ig = floor(duni() * ntpguest) + 1
@@ -827,6 +850,7 @@ program fmc_gcmc
       end if
     end if
 
+    dele_last = merge(estep, 0.d0, accepted)   ! delE(iguest), gcmc.f:861,955,1124
     if (do_trace) write(402, '(i10,i3,i3,i3,i7,5es25.16e3)') gcmccount, ig, kind, &
       merge(1, 0, accepted), nmol(ig), estep, out(1), out(2), out(3), delrc
 
@@ -1052,6 +1076,124 @@ contains
     do ii = 1, gns(g)
       nx(ii) = nx(ii) + x; ny(ii) = ny(ii) + y; nz(ii) = nz(ii) + z
     end do
+  end subroutine
+
+  subroutine read_revive()
+    ! revread (utility_pack.f:1835-1862): production flag, molecule counts,
+    ! rolling statistics, atom count, per molecule type x/y/z of every
+    ! site, energies (replaced by the single_point energy, as gcmc.f:555-560
+    ! does)
+    integer :: u, nn, ng, kk, ns0, tot
+    integer, allocatable :: num(:)
+    real(8), allocatable :: bx(:), by(:), bz(:)
+    logical :: lp
+    real(8) :: erev(maxg)
+    allocate(num(ntpmls))
+    open(newunit=u, file='branch01/REVIVE', form='unformatted', status='old', action='read')
+    read(u) lp
+    read(u) num
+    read(u) chainstats(1:14 * ntpguest + 1)
+    read(u) tot
+    kk = 0
+    do nn = 1, ntpmls
+      ns0 = num(nn) * mol_nsite(nn)
+      allocate(bx(max(1, ns0)), by(max(1, ns0)), bz(max(1, ns0)))
+      read(u) bx(1:ns0)
+      read(u) by(1:ns0)
+      read(u) bz(1:ns0)
+      ng = mol_gi(nn)
+      if (ng > 0) then
+        if (ns0 > maxguest) call fail('REVIVE holds more guest atoms than max guest atoms (error 75)')
+        nmol(ng) = num(nn)
+        gx(1:ns0, ng) = bx(1:ns0)
+        gy(1:ns0, ng) = by(1:ns0)
+        gz(1:ns0, ng) = bz(1:ns0)
+      else
+        if (num(nn) /= mol_nummols(nn)) call fail('REVIVE framework differs from FIELD')
+        fx(kk + 1:kk + ns0) = bx(1:ns0)
+        fy(kk + 1:kk + ns0) = by(1:ns0)
+        fz(kk + 1:kk + ns0) = bz(1:ns0)
+        kk = kk + ns0
+      end if
+      deallocate(bx, by, bz)
+    end do
+    read(u) erev(1:ntpguest)
+    close(u)
+    rev_prod = lp
+  end subroutine
+
+  subroutine write_restart()
+    ! revive (utility_pack.f:1772-1814): REVCON with every atom, guests
+    ! first then the frameworks (the condense order, utility_pack.f:3337-
+    ! 3380), and the unformatted REVIVE
+    integer :: u, nn, ng, m2, s2, kk, a, ns0
+    integer, allocatable :: num(:)
+    open(newunit=u, file='branch01/REVCON', form='formatted', status='replace')
+    write(u, '(a80)') cfgname
+    write(u, '(2i10,e20.10)') 0, imcon, dele_last
+    if (imcon > 0) write(u, '(3f20.12)') cell
+    a = 0
+    do nn = 1, ntpmls
+      ng = mol_gi(nn)
+      if (ng == 0) cycle
+      do m2 = 1, nmol(ng)
+        do s2 = 1, gns(ng)
+          a = a + 1
+          kk = (m2 - 1) * gns(ng) + s2
+          write(u, '(a8,i10)') mol_name(s2, nn), a
+          write(u, '(3g20.10)') gx(kk, ng), gy(kk, ng), gz(kk, ng)
+        end do
+      end do
+    end do
+    kk = 0
+    do nn = 1, ntpmls
+      if (mol_gi(nn) > 0) cycle
+      do m2 = 1, mol_nummols(nn)
+        do s2 = 1, mol_nsite(nn)
+          a = a + 1
+          kk = kk + 1
+          write(u, '(a8,i10)') mol_name(s2, nn), a
+          write(u, '(3g20.10)') fx(kk), fy(kk), fz(kk)
+        end do
+      end do
+    end do
+    close(u)
+    allocate(num(ntpmls))
+    do nn = 1, ntpmls
+      num(nn) = mol_nummols(nn)
+      if (mol_gi(nn) > 0) num(nn) = nmol(mol_gi(nn))
+    end do
+    chainstats(1) = dble(prodcount)
+    do ng = 1, ntpguest
+      kk = 1 + 14 * (ng - 1)
+      chainstats(kk + 1) = st_n(ng)
+      chainstats(kk + 2) = st_e(ng)
+      chainstats(kk + 3) = st_en(ng)
+      chainstats(kk + 4) = st_n2(ng)
+      chainstats(kk + 5) = st_e2(ng)
+    end do
+    open(newunit=u, file='branch01/REVIVE', form='unformatted', status='replace')
+    write(u) production
+    write(u) num
+    write(u) chainstats(1:14 * ntpguest + 1)
+    write(u) a
+    kk = 0
+    do nn = 1, ntpmls
+      ng = mol_gi(nn)
+      ns0 = num(nn) * mol_nsite(nn)
+      if (ng > 0) then
+        write(u) gx(1:ns0, ng)
+        write(u) gy(1:ns0, ng)
+        write(u) gz(1:ns0, ng)
+      else
+        write(u) fx(kk + 1:kk + ns0)
+        write(u) fy(kk + 1:kk + ns0)
+        write(u) fz(kk + 1:kk + ns0)
+        kk = kk + ns0
+      end if
+    end do
+    write(u) energy(1:ntpguest)
+    close(u)
   end subroutine
 
   subroutine block_stats(g)

@@ -110,3 +110,115 @@ def test_gpu_driver_matches_oracle_driver(oracle_driver, tmp_path, name):
     assert np.array_equal(tg[:, :5], tc[:, :5])  # kinds, accept/reject, N
     rel = np.abs(tg[:, 5] - tc[:, 5]) / np.maximum(np.abs(tc[:, 5]), 1e-3)
     assert rel.max() < 1e-10, rel.max()
+
+
+def _restart_run_dir(d, ex_dir, steps=1000):
+    """The reference's restart example set up as a restart: CONFIG, FIELD,
+    branch01/REVIVE, and a CONTROL asking for `restart`, a fixed seed and a
+    short production run."""
+    os.makedirs(os.path.join(d, "branch01"), exist_ok=True)
+    for f in ("CONFIG", "FIELD"):
+        shutil.copy(os.path.join(ex_dir, f), os.path.join(d, f))
+    shutil.copy(os.path.join(ex_dir, "REVIVE"), os.path.join(d, "branch01", "REVIVE"))
+    for f in ("CONFIG", "FIELD", os.path.join("branch01", "REVIVE")):
+        os.chmod(os.path.join(d, f), 0o644)
+    s = open(os.path.join(ex_dir, "CONTROL")).read()
+    s = s.replace("steps    100000000", f"steps    {steps}")
+    s = s.replace("equilibration    -1500000", "equilibration    0")
+    s = s.replace("finish", "seed 12 34 56 78\ntrace\nrestart\nfinish")
+    open(os.path.join(d, "CONTROL"), "w").write(s)
+
+
+def _initial_energy(d):
+    for line in open(os.path.join(d, "OUTPUT")):
+        if "Initial guest energy" in line:
+            return float(line.split()[-1])
+    raise AssertionError("no initial energy in OUTPUT")
+
+
+def test_driver_restart_reads_and_writes_revive(oracle_driver, oracle, restart_example,
+                                                tmp_path):
+    """The driver restarted from the reference's REVIVE (gcmc.f:349-410)
+    starts from known answer 2, and the REVIVE/REVCON it writes at move
+    1000 (gcmc.f:737-740) hold the state after 999 moves:
+    * molecule count, rolling <N> and the accumulated energy follow the
+      move trace;
+    * the single_point energy of the written configuration plus the
+      starting tail equals the accumulated energy (the move loop's elrc
+      starts at zero, gcmc.f:138-139, so the first accepted insert or
+      delete adds the whole tail once);
+    * the REVCON equals, byte for byte, the one written from the REVIVE
+      coordinates by the Python writer that reproduces the reference's
+      own REVCON;
+    * a second restart from the written REVIVE starts from that state's
+      single_point energy."""
+    from fastmc_amd import inputs
+    from conftest import revive_config
+    ex = restart_example
+    d = str(tmp_path)
+    _restart_run_dir(d, ex["dir"])
+    tr = _run(oracle_driver, d)
+    kat = ex["kat"]["revive_energy_kcal"]
+    assert abs(_initial_energy(d) - kat) < 1e-6
+    field, config = ex["field"], ex["config"]
+    rev = inputs.read_revive(os.path.join(d, "branch01", "REVIVE"), field)
+    pre = tr[tr[:, 0] < 1000]
+    assert len(pre) == 999 and rev.is_production
+    assert rev.nummols == [int(pre[-1, 4]), 36]
+    assert rev.chainstats[0] == 999.0
+    assert abs(rev.chainstats[1] - pre[:, 4].mean()) < 1e-9 * pre[:, 4].mean()
+    # accumulated energy from the starting single_point
+    f0, c0, _ = revive_config(ex)
+    s0, _, _ = oracle.system_from_inputs(f0, c0, ex["control"])
+    e0 = s0.energy_full()
+    e = sum(e0[1:6]) / field.engunit
+    for row in pre:
+        if row[3]:
+            e = e + row[5]
+    assert abs(rev.energy[0] - e) < 1e-9
+    insdel = pre[(pre[:, 2] <= 2) & (pre[:, 3] == 1)]
+    assert len(insdel) > 0
+    f1, c1 = inputs.apply_revive(field, config, rev)
+    s1, _, _ = oracle.system_from_inputs(f1, c1, ex["control"])
+    e1 = s1.energy_full()
+    e_sp = sum(e1[1:6]) / field.engunit
+    tail0 = e0[5] / field.engunit
+    assert abs(e_sp + tail0 - rev.energy[0]) < 1e-7, (e_sp, tail0, rev.energy[0])
+    # REVCON (eng = the last move's accepted estep, delE at gcmc.f:737-741)
+    last = pre[-1]
+    want = tmp_path / "REVCON.want"
+    inputs.write_revcon(str(want), config.title, config.imcon, config.cell,
+                        inputs.revcon_atoms(field, rev.coords),
+                        eng=last[5] if last[3] else 0.0)
+    assert open(os.path.join(d, "branch01", "REVCON")).read() == want.read_text()
+    # restart from the driver's own REVIVE
+    _run(oracle_driver, d)
+    assert abs(_initial_energy(d) - e_sp) < 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_driver_restart_matches_oracle_driver(oracle_driver, restart_example, tmp_path):
+    """The restart run on the MI355X library: the same moves, decisions and
+    loadings as the oracle-linked driver, estep within 1e-10 relative, and
+    the same REVIVE state."""
+    from fastmc_amd import inputs
+    if not os.path.exists(DRV_GPU):
+        pytest.fail(f"{DRV_GPU} missing: build first")
+    da, db = tmp_path / "gpu", tmp_path / "cpu"
+    da.mkdir(); db.mkdir()
+    for d in (da, db):
+        _restart_run_dir(str(d), restart_example["dir"])
+    tg = _run(DRV_GPU, str(da))
+    tc = _run(oracle_driver, str(db))
+    assert tg.shape == tc.shape
+    assert np.array_equal(tg[:, :5], tc[:, :5])
+    rel = np.abs(tg[:, 5] - tc[:, 5]) / np.maximum(np.abs(tc[:, 5]), 1e-3)
+    assert rel.max() < 1e-10, rel.max()
+    field = restart_example["field"]
+    rg = inputs.read_revive(str(da / "branch01" / "REVIVE"), field)
+    rc = inputs.read_revive(str(db / "branch01" / "REVIVE"), field)
+    assert rg.nummols == rc.nummols
+    for a, b in zip(rg.coords, rc.coords):
+        for c in range(3):
+            assert np.array_equal(a[c], b[c])
+    assert abs(rg.energy[0] - rc.energy[0]) < 1e-9
