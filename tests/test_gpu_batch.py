@@ -250,6 +250,49 @@ def test_batch_fp64_triclinic_revive(fmc, oracle, restart_example):
     print(f"triclinic revive: {int(ref[:, 2].sum())} accepted, worst estep rel {worst:.2e}")
 
 
+def test_batch_binary_example(fmc, oracle, restart_example):
+    """The reference's binary example (CO2 + TIP4P-Ew with its massless
+    charge site, started from the unary REVCON): FP64 gives the oracle
+    chain's trajectory; mixed precision makes the same decisions, with
+    estep within 1e-4 kcal/mol of FP64 where |estep| < 100 kcal/mol and
+    within 1e-4 relative on overlapping trials (|estep| up to ~1e9, all
+    rejected)."""
+    from fastmc_amd.batch import Batch
+    from test_restart_files import binary_example
+    field, config, control = binary_example(restart_example)
+    gids = sorted(control.guests)
+    press = [control.guests[g].press_pa for g in gids]
+    freq = [control.guests[g].freq for g in gids]
+    n, seed = 600, (31, 41, 59, 26)
+    s, _, _ = oracle.system_from_inputs(field, config, control)
+    ch = oracle.OracleChain(s, control.temp, press, seed, freq=freq, delr=control.delr)
+    recs = ch.run(n)
+    ref = np.array([[r.kind, r.iguest, r.accepted, r.nmol_after, r.estep] for r in recs])
+    ctx = fmc.context_from_inputs(field, config, control)
+    logs = []
+    for prec in (0, 1):
+        b = Batch(1, prec)
+        b.set_tile(256)
+        cid = b.add_chain(ctx, control.temp, press, seed, freq=freq, delr=control.delr)
+        b.enable_log(cid, n)
+        b.run(n)
+        logs.append(b.get_log(cid, n))
+        b.close()
+    worst = _compare(logs[0], ref)
+    same = np.all(logs[1][:, :4] == ref[:, :4], axis=1)
+    k = int(np.argmin(same)) if not same.all() else n
+    d = np.abs(logs[1][:k, 4] - ref[:k, 4])
+    small = np.abs(ref[:k, 4]) < 100.0
+    dmix = d[small].max()
+    rbig = (d[~small] / np.abs(ref[:k, 4][~small])).max() if (~small).any() else 0.0
+    print(f"binary example: {int(ref[:, 2].sum())} accepted, water moves "
+          f"{int((ref[:, 1] == 1).sum())}, worst FP64 estep rel {worst:.2e}, "
+          f"mixed identical for {k} moves, max |dE| {dmix:.2e} kcal/mol "
+          f"(|estep| < 100), max rel {rbig:.2e} on {int((~small).sum())} overlaps")
+    assert (ref[:, 1] == 1).sum() > 100
+    assert k > 100 and dmix < 1e-4 and rbig < 1e-4
+
+
 def test_capacity_fault_reported(fmc):
     """Guest capacity exhausted (the reference's error 75, gcmc.f:870): the
     chain stops inserting, flags the fault, and poll() hands it back."""

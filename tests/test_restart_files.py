@@ -106,3 +106,27 @@ def test_fortran_editing_matches_flang(tmp_path):
     for i, v in enumerate(vals):
         assert inputs.fortran_g(v) == out[2 * i], (v, out[2 * i])
         assert inputs.fortran_e(v) == out[2 * i + 1], (v, out[2 * i + 1])
+
+
+def binary_example(restart_example):
+    """The reference's binary example as its README sets it up: the binary
+    FIELD with CO2 NUMMOLS 90 and the unary REVCON as CONFIG."""
+    from fastmc_amd import inputs
+    d = os.path.join(GOLDEN, "binary_example")
+    field = inputs.read_field(os.path.join(d, "FIELD"))
+    field.molecules[0].nummols = 90
+    config = inputs.read_config(os.path.join(restart_example["dir"], "REVCON"), field)
+    control = inputs.read_control(os.path.join(d, "CONTROL"))
+    return field, config, control
+
+
+def test_binary_example_starts_from_known_answer_2(oracle, restart_example):
+    """Adding TIP4P-Ew as a second, empty guest type leaves the energy of
+    the 90-CO2 configuration at known answer 2 (-731.5072668 kcal/mol);
+    the REVCON's 10-digit coordinates move it by less than 1e-6."""
+    field, config, control = binary_example(restart_example)
+    assert [g.press_pa for _, g in sorted(control.guests.items())] == [99036.4, 441.1]
+    s, _, _ = oracle.system_from_inputs(field, config, control)
+    e = s.energy_full()
+    spe = sum(e[1:6]) / field.engunit
+    assert abs(spe - restart_example["kat"]["revive_energy_kcal"]) < 1e-6
