@@ -851,8 +851,8 @@ program fmc_gcmc
     end if
 
     dele_last = merge(estep, 0.d0, accepted)   ! delE(iguest), gcmc.f:861,955,1124
-    if (do_trace) write(402, '(i10,i3,i3,i3,i7,5es25.16e3)') gcmccount, ig, kind, &
-      merge(1, 0, accepted), nmol(ig), estep, out(1), out(2), out(3), delrc
+    if (do_trace) write(402, '(i10,i3,i3,i3,i7,6es25.16e3)') gcmccount, ig, kind, &
+      merge(1, 0, accepted), nmol(ig), estep, out(1), out(2), out(3), delrc, energy(ig)
 
     ! production statistics (gcmc.f:1814-1870)
     if (production) then
@@ -1197,33 +1197,91 @@ contains
   end subroutine
 
   subroutine block_stats(g)
+    ! End-of-run statistics of guest g for this single branch, in the
+    ! reference's arithmetic: five windows, window x summing the samples
+    ! (x-1)*nwind ... (x-1)*nwind+nwind (nwind+1 of them, a sample past the
+    ! last one reads zero) and dividing by nwind = int(samples/5); Welford
+    ! mean and population deviation of the window <N>, <E>, <EN>, <N^2>,
+    ! <E^2>, Q_st, C_v (gcmc.f:1969-2031); Q_st and C_v of the means
+    ! (gcmc.f:2051-2100); the union over branches, here one, weighted by
+    ! the sample count (gcmc.f:2366-2432); the report of gcmc.f:2280-2321.
+    ! Beyond maxsamp samples the averages are the rolling ones and the
+    ! errors (from the reference's rolling windows) are not computed: NaN.
+    use ieee_arithmetic
     integer, intent(in) :: g
-    integer :: nb, b, ns0, ii
-    real(8) :: bm(5), mean, sd, qst
-    nb = int(min(prodcount, int(maxsamp, 8)) / 5)
-    write(6, "(/,'final stats for guest ',i2)") g
-    write(6, "(a36,f15.6)") '<N>: ', st_n(g)
-    write(6, "(a36,f15.6)") '<E>: ', st_e(g)
-    write(6, "(a36,f15.6)") '<E*N>: ', st_en(g)
-    write(6, "(a36,f15.6)") '<N*N>: ', st_n2(g)
-    write(6, "(a36,f15.6)") '<E*E>: ', st_e2(g)
-    if (st_n2(g) - st_n(g) ** 2 > 0.d0) then
-      qst = -1.d0 * (st_en(g) - st_e(g) * st_n(g)) / (st_n2(g) - st_n(g) * st_n(g)) + rgas * temp
-      write(6, "(a60,f15.6)") 'Isosteric heat of adsorption (kcal/mol):', qst
-    end if
-    if (nb > 0) then
-      do b = 1, 5
-        ns0 = (b - 1) * nb
-        bm(b) = 0.d0
-        do ii = 1, nb
-          bm(b) = bm(b) + gasmol(ns0 + ii, g)
+    integer :: nwind, x, z
+    integer(8) :: y, k
+    real(8) :: sw(5), w(7), mean(7), m2(7), delta, a, e, cs(6), sd(8), av(8)
+    real(8) :: wt, ua(8), us(8)
+    wt = dble(prodcount)
+    if (prodcount <= int(maxsamp, 8)) then
+      nwind = int(wt / 5)
+      mean = 0.d0
+      m2 = 0.d0
+      do x = 1, 5
+        y = int(x - 1, 8) * nwind
+        sw = 0.d0
+        do k = y + 1, y + nwind + 1
+          a = 0.d0
+          e = 0.d0
+          if (k <= prodcount) then
+            a = gasmol(k, g)
+            e = gaseng(k, g)
+          end if
+          sw(1) = sw(1) + a
+          sw(2) = sw(2) + e
+          sw(3) = sw(3) + a * e
+          sw(4) = sw(4) + a * a
+          sw(5) = sw(5) + e * e
         end do
-        bm(b) = bm(b) / nb
+        w(1:5) = sw(1:5) / nwind
+        w(6) = calc_qst(w(5), w(2), w(1), w(4), w(3))
+        w(7) = calc_cv(w(5), w(2), w(1), w(4), w(3))
+        do z = 1, 7
+          delta = w(z) - mean(z)
+          mean(z) = mean(z) + delta / x
+          m2(z) = m2(z) + delta * (w(z) - mean(z))
+        end do
       end do
-      mean = sum(bm) / 5.d0
-      sd = sqrt(sum((bm - mean) ** 2) / 5.d0)
-      write(6, "(/,a50,f15.6)") 'average number of guests:', st_n(g)
-      write(6, "(a50,f15.6)") 'standard error:', sd
+      cs(1:6) = mean(1:6)
+      sd(1:7) = sqrt(m2(1:7) / 5)
+      sd(8) = 0.d0
+    else
+      cs = (/ st_n(g), st_e(g), st_en(g), st_n2(g), st_e2(g), 0.d0 /)
+      sd = ieee_value(0.d0, ieee_quiet_nan)
     end if
+    av(1:5) = cs(1:5)
+    av(6) = calc_qst(cs(5), cs(2), cs(1), cs(4), cs(3))
+    av(7) = calc_cv(cs(5), cs(2), cs(1), cs(4), cs(3))
+    av(8) = cs(6)
+    ua = (wt * av) / wt
+    ua(6) = calc_qst(ua(5), ua(2), ua(1), ua(4), ua(3))
+    ua(7) = calc_cv(ua(5), ua(2), ua(1), ua(4), ua(3))
+    us = sqrt((wt * (sd ** 2 + av ** 2)) / wt - ua ** 2)
+    write(6, "(/,'final stats for guest ',i2,/)") g
+    write(6, "(/,a36,f15.6,a5,f12.3,/,a36,f15.6,a5,f12.3,/,a36,f15.6,a5,f12.3,/," // &
+             "a36,f15.6,a5,f12.3,/,a36,f15.6,a5,f12.3,/)") &
+      '<N>: ', ua(1), ' +/- ', us(1), '<E>: ', ua(2), ' +/- ', us(2), &
+      '<E*N>: ', ua(3), ' +/- ', us(3), '<N*N>: ', ua(4), ' +/- ', us(4), &
+      '<E*E>: ', ua(5), ' +/- ', us(5)
+    write(6, "(/,a60,f15.6,/,a60,f15.6)") 'average number of guests: ', ua(1), &
+      'standard error: ', us(1)
+    write(6, "(a60,f15.6,/,a60,f15.6,/,a60,f15.6,/,a60,f15.6,/,a60,i15,/)") &
+      'Isosteric heat of adsorption (kcal/mol): ', ua(6), 'Isosteric heat error: ', us(6), &
+      'Heat capacity, Cv (kcal/mol/K): ', ua(7), 'Heat capacity error: ', us(7), &
+      'Total steps counted: ', int(wt)
   end subroutine
+
+  real(8) function calc_qst(e2, e, n, n2, en)
+    ! utility_pack.f:2280-2290
+    real(8), intent(in) :: e2, e, n, n2, en
+    calc_qst = -1.d0 * (en - e * n) / (n2 - n * n) + rgas * temp
+  end function
+
+  real(8) function calc_cv(e2, e, n, n2, en)
+    ! utility_pack.f:2305-2316
+    real(8), intent(in) :: e2, e, n, n2, en
+    calc_cv = (3.d0 / 2.d0 * rgas) + (e2 - (e * e) - (en - e * n) ** 2 &
+              / (n2 - n * n)) / (n * kboltz * temp * temp)
+  end function
 end program fmc_gcmc
