@@ -43,6 +43,12 @@ namespace fmc {
 
 constexpr int kBTab = 17;   // phase-table entries per dimension (kmax <= 16)
 constexpr int kLogFields = 8;
+constexpr int kWin = 5;          // statistics windows (gcmc.f:119)
+constexpr int kWinQ = 5;         // sums per window: N, E, E*N, N^2, E^2
+constexpr int kWinSnap = 2 * kWin - 1;   // running-sum snapshots that bound them
+constexpr int kWinDoubles = kWinSnap * kWinQ * kMaxGuestTypes;
+constexpr int kWinMax = 500000;  // samples beyond which the reference keeps
+                                 // rolling averages instead (gcmc.f:1971)
 constexpr int kLdsTypes = 32;   // atom types whose pair rows sit in LDS
 
 // The small per-system constants the serial lane and the per-move set-up
@@ -94,6 +100,13 @@ struct BChain {
   int sys;
   int ntpguest;
   int cap_atoms;
+  // end-of-run windows (gcmc.f:1969-2031): window j sums the production
+  // samples j*L ... j*L + L (L = win_len), the difference of the running
+  // sums after j*L + L + 1 samples and after j*L samples.  The running sums
+  // are copied to the snapshot area that follows S(k) in the sre buffer
+  // (sre + nk: [snapshot][quantity][guest]) when the production count
+  // reaches L, L+1, 2L, 2L+1, 3L, 3L+1, 4L, 4L+1, 5L+1.
+  int win_len;    // 0: no windows (no move limit, or too many samples)
   double* sre;
   double* sim;
   double* dsre;   // dS scratch for the streaming (large-Nk) kernel
@@ -126,6 +139,7 @@ struct BChain {
   double* log;
   long long log_cap, log_n;
   int fault;   // 1: guest capacity reached (reference error 75)
+  int win_snap;   // snapshots taken
   long long stop_at;   // the chain idles once steps reaches this
   long long recomp_every;   // > 0: FP64 recompute inside the launch every this many moves
   // FP64 full-energy bookkeeping: e_acc is the running sum of accepted
@@ -535,10 +549,30 @@ __device__ void propose(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D) {
   D.nguest = na;
 }
 
+// Production count at which snapshot k of the running sums is taken.
+__host__ __device__ inline long long win_target(int k, int L) {
+  return k < 2 * kWin - 2 ? static_cast<long long>(k / 2 + 1) * L + (k & 1)
+                          : static_cast<long long>(kWin) * L + 1;
+}
+
+__device__ void window_snapshot(BChain& C, double* snaps) {
+  while (C.win_snap < kWinSnap && C.prod_steps == win_target(C.win_snap, C.win_len)) {
+    double* w = snaps + C.win_snap * kWinQ * kMaxGuestTypes;
+    for (int g = 0; g < kMaxGuestTypes; ++g) {
+      gstore(w + 0 * kMaxGuestTypes + g, C.sum_n[g]);
+      gstore(w + 1 * kMaxGuestTypes + g, C.sum_e[g]);
+      gstore(w + 2 * kMaxGuestTypes + g, C.sum_en[g]);
+      gstore(w + 3 * kMaxGuestTypes + g, C.sum_n2[g]);
+      gstore(w + 4 * kMaxGuestTypes + g, C.sum_e2[g]);
+    }
+    C.win_snap++;
+  }
+}
+
 // Acceptance and bookkeeping on the serial lane (gcmc.f:938-1002, 1106-1172,
 // 1326-1378, 1814-1870).
 __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
-                       double e1, double e2, double ev) {
+                       double e1, double e2, double ev, int nk) {
   const int ig = D.ig;
   const int nmols = C.nmol[ig];
   double estep = 0.0, delrc = 0.0;
@@ -597,6 +631,7 @@ __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
       C.sum_en[g] += e * n;
       C.sum_e2[g] += e * e;
     }
+    if (C.win_len > 0) window_snapshot(C, C.sre + nk);
   }
   if (C.log && C.log_n < C.log_cap) {
     double* r = C.log + kLogFields * C.log_n;
@@ -1025,7 +1060,7 @@ __global__ void __launch_bounds__(BT)
         c += red[2][w];
       }
       a = Z.rvolm2 * kR4pie0 * a / Z.epsq;
-      decide(Z, C, R, D, a, b, c);
+      decide(Z, C, R, D, a, b, c, nk);
       if (budget_ticks > 0 &&
           static_cast<long long>(__builtin_amdgcn_s_memrealtime()) - t_start >= budget_ticks)
         stop_flag = 1;
@@ -1914,7 +1949,8 @@ int snapshot(fmc_batch* b, Snapshot* s, bool restore) {
   for (int id : b->active_ids) {
     const BChain& C = s->chains[id];
     const BSys& Y = b->systems[C.sys]->dev;
-    std::vector<std::pair<double*, size_t>> bufs = {{C.sre, (size_t)Y.nk},
+    // S(k) and the window snapshots that follow it, the guest coordinates
+    std::vector<std::pair<double*, size_t>> bufs = {{C.sre, (size_t)Y.nk + kWinDoubles},
                                                     {C.sim, (size_t)Y.nk}};
     for (int g = 0; g < Y.q.ngt; ++g) {
       bufs.push_back({C.gx[g], (size_t)C.cap_atoms});
@@ -2147,8 +2183,9 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
     C.cap_atoms = ctx->cap_atoms;
     const int nk = ctx->nk;
     auto& K = ch->keep;
-    // S(k) is set by the admission recompute below
-    if ((r = dput(K, &C.sre, static_cast<double*>(nullptr), nk)) ||
+    // S(k) is set by the admission recompute below; the window snapshots
+    // follow it in the same buffer
+    if ((r = dput(K, &C.sre, static_cast<double*>(nullptr), nk + kWinDoubles)) ||
         (r = dput(K, &C.sim, static_cast<double*>(nullptr), nk)) ||
         (r = dput(K, &C.dsre, static_cast<double*>(nullptr), nk)) ||
         (r = dput(K, &C.dsim, static_cast<double*>(nullptr), nk)))
@@ -2195,6 +2232,17 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
     C.rota_rotangle = C.rotangle;
     C.elrc = 0.0;
     C.stop_at = p->max_moves > 0 ? p->max_moves : std::numeric_limits<long long>::max();
+    {
+      const long long prod = p->max_moves - std::max(0LL, p->eq_steps);
+      C.win_len = (p->max_moves > 0 && prod >= kWin && prod <= kWinMax)
+                      ? static_cast<int>(prod / kWin) : 0;
+      C.win_snap = 0;
+      if (hipMemsetAsync(C.sre + nk, 0, kWinDoubles * sizeof(double), b->setup) !=
+          hipSuccess) {
+        r = FMC_ERR_HIP;
+        break;
+      }
+    }
     C.recomp_every = p->recompute_every > 0 ? p->recompute_every : 0;
     duni_setup(p->seed[0], p->seed[1], p->seed[2], p->seed[3], &C);
     for (auto& kb : K) ch->bytes += kb->n;
@@ -2254,7 +2302,8 @@ constexpr char kCkMagic[8] = {'F', 'M', 'C', 'C', 'H', 'K', 0, 0};
 constexpr int kCkVersion = 1;
 
 long long ck_size(const BChain& C, int nk, const int* ns) {
-  long long n = sizeof(CkHeader) + sizeof(BChain) + 2LL * nk * sizeof(double);
+  long long n = sizeof(CkHeader) + sizeof(BChain) + 2LL * nk * sizeof(double) +
+                kWinDoubles * sizeof(double);
   for (int g = 0; g < C.ntpguest; ++g)
     n += 3LL * C.nmol[g] * ns[g] * sizeof(double);
   return n;
@@ -2306,6 +2355,8 @@ extern "C" int fmc_batch_checkpoint(fmc_batch* b, int id, void* buf, long long c
   out += nk * sizeof(double);
   BCHECK(hipMemcpy(out, C.sim, nk * sizeof(double), hipMemcpyDeviceToHost));
   out += nk * sizeof(double);
+  BCHECK(hipMemcpy(out, C.sre + nk, kWinDoubles * sizeof(double), hipMemcpyDeviceToHost));
+  out += kWinDoubles * sizeof(double);
   for (int t = 0; t < C.ntpguest; ++t) {
     const size_t n = static_cast<size_t>(C.nmol[t]) * ns[t];
     for (double* src : {C.gx[t], C.gy[t], C.gz[t]}) {
@@ -2353,11 +2404,15 @@ extern "C" int fmc_batch_restore_chain(fmc_batch* b, const fmc_ctx* ctx, const v
     auto& K = ch->keep;
     const double* sre = reinterpret_cast<const double*>(in);
     const double* sim = sre + h.nk;
-    if ((r = dput(K, &C.sre, sre, h.nk)) || (r = dput(K, &C.sim, sim, h.nk)) ||
+    const double* win = sim + h.nk;
+    std::vector<double> sre_win(sre, sre + h.nk);
+    sre_win.insert(sre_win.end(), win, win + kWinDoubles);
+    if ((r = dput(K, &C.sre, sre_win.data(), sre_win.size())) ||
+        (r = dput(K, &C.sim, sim, h.nk)) ||
         (r = dput(K, &C.dsre, static_cast<double*>(nullptr), h.nk)) ||
         (r = dput(K, &C.dsim, static_cast<double*>(nullptr), h.nk)))
       break;
-    const double* site = sim + h.nk;
+    const double* site = win + kWinDoubles;
     std::vector<double> stage(C.cap_atoms);
     for (int t = 0; t < h.ntpguest && !r; ++t) {
       const size_t n = static_cast<size_t>(C.nmol[t]) * h.ns[t];
@@ -2555,6 +2610,27 @@ extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
   st->drift = C.drift;
   st->sk_dev = C.sk_dev;
   st->recomputes = C.n_recompute;
+  st->win_len = C.win_len;
+  st->win_closed = 0;
+  if (C.win_len > 0) {
+    // running sums at each window bound: a snapshot, or the current sums
+    // for a bound not reached (samples past the end read zero)
+    double w[kWinDoubles];
+    BCHECK(hipMemcpy(w, C.sre + b->systems[C.sys]->dev.nk, sizeof(w),
+                     hipMemcpyDeviceToHost));
+    const double* now[kWinQ] = {C.sum_n, C.sum_e, C.sum_en, C.sum_n2, C.sum_e2};
+    auto at = [&](int k, int q, int g) {
+      return k < C.win_snap ? w[(k * kWinQ + q) * kMaxGuestTypes + g] : now[q][g];
+    };
+    for (int j = 0; j < kWin; ++j) {
+      const int k_end = j < kWin - 1 ? 2 * j + 1 : kWinSnap - 1;   // count jL+L+1
+      const int k_beg = 2 * j - 2;                                  // count jL
+      if (k_end < C.win_snap) st->win_closed = j + 1;
+      for (int q = 0; q < kWinQ; ++q)
+        for (int g = 0; g < kMaxGuestTypes; ++g)
+          st->win[j][q][g] = at(k_end, q, g) - (j == 0 ? 0.0 : at(k_beg, q, g));
+    }
+  }
   return C.fault ? FMC_ERR_CAPACITY : FMC_OK;
 }
 

@@ -168,7 +168,29 @@ class Batch:
                     sum_e2=list(st.sum_e2)[:ng], delrdisp=list(st.delrdisp)[:ng],
                     rotangle=st.rotangle, e_acc=st.e_acc, e_full=st.e_full,
                     drift=st.drift, sk_dev=st.sk_dev, recomputes=st.recomputes,
+                    win_len=st.win_len, win_closed=st.win_closed,
+                    win=[[[st.win[j][q][g] for g in range(ng)] for q in range(5)]
+                         for j in range(5)],
                     capacity_fault=(r == -4))
+
+    def report(self, cid: int, temp: float):
+        """The reference's end-of-run statistics of a chain that has made
+        its max_moves (gcmc.f:1969-2100), one stats.NodeStats per guest
+        type: <N>, <E>, <EN>, <N^2>, <E^2>, Q_st, C_v and their errors from
+        the five windows the kernel accumulated.  Combine replicas of one
+        state point with stats.union."""
+        from . import stats as S
+        st = self.stats(cid)
+        if st["win_len"] <= 0:
+            raise ValueError("chain has no statistics windows (needs max_moves with "
+                             "5 to 500000 production moves)")
+        if st["win_closed"] < 4 or st["prod_steps"] < 5 * st["win_len"]:
+            raise ValueError("chain has not finished its production moves")
+        out = []
+        for g in range(len(st["nmol"])):
+            sums = [[st["win"][j][q][g] for q in range(5)] for j in range(5)]
+            out.append(S.node_stats(sums, st["win_len"], float(st["prod_steps"]), temp))
+        return out
 
     def enable_log(self, cid: int, cap: int):
         check(self.L.fmc_batch_enable_log(self.h, cid, cap), "fmc_batch_enable_log")

@@ -49,7 +49,9 @@ class ChainStats(C.Structure):
                 ("rota_rotangle", C.c_double), ("e_acc", C.c_double),
                 ("e_full", C.c_double), ("drift", C.c_double),
                 ("sk_dev", C.c_double), ("recomputes", C.c_longlong),
-                ("sum_en", C.c_double * 4), ("sum_e2", C.c_double * 4)]
+                ("sum_en", C.c_double * 4), ("sum_e2", C.c_double * 4),
+                ("win_len", C.c_longlong), ("win_closed", C.c_int),
+                ("win", ((C.c_double * 4) * 5) * 5)]
 
 
 _lib = None

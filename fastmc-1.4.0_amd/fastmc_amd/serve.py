@@ -11,6 +11,10 @@ GPU time, every chain moving at its own speed); requests that arrive
 while a slice runs are built and admitted on the host meanwhile and join
 the next slice, and finished chains leave at slice boundaries.
 
+Each result carries the raw chain statistics and, per guest type, the
+reference's end-of-run report (<N>, <E>, Q_st, C_v and their errors from
+five windows of the production samples, stats.NodeStats).
+
 A chain's arithmetic does not depend on which other chains share its
 launch (one work-group per chain, a fixed tile), so a request's result is
 the same alone and in the mix.  Admission runs on its own thread (the C
@@ -49,6 +53,9 @@ class Result:
     done: float
     stats: dict
     footprint: int
+    report: Optional[list] = None   # per guest type: stats.NodeStats, the
+                                    # reference's end-of-run statistics
+
 
     @property
     def latency(self) -> float:
@@ -83,11 +90,15 @@ class Server:
             with cv:
                 req, t_adm, need = self._live.pop(cid)
             st = self.batch.stats(cid)
+            rep = None
+            if st["win_len"] > 0 and not st["capacity_fault"]:
+                rep = self.batch.report(cid, req.temp)
             self.batch.remove_chain(cid)
             with cv:
                 self._used -= need
                 self.moves_done += st["steps"]
-                self.results[req.rid] = Result(req.rid, req.arrival, t_adm, now, st, need)
+                self.results[req.rid] = Result(req.rid, req.arrival, t_adm, now, st, need,
+                                               rep)
                 cv.notify_all()
 
     def run(self, requests: Sequence[Request], realtime: bool = True) -> List[Result]:

@@ -187,6 +187,15 @@ typedef struct fmc_chain_stats {
   long long recomputes;
   double sum_en[4], sum_e2[4]; /* production sums of E*N and E^2
                                 * (gcmc.f:1838-1848; isosteric heat) */
+  /* end-of-run windows (gcmc.f:1969-2031) for chains with a move limit of
+   * at most 500000 production moves: win_len = int(production moves / 5);
+   * win[j][q][g] sums quantity q (N, E, E*N, N^2, E^2) of guest g over the
+   * production samples j*win_len ... j*win_len + win_len (the reference's
+   * windows of win_len + 1 samples, to be divided by win_len); win_closed
+   * windows are complete, the next one holds what has been sampled */
+  long long win_len;
+  int win_closed;
+  double win[5][5][4];
 } fmc_chain_stats;
 
 /* max_chains chains; precision FMC_PREC_*.  */

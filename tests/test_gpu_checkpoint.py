@@ -8,7 +8,7 @@ pytestmark = pytest.mark.gpu
 
 KEYS = ("steps", "prod_steps", "nmol", "energy", "accepted", "attempted", "sum_n",
         "sum_n2", "sum_e", "sum_en", "sum_e2", "delrdisp", "rotangle", "e_acc",
-        "e_full", "drift", "sk_dev", "recomputes")
+        "e_full", "drift", "sk_dev", "recomputes", "win_len", "win_closed", "win")
 
 
 @pytest.fixture(scope="module")
@@ -208,3 +208,56 @@ def test_batch_revive_records_the_chain_state(fmc, oracle, restart_example, tmp_
                         atoms)
     c3 = inputs.read_config(str(tmp_path / "REVCON"), f2)
     assert np.allclose(np.array(c3.coords[0])[:, 0], back.coords[0][0], rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_window_statistics_follow_the_move_log(fmc, case):
+    """The kernel's end-of-run windows (production samples j*L ... j*L + L,
+    L = int(samples / 5)) equal the windows of the samples rebuilt from the
+    FP64 move log, across launch boundaries; the last window misses the
+    sample past the end, as the reference's does; the report equals the
+    restatement applied to the rebuilt samples."""
+    from fastmc_amd import stats
+    A, P, seed, freq = _cases()[case]
+    ctx = fmc.Context.from_arrays(A)
+    M, Q = 1000, 150
+    b = _batch(1)
+    c = b.add_chain(ctx, 298.0, P, seed, freq=freq, delr=A["delr"], eq_steps=Q,
+                    max_moves=M)
+    st0 = b.stats(c)
+    assert st0["win_len"] == (M - Q) // 5 == 170
+    b.enable_log(c, M)
+    for n in (300, 300, 400):
+        b.run(n)
+    st = b.stats(c)
+    log = b.get_log(c, M)
+    rep = b.report(c, 298.0)
+    b.close()
+    assert st["steps"] == M and st["prod_steps"] == M - Q and st["win_closed"] == 4
+    ng = len(P)
+    nm = list(st0["nmol"])
+    en = list(st0["energy"])
+    Ns = [[] for _ in range(ng)]
+    Es = [[] for _ in range(ng)]
+    for i in range(M):
+        g = int(log[i, 1])
+        nm[g] = int(log[i, 3])
+        if log[i, 2]:
+            en[g] = en[g] + log[i, 4]
+        if i >= Q:
+            for h in range(ng):
+                Ns[h].append(float(nm[h]))
+                Es[h].append(en[h])
+    for g in range(ng):
+        sums, nwind = stats.window_sums(Ns[g], Es[g])
+        assert nwind == st["win_len"]
+        got = np.array([[st["win"][j][q][g] for q in range(5)] for j in range(5)])
+        assert np.allclose(got, np.array(sums), rtol=1e-12, atol=1e-9), g
+        want = stats.from_samples(Ns[g], Es[g], 298.0)
+        for z in range(7):
+            a, w = rep[g].avg[z], want.avg[z]
+            assert (np.isnan(a) and np.isnan(w)) or abs(a - w) <= 1e-9 * max(1.0, abs(w))
+            a, w = rep[g].std[z], want.std[z]
+            assert (np.isnan(a) and np.isnan(w)) or abs(a - w) <= 1e-7 * max(1.0, abs(w))
+        print(f"guest {g}: <N> {rep[g].avg[0]:.4f} +/- {rep[g].std[0]:.4f}, "
+              f"Q_st {rep[g].avg[5]:.3f} +/- {rep[g].std[5]:.3f}")
