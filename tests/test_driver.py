@@ -110,6 +110,13 @@ def test_gpu_driver_matches_oracle_driver(oracle_driver, tmp_path, name):
     assert np.array_equal(tg[:, :5], tc[:, :5])  # kinds, accept/reject, N
     rel = np.abs(tg[:, 5] - tc[:, 5]) / np.maximum(np.abs(tc[:, 5]), 1e-3)
     assert rel.max() < 1e-10, rel.max()
+    # the end-of-run reports agree to the printed digits
+    from test_stats import _report
+    rg, rc = _report(da / "OUTPUT"), _report(db / "OUTPUT")
+    assert rg.keys() == rc.keys() and "average number of guests:" in rg
+    for k in rg:
+        a, b = np.atleast_1d(rg[k]), np.atleast_1d(rc[k])
+        assert np.allclose(a, b, rtol=0, atol=2e-6, equal_nan=True), (k, a, b)
 
 
 def _restart_run_dir(d, ex_dir, steps=1000):
