@@ -214,7 +214,9 @@ int fmc_batch_remove_chain(fmc_batch *b, int chain_id);
  * the duni table, the adaptive step sizes, the counters and the running
  * S(k), so a restored chain continues the identical trajectory (same
  * precision).  The bytes are opaque and valid for this library build.
- * buf == NULL: *len gets the size.  Waits for the launch in flight. */
+ * buf == NULL: *len gets the size.  Waits for the launch in flight; call it
+ * from the thread that enqueues launches (a launch enqueued meanwhile by
+ * another thread races with the copy). */
 int fmc_batch_checkpoint(fmc_batch *b, int chain_id, void *buf, long long cap,
                          long long *len);
 /* add a chain from a checkpoint; ctx must describe the system the chain was
