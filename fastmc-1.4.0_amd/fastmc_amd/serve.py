@@ -63,6 +63,15 @@ class Result:
 
 
 @dataclass
+class Suspended:
+    """A request whose chain was checkpointed by a stopped run
+    (fmc_batch_checkpoint); a later run resumes it where it stopped."""
+    request: Request
+    admitted: float
+    blob: bytes
+
+
+@dataclass
 class Server:
     slots: int = 256
     precision: int = PREC_MIXED
@@ -81,6 +90,8 @@ class Server:
         self._inflight = 0                  # admissions under way
         self.slices = 0
         self.moves_done = 0
+        self.suspended: List[Suspended] = []   # live chains of a stopped run
+        self.unstarted: List[Request] = []     # requests a stopped run left
 
     def close(self):
         self.batch.close()
@@ -101,30 +112,58 @@ class Server:
                                                rep)
                 cv.notify_all()
 
-    def run(self, requests: Sequence[Request], realtime: bool = True) -> List[Result]:
+    def run(self, requests: Sequence[Request], realtime: bool = True,
+            resume: Sequence[Suspended] = (), stop_after_slices: Optional[int] = None
+            ) -> List[Result]:
         """Serve the requests; with realtime, each arrives at its arrival
         time (seconds after the start), else all at once.  An admission
         thread builds each request's system and adds its chain while the
         GPU runs slices; this thread launches slices back to back and
-        retires finished chains."""
+        retires finished chains.
+
+        resume: chains of an earlier stopped run, restored from their
+        checkpoints before the first slice; each continues its trajectory
+        exactly (its admission time is the earlier run's).
+        stop_after_slices: after that many slices, stop admitting,
+        checkpoint every live chain into self.suspended (their requests'
+        results are not produced) and leave the requests not yet admitted in
+        self.unstarted; the results returned are those of the requests
+        (resumed ones first) that finished."""
         pending = sorted(requests, key=lambda r: r.arrival)
         cv = threading.Condition()
         t0 = self.clock()
-        state = {"done": False, "error": None}
+        state = {"done": False, "error": None, "stop": False}
+        self.suspended, self.unstarted = [], []
+        for sus in resume:
+            ctx = Context.from_arrays(sus.request.arrays)
+            need = footprint(ctx, self.precision)
+            cid = self.batch.restore_chain(ctx, sus.blob)
+            ctx.close()
+            self._live[cid] = (sus.request, sus.admitted, need)
+            self._used += need
 
         def admitter():
             try:
-                for req in pending:
+                for i, req in enumerate(pending):
                     if realtime:
                         delay = req.arrival - (self.clock() - t0)
                         if delay > 0:
-                            time.sleep(delay)
+                            with cv:
+                                cv.wait_for(lambda: state["stop"], timeout=delay)
+                    if state["stop"]:
+                        self.unstarted = list(pending[i:])
+                        return
                     ctx = Context.from_arrays(req.arrays)
                     need = footprint(ctx, self.precision)
                     with cv:
-                        while (len(self._live) + self._inflight >= self.slots or
-                               self._used + need > self.hbm_budget):
+                        while (not state["stop"] and
+                               (len(self._live) + self._inflight >= self.slots or
+                                self._used + need > self.hbm_budget)):
                             cv.wait()
+                        if state["stop"]:
+                            self.unstarted = list(pending[i:])
+                            ctx.close()
+                            return
                         self._inflight += 1
                         self._used += need
                     cid = self.batch.add_chain(ctx, req.temp, req.press_pa, req.seed,
@@ -144,6 +183,7 @@ class Server:
 
         th = threading.Thread(target=admitter, daemon=True)
         th.start()
+        n_slices = 0
         try:
             while True:
                 with cv:
@@ -159,13 +199,28 @@ class Server:
                 else:
                     self.batch.launch(self.slice_moves)
                 self.slices += 1
+                n_slices += 1
                 self.batch.sync()
                 self._retire(self.clock() - t0, cv)
+                if stop_after_slices is not None and n_slices >= stop_after_slices:
+                    with cv:
+                        state["stop"] = True
+                        cv.notify_all()
+                    break
         finally:
             th.join()
         if state["error"] is not None:
             raise state["error"]
-        return [self.results[r.rid] for r in requests]
+        if state["stop"]:
+            # admissions have ended: checkpoint and release every live chain
+            for cid in sorted(self._live):
+                req, t_adm, need = self._live[cid]
+                self.suspended.append(Suspended(req, t_adm, self.batch.checkpoint(cid)))
+                self.batch.remove_chain(cid)
+                self._used -= need
+            self._live.clear()
+        asked = [sus.request for sus in resume] + list(requests)
+        return [self.results[r.rid] for r in asked if r.rid in self.results]
 
 
 def percentile(v: Sequence[float], q: float) -> float:

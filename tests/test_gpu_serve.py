@@ -51,3 +51,31 @@ def test_serve_budget_slices_same_results():
     for x, y in zip(ra, rb):
         for key in KEYS:
             assert x.stats[key] == y.stats[key], (x.rid, key)
+
+
+def test_serve_suspend_and_resume():
+    """A run stopped after a few slices checkpoints its live chains; a new
+    server resumes them together with the requests never admitted, and
+    every request ends exactly as in one uninterrupted run."""
+    import fastmc_amd
+    from fastmc_amd import serve
+    fastmc_amd.lib()
+    reqs = serve.poisson_requests(7, rate=2000.0, seed=13, moves=600)
+    whole = serve.Server(slots=3, slice_moves=150)
+    want = {r.rid: r for r in whole.run(reqs, realtime=False)}
+    whole.close()
+    first = serve.Server(slots=3, slice_moves=150)
+    done1 = first.run(reqs, realtime=False, stop_after_slices=3)
+    sus, rest = first.suspended, first.unstarted
+    first.close()
+    assert sus and rest and not first._live
+    assert len(done1) + len(sus) + len(rest) == len(reqs)
+    second = serve.Server(slots=3, slice_moves=150)
+    done2 = second.run(rest, realtime=False, resume=sus)
+    second.close()
+    got = {r.rid: r for r in done1 + done2}
+    assert sorted(got) == sorted(want)
+    for rid in want:
+        for key in KEYS:
+            assert got[rid].stats[key] == want[rid].stats[key], (rid, key)
+    print(f"suspended {len(sus)} live, {len(rest)} unstarted; all {len(got)} equal")
