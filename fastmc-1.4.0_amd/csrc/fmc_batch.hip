@@ -755,8 +755,16 @@ __global__ void __launch_bounds__(BT)
   using T = typename std::conditional<MIXED, float, double>::type;
   __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
   __shared__ Cplx<T> zt[2 * kMaxSites * kZRow];           // Z_p[n], signed n
-  __shared__ MoveDesc D;
-  __shared__ float rng_u[97];
+  // Move descriptors and duni tables, two of each: while the waves work on
+  // one move, the last wave proposes the next one on copies of the chain
+  // state as if the current move were rejected (most are); the serial lane
+  // adopts that proposal when the move is indeed rejected.
+  __shared__ MoveDesc Dbuf[2];
+  __shared__ float rng_u[2][97];
+  __shared__ BChain Cs;                 // chain state for the next proposal
+  __shared__ int dsel, rsel, spec_hit;  // current descriptor / table, adopt?
+  __shared__ float cur_c, spec_c;       // duni state after this / the next proposal
+  __shared__ int cur_ir, cur_jr, spec_ir, spec_jr;
   __shared__ double red[4][BT / kWave];
   __shared__ int stop_flag;
   __shared__ int shift_lo, shift_hi;
@@ -823,8 +831,11 @@ __global__ void __launch_bounds__(BT)
   if (tid == 0) stop_flag = 0;
   __syncthreads();
   if (tid == 0) {
-    for (int i = 0; i < 97; ++i) rng_u[i] = C.u[i];
-    R.u = rng_u;
+    for (int i = 0; i < 97; ++i) rng_u[0][i] = C.u[i];
+    R.u = rng_u[0];
+    dsel = 0;
+    rsel = 0;
+    spec_hit = 0;
     R.c = C.c; R.cd = C.cd; R.cm = C.cm; R.ir = C.ir; R.jr = C.jr;
   }
   __syncthreads();
@@ -833,8 +844,14 @@ __global__ void __launch_bounds__(BT)
     // C.steps and stop_flag were last written before the barrier ending the
     // previous move: every thread takes the same branch
     if (C.steps >= C.stop_at || stop_flag) break;
-    if (tid == 0) propose(Z, C, R, D);
+    if (tid == 0) {
+      if (!spec_hit) propose(Z, C, R, Dbuf[dsel]);
+      cur_c = R.c;
+      cur_ir = R.ir;
+      cur_jr = R.jr;
+    }
     __syncthreads();
+    MoveDesc& D = Dbuf[dsel];
     const int kind = D.kind;
     const int ns = D.ns;
     const int ig = D.ig;
@@ -947,12 +964,39 @@ __global__ void __launch_bounds__(BT)
           }
         }
       }
+      // ---- the next move's proposal, by the last wave, on copies of the
+      //      chain state and duni table as this move leaves them if it is
+      //      rejected (its acceptance draw taken, if it makes one) ----
+      constexpr int RT = BT - kWave;   // threads on real space
+      if (tid >= RT) {
+        const int lane = tid - RT;
+        const int* src = reinterpret_cast<const int*>(&C);
+        int* dst = reinterpret_cast<int*>(&Cs);
+        for (int i = lane; i < static_cast<int>(sizeof(BChain) / 4); i += kWave) dst[i] = src[i];
+        for (int i = lane; i < 97; i += kWave) rng_u[rsel ^ 1][i] = rng_u[rsel][i];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) {
+          Rng Rs;
+          Rs.u = rng_u[rsel ^ 1];
+          Rs.c = cur_c;
+          Rs.cd = C.cd;
+          Rs.cm = C.cm;
+          Rs.ir = cur_ir;
+          Rs.jr = cur_jr;
+          if (kind != 0) (void)Rs.next();
+          propose(Z, Cs, Rs, Dbuf[dsel ^ 1]);
+          spec_c = Rs.c;
+          spec_ir = Rs.ir;
+          spec_jr = Rs.jr;
+        }
+      }
       // ---- real space: framework candidates of every position's cell and
-      //      every guest atom, one flattened loop (ewald2 + srfrce).  It
-      //      reads only what the set-up phases wrote, so no barrier
-      //      separates it from the reciprocal part: a wave that finishes
-      //      its k-vectors starts its pairs at once ----
-      {
+      //      every guest atom, one flattened loop (ewald2 + srfrce), on all
+      //      but the last wave.  It reads only what the set-up phases
+      //      wrote, so no barrier separates it from the reciprocal part: a
+      //      wave that finishes its k-vectors starts its pairs at once ----
+      if (tid < RT) {
         const int gcap = Y.gcap;
         const int nfwp = np * gcap;
         const int ng = D.nguest;
@@ -963,11 +1007,11 @@ __global__ void __launch_bounds__(BT)
         // kU items per thread per pass: their index loads, then their
         // coordinate loads, issue together (one memory latency per pass)
         constexpr int kU = 4;
-        for (int base = tid; base < total; base += kU * BT) {
+        for (int base = tid; base < total; base += kU * RT) {
           int pp[kU], aa[kU];
 #pragma unroll
           for (int u = 0; u < kU; ++u) {
-            const int idx = base + u * BT;
+            const int idx = base + u * RT;
             pp[u] = -1;
             aa[u] = 0;
             if (idx < nfwp) {
@@ -1064,6 +1108,30 @@ __global__ void __launch_bounds__(BT)
       if (budget_ticks > 0 &&
           static_cast<long long>(__builtin_amdgcn_s_memrealtime()) - t_start >= budget_ticks)
         stop_flag = 1;
+      // adopt the next proposal when this move was rejected and another
+      // move follows in this launch (a launch never ends holding draws of a
+      // move it has not made)
+      // (a no-op move has no parallel phase, hence no proposal made)
+      spec_hit = (kind != 0 && !D.accepted && !stop_flag && C.steps < C.stop_at &&
+                  mv + 1 < nmoves) ? 1 : 0;
+      if (spec_hit) {
+        for (int g = 0; g < kMaxGuestTypes; ++g) {
+          C.disp_count[g] = Cs.disp_count[g];
+          C.tran_count[g] = Cs.tran_count[g];
+          C.delrdisp[g] = Cs.delrdisp[g];
+          C.tran_delr[g] = Cs.tran_delr[g];
+        }
+        C.rota_count = Cs.rota_count;
+        C.rotangle = Cs.rotangle;
+        C.rota_rotangle = Cs.rota_rotangle;
+        C.fault = Cs.fault;
+        rsel ^= 1;
+        R.u = rng_u[rsel];
+        R.c = spec_c;
+        R.ir = spec_ir;
+        R.jr = spec_jr;
+        dsel ^= 1;
+      }
       shift_lo = 0;
       shift_hi = 0;
       if (D.accepted && kind == FMC_DELETE) {
@@ -1193,7 +1261,7 @@ __global__ void __launch_bounds__(BT)
     }
   }
   if (tid == 0) {
-    for (int i = 0; i < 97; ++i) C.u[i] = rng_u[i];
+    for (int i = 0; i < 97; ++i) C.u[i] = rng_u[rsel][i];
     C.c = R.c; C.ir = R.ir; C.jr = R.jr;
   }
   __syncthreads();
