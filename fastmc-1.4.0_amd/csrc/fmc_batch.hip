@@ -569,6 +569,80 @@ __device__ void window_snapshot(BChain& C, double* snaps) {
   }
 }
 
+// Production statistics of one move (gcmc.f:1814-1848), after C.steps has
+// counted it; true when it is a production sample.
+__device__ __forceinline__ bool production_sample(BChain& C) {
+  if (C.steps <= C.eq_steps) return false;
+  C.prod_steps++;
+  for (int g = 0; g < C.ntpguest; ++g) {
+    const double n = C.nmol[g];
+    const double e = C.energy[g];
+    C.sum_n[g] += n;
+    C.sum_n2[g] += n * n;
+    C.sum_e[g] += e;
+    C.sum_en[g] += e * n;
+    C.sum_e2[g] += e * e;
+  }
+  return true;
+}
+
+__device__ __forceinline__ void log_move(BChain& C, int slot, int ig, int acc, double estep,
+                                         double e1, double e2, double ev) {
+  if (C.log && C.log_n < C.log_cap) {
+    double* r = C.log + kLogFields * C.log_n;
+    r[0] = slot;
+    r[1] = ig;
+    r[2] = acc;
+    r[3] = C.nmol[ig];
+    r[4] = estep;
+    r[5] = e1;
+    r[6] = e2;
+    r[7] = ev;
+    C.log_n++;
+  }
+}
+
+__device__ __forceinline__ int move_slot(const MoveDesc& D) {
+  return D.kind == 0 ? 0 : (D.sub ? D.sub : D.kind);
+}
+
+// The decision alone (decide's arithmetic, gcmc.f:938-949, 1106-1116,
+// 1326-1338), with the acceptance draw given: estep, the logged e1 and
+// whether the move is accepted.  delrc is the tail change of an insert or
+// delete.
+__device__ __forceinline__ bool decision(const SysQ& Z, const BChain& C, const MoveDesc& D,
+                                         double e1, double e2, double ev, double delrc,
+                                         double rande, double& estep, double& e1out) {
+  const int ig = D.ig;
+  const int nmols = C.nmol[ig];
+  bool acc = false;
+  estep = 0.0;
+  if (D.kind == FMC_INSERT || D.kind == FMC_DELETE) {
+    if (D.kind == FMC_INSERT) {
+      e1 = e1 + Z.engsic[ig];
+      estep = (e1 + e2 + Z.e3[ig] + ev + delrc) / C.engunit;
+    } else {
+      e1 = e1 - Z.engsic[ig];
+      estep = (e1 - e2 - Z.e3[ig] - ev + delrc) / C.engunit;
+    }
+    acc = accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp, C.beta, D.kind);
+  } else if (D.kind == FMC_DISPLACE) {
+    estep = (e1 + e2 + ev) / C.engunit;
+    acc = estep < 0.0 ||
+          accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp, C.beta,
+                      FMC_DISPLACE);
+  }
+  e1out = e1;
+  return acc;
+}
+
+__device__ __forceinline__ double tail_change(const SysQ& Z, const BChain& C, const MoveDesc& D) {
+  int n2[kMaxGuestTypes];
+  for (int g = 0; g < Z.ngt; ++g) n2[g] = C.nmol[g];
+  n2[D.ig] += (D.kind == FMC_INSERT) ? 1 : -1;
+  return tail_of(Z, n2) - C.elrc;
+}
+
 // Acceptance and bookkeeping on the serial lane (gcmc.f:938-1002, 1106-1172,
 // 1326-1378, 1814-1870).
 __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
@@ -577,12 +651,9 @@ __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
   const int nmols = C.nmol[ig];
   double estep = 0.0, delrc = 0.0;
   bool acc = false;
-  const int slot = D.kind == 0 ? 0 : (D.sub ? D.sub : D.kind);
+  const int slot = move_slot(D);
   if (D.kind == FMC_INSERT || D.kind == FMC_DELETE) {
-    int n2[kMaxGuestTypes];
-    for (int g = 0; g < Z.ngt; ++g) n2[g] = C.nmol[g];
-    n2[ig] += (D.kind == FMC_INSERT) ? 1 : -1;
-    delrc = tail_of(Z, n2) - C.elrc;
+    delrc = tail_change(Z, C, D);
     if (D.kind == FMC_INSERT) {
       e1 = e1 + Z.engsic[ig];
       estep = (e1 + e2 + Z.e3[ig] + ev + delrc) / C.engunit;
@@ -620,31 +691,8 @@ __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
   if (acc) C.accepted[slot]++;
   D.accepted = acc ? 1 : 0;
   C.steps++;
-  if (C.steps > C.eq_steps) {
-    C.prod_steps++;
-    for (int g = 0; g < C.ntpguest; ++g) {
-      const double n = C.nmol[g];
-      const double e = C.energy[g];
-      C.sum_n[g] += n;
-      C.sum_n2[g] += n * n;
-      C.sum_e[g] += e;
-      C.sum_en[g] += e * n;
-      C.sum_e2[g] += e * e;
-    }
-    if (C.win_len > 0) window_snapshot(C, C.sre + nk);
-  }
-  if (C.log && C.log_n < C.log_cap) {
-    double* r = C.log + kLogFields * C.log_n;
-    r[0] = slot;
-    r[1] = ig;
-    r[2] = D.accepted;
-    r[3] = C.nmol[ig];
-    r[4] = estep;
-    r[5] = e1;
-    r[6] = e2;
-    r[7] = ev;
-    C.log_n++;
-  }
+  if (production_sample(C) && C.win_len > 0) window_snapshot(C, C.sre + nk);
+  log_move(C, slot, ig, D.accepted, estep, e1, e2, ev);
 }
 
 // Exact S(k) of a chain: framework part plus every guest atom, FP64 with
@@ -759,17 +807,21 @@ __global__ void __launch_bounds__(BT)
   // one move, the last wave proposes the next one on copies of the chain
   // state as if the current move were rejected (most are); the serial lane
   // adopts that proposal when the move is indeed rejected.
+  // The chain state is double-buffered as well: the last wave also books
+  // the current move as rejected in the other buffer, so that a rejection
+  // costs the serial lane its decision and a buffer flip.
   __shared__ MoveDesc Dbuf[2];
   __shared__ float rng_u[2][97];
-  __shared__ BChain Cs;                 // chain state for the next proposal
-  __shared__ int dsel, rsel, spec_hit;  // current descriptor / table, adopt?
+  __shared__ BChain Cbuf[2];
+  __shared__ int csel, dsel, rsel, spec_hit;   // current buffers, adopted?
   __shared__ float cur_c, spec_c;       // duni state after this / the next proposal
   __shared__ int cur_ir, cur_jr, spec_ir, spec_jr;
+  __shared__ double pre_rande, pre_delrc;   // this move's draw and tail change
   __shared__ double red[4][BT / kWave];
   __shared__ int stop_flag;
   __shared__ int shift_lo, shift_hi;
-  __shared__ BChain C;
   __shared__ SysQ Z;   // the system's small constants, for the whole launch
+  BChain& C = Cbuf[0];   // the launch's first state (the loop rebinds it)
 
   const int tid = threadIdx.x;
   // The chain's mutable state and the system's small constants are copied
@@ -833,6 +885,7 @@ __global__ void __launch_bounds__(BT)
   if (tid == 0) {
     for (int i = 0; i < 97; ++i) rng_u[0][i] = C.u[i];
     R.u = rng_u[0];
+    csel = 0;
     dsel = 0;
     rsel = 0;
     spec_hit = 0;
@@ -841,6 +894,7 @@ __global__ void __launch_bounds__(BT)
   __syncthreads();
 
   for (long long mv = 0; mv < nmoves; ++mv) {
+    BChain& C = Cbuf[csel];
     // C.steps and stop_flag were last written before the barrier ending the
     // previous move: every thread takes the same branch
     if (C.steps >= C.stop_at || stop_flag) break;
@@ -970,8 +1024,9 @@ __global__ void __launch_bounds__(BT)
       constexpr int RT = BT - kWave;   // threads on real space
       if (tid >= RT) {
         const int lane = tid - RT;
+        BChain& Cn = Cbuf[csel ^ 1];
         const int* src = reinterpret_cast<const int*>(&C);
-        int* dst = reinterpret_cast<int*>(&Cs);
+        int* dst = reinterpret_cast<int*>(&Cn);
         for (int i = lane; i < static_cast<int>(sizeof(BChain) / 4); i += kWave) dst[i] = src[i];
         for (int i = lane; i < 97; i += kWave) rng_u[rsel ^ 1][i] = rng_u[rsel][i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -984,8 +1039,15 @@ __global__ void __launch_bounds__(BT)
           Rs.cm = C.cm;
           Rs.ir = cur_ir;
           Rs.jr = cur_jr;
-          if (kind != 0) (void)Rs.next();
-          propose(Z, Cs, Rs, Dbuf[dsel ^ 1]);
+          // this move booked as rejected: its acceptance draw (every
+          // rejected move with a parallel phase takes one), the attempt,
+          // the step and the production sample
+          pre_rande = Rs.next();
+          pre_delrc = (kind == FMC_INSERT || kind == FMC_DELETE) ? tail_change(Z, C, D) : 0.0;
+          Cn.attempted[move_slot(D)]++;
+          Cn.steps++;
+          (void)production_sample(Cn);
+          propose(Z, Cn, Rs, Dbuf[dsel ^ 1]);
           spec_c = Rs.c;
           spec_ir = Rs.ir;
           spec_jr = Rs.jr;
@@ -1104,34 +1166,34 @@ __global__ void __launch_bounds__(BT)
         c += red[2][w];
       }
       a = Z.rvolm2 * kR4pie0 * a / Z.epsq;
-      decide(Z, C, R, D, a, b, c, nk);
       if (budget_ticks > 0 &&
           static_cast<long long>(__builtin_amdgcn_s_memrealtime()) - t_start >= budget_ticks)
         stop_flag = 1;
-      // adopt the next proposal when this move was rejected and another
-      // move follows in this launch (a launch never ends holding draws of a
-      // move it has not made)
-      // (a no-op move has no parallel phase, hence no proposal made)
-      spec_hit = (kind != 0 && !D.accepted && !stop_flag && C.steps < C.stop_at &&
-                  mv + 1 < nmoves) ? 1 : 0;
-      if (spec_hit) {
-        for (int g = 0; g < kMaxGuestTypes; ++g) {
-          C.disp_count[g] = Cs.disp_count[g];
-          C.tran_count[g] = Cs.tran_count[g];
-          C.delrdisp[g] = Cs.delrdisp[g];
-          C.tran_delr[g] = Cs.tran_delr[g];
+      // A rejected move, when another move follows in this launch (a
+      // launch never ends holding draws of a move it has not made), takes
+      // the other buffers: the move booked as rejected and the next move
+      // proposed.  Otherwise the serial lane books the move itself.  (A
+      // no-op move has no parallel phase, hence nothing prepared.)
+      spec_hit = 0;
+      if (kind != 0 && !stop_flag && C.steps + 1 < C.stop_at && mv + 1 < nmoves) {
+        double estep, e1x;
+        if (!decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x) &&
+            !(kind == FMC_DISPLACE && estep < 0.0)) {
+          spec_hit = 1;
+          D.accepted = 0;
+          csel ^= 1;
+          rsel ^= 1;
+          dsel ^= 1;
+          R.u = rng_u[rsel];
+          R.c = spec_c;
+          R.ir = spec_ir;
+          R.jr = spec_jr;
+          BChain& Cn = Cbuf[csel];
+          if (Cn.win_len > 0 && Cn.steps > Cn.eq_steps) window_snapshot(Cn, Cn.sre + nk);
+          log_move(Cn, move_slot(D), ig, 0, estep, e1x, b, c);
         }
-        C.rota_count = Cs.rota_count;
-        C.rotangle = Cs.rotangle;
-        C.rota_rotangle = Cs.rota_rotangle;
-        C.fault = Cs.fault;
-        rsel ^= 1;
-        R.u = rng_u[rsel];
-        R.c = spec_c;
-        R.ir = spec_ir;
-        R.jr = spec_jr;
-        dsel ^= 1;
       }
+      if (!spec_hit) decide(Z, C, R, D, a, b, c, nk);
       shift_lo = 0;
       shift_hi = 0;
       if (D.accepted && kind == FMC_DELETE) {
@@ -1140,6 +1202,9 @@ __global__ void __launch_bounds__(BT)
       }
     }
     __syncthreads();
+    {
+    // the chain state after this move (the other buffer if it was rejected)
+    BChain& C = Cbuf[csel];
     if (D.accepted) {
       // S(k) <- S_new (ewald.f:260-293, gcmc.f:971-974)
       if (STREAM) {
@@ -1248,25 +1313,27 @@ __global__ void __launch_bounds__(BT)
       }
       __syncthreads();
     }
+    }
   }
   // write the state back
+  BChain& Cf = Cbuf[csel];
   if (!STREAM) {
 #pragma unroll
     for (int j = 0; j < KPT; ++j) {
       const int k = tid + j * BT;
       if (j < kpt && k < nk) {
-        C.sre[k] = sre[j];
-        C.sim[k] = sim[j];
+        Cf.sre[k] = sre[j];
+        Cf.sim[k] = sim[j];
       }
     }
   }
   if (tid == 0) {
-    for (int i = 0; i < 97; ++i) C.u[i] = rng_u[rsel][i];
-    C.c = R.c; C.ir = R.ir; C.jr = R.jr;
+    for (int i = 0; i < 97; ++i) Cf.u[i] = rng_u[rsel][i];
+    Cf.c = R.c; Cf.ir = R.ir; Cf.jr = R.jr;
   }
   __syncthreads();
   {
-    const int* src = reinterpret_cast<const int*>(&C);
+    const int* src = reinterpret_cast<const int*>(&Cf);
     int* dst = reinterpret_cast<int*>(&Cg);
     for (int i = tid; i < static_cast<int>(sizeof(BChain) / 4); i += BT) dst[i] = src[i];
   }
