@@ -636,6 +636,33 @@ __device__ __forceinline__ bool decision(const SysQ& Z, const BChain& C, const M
   return acc;
 }
 
+// A bound on estep above which the move is certainly rejected for the
+// acceptance draw rande: energy_eval accepts when rande < pref exp(-beta
+// estep) (pref 1 for a displacement, the volume-pressure factor for an
+// insert or delete), so estep > E* = -ln(rande / pref) / beta rejects.  The
+// bound adds a margin of 1e-6 (1 + |E*|), far beyond the rounding of exp and
+// log; a move at or below it is decided exactly.  Returns false when no
+// bound applies (rande 0, a tiny or zero prefactor, the zero-pressure
+// delete, a no-op).
+__device__ bool reject_bound(const BChain& C, const MoveDesc& D, double rande, double* thr) {
+  const int ig = D.ig;
+  double pref;
+  if (D.kind == FMC_DISPLACE) {
+    pref = 1.0;
+  } else if (D.kind == FMC_INSERT) {
+    pref = C.statvolm * C.press[ig] / (static_cast<double>(C.nmol[ig] + 1) * kBoltz * C.temp);
+  } else if (D.kind == FMC_DELETE) {
+    if (fabs(C.press[ig]) < 1e-7) return false;
+    pref = static_cast<double>(C.nmol[ig]) * kBoltz * C.temp / (C.statvolm * C.press[ig]);
+  } else {
+    return false;
+  }
+  if (!(rande > 0.0) || !(pref > 1e-300) || !(pref < 1e300)) return false;
+  const double e = -log(rande / pref) / C.beta;
+  *thr = e + 1e-6 * (1.0 + fabs(e));
+  return true;
+}
+
 __device__ __forceinline__ double tail_change(const SysQ& Z, const BChain& C, const MoveDesc& D) {
   int n2[kMaxGuestTypes];
   for (int g = 0; g < Z.ngt; ++g) n2[g] = C.nmol[g];
@@ -817,6 +844,8 @@ __global__ void __launch_bounds__(BT)
   __shared__ float cur_c, spec_c;       // duni state after this / the next proposal
   __shared__ int cur_ir, cur_jr, spec_ir, spec_jr;
   __shared__ double pre_rande, pre_delrc;   // this move's draw and tail change
+  __shared__ double pre_thr;                // estep above which it is rejected
+  __shared__ int pre_fast;                  // pre_thr applies
   __shared__ double red[4][BT / kWave];
   __shared__ int stop_flag;
   __shared__ int shift_lo, shift_hi;
@@ -1025,9 +1054,10 @@ __global__ void __launch_bounds__(BT)
       if (tid >= RT) {
         const int lane = tid - RT;
         BChain& Cn = Cbuf[csel ^ 1];
-        const int* src = reinterpret_cast<const int*>(&C);
-        int* dst = reinterpret_cast<int*>(&Cn);
-        for (int i = lane; i < static_cast<int>(sizeof(BChain) / 4); i += kWave) dst[i] = src[i];
+        const long long* src = reinterpret_cast<const long long*>(&C);
+        long long* dst = reinterpret_cast<long long*>(&Cn);
+        static_assert(sizeof(BChain) % 8 == 0, "BChain copies in 8-byte words");
+        for (int i = lane; i < static_cast<int>(sizeof(BChain) / 8); i += kWave) dst[i] = src[i];
         for (int i = lane; i < 97; i += kWave) rng_u[rsel ^ 1][i] = rng_u[rsel][i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
@@ -1042,8 +1072,7 @@ __global__ void __launch_bounds__(BT)
           // this move booked as rejected: its acceptance draw (every
           // rejected move with a parallel phase takes one), the attempt,
           // the step and the production sample
-          pre_rande = Rs.next();
-          pre_delrc = (kind == FMC_INSERT || kind == FMC_DELETE) ? tail_change(Z, C, D) : 0.0;
+          (void)Rs.next();
           Cn.attempted[move_slot(D)]++;
           Cn.steps++;
           (void)production_sample(Cn);
@@ -1058,6 +1087,25 @@ __global__ void __launch_bounds__(BT)
       //      but the last wave.  It reads only what the set-up phases
       //      wrote, so no barrier separates it from the reciprocal part: a
       //      wave that finishes its k-vectors starts its pairs at once ----
+      // the decision's inputs, by the first lane of the wave before it (it
+      // finishes its real-space share ahead of the preparing wave): the
+      // acceptance draw (read without taking it), the tail change, the
+      // rejection bound
+      if (tid == RT - kWave) {
+        const float* u = rng_u[rsel];
+        float uni = u[cur_ir - 1] - u[cur_jr - 1];
+        if (uni < 0.0f) uni = uni + 1.0f;
+        float c = cur_c - C.cd;
+        if (c < 0.0f) c = c + C.cm;
+        uni = uni - c;
+        if (uni < 0.0f) uni = uni + 1.0f;
+        const double rande = uni;
+        pre_rande = rande;
+        pre_delrc = (kind == FMC_INSERT || kind == FMC_DELETE) ? tail_change(Z, C, D) : 0.0;
+        double thr = 0.0;
+        pre_fast = reject_bound(C, D, rande, &thr) ? 1 : 0;
+        pre_thr = thr;
+      }
       if (tid < RT) {
         const int gcap = Y.gcap;
         const int nfwp = np * gcap;
@@ -1177,8 +1225,25 @@ __global__ void __launch_bounds__(BT)
       spec_hit = 0;
       if (kind != 0 && !stop_flag && C.steps + 1 < C.stop_at && mv + 1 < nmoves) {
         double estep, e1x;
-        if (!decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x) &&
-            !(kind == FMC_DISPLACE && estep < 0.0)) {
+        bool rej;
+        if (pre_fast) {
+          // estep and the logged e1 as decide forms them; the bound first
+          e1x = a;
+          if (kind == FMC_INSERT) {
+            e1x = e1x + Z.engsic[ig];
+            estep = (e1x + b + Z.e3[ig] + c + pre_delrc) / C.engunit;
+          } else if (kind == FMC_DELETE) {
+            e1x = e1x - Z.engsic[ig];
+            estep = (e1x - b - Z.e3[ig] - c + pre_delrc) / C.engunit;
+          } else {
+            estep = (e1x + b + c) / C.engunit;
+          }
+          rej = estep > pre_thr ||
+                !decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x);
+        } else {
+          rej = !decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x);
+        }
+        if (rej && !(kind == FMC_DISPLACE && estep < 0.0)) {
           spec_hit = 1;
           D.accepted = 0;
           csel ^= 1;
