@@ -922,18 +922,23 @@ __global__ void __launch_bounds__(BT)
   }
   __syncthreads();
 
+  // moves left until the next in-launch recompute (the chain's cadence)
+
   for (long long mv = 0; mv < nmoves; ++mv) {
     BChain& C = Cbuf[csel];
     // C.steps and stop_flag were last written before the barrier ending the
     // previous move: every thread takes the same branch
     if (C.steps >= C.stop_at || stop_flag) break;
+    // a prepared move is in place already (written before the last
+    // barrier); only a fresh proposal needs one
+    const bool fresh = !spec_hit;
     if (tid == 0) {
-      if (!spec_hit) propose(Z, C, R, Dbuf[dsel]);
+      if (fresh) propose(Z, C, R, Dbuf[dsel]);
       cur_c = R.c;
       cur_ir = R.ir;
       cur_jr = R.jr;
     }
-    __syncthreads();
+    if (fresh) __syncthreads();
     MoveDesc& D = Dbuf[dsel];
     const int kind = D.kind;
     const int ns = D.ns;
@@ -1318,12 +1323,12 @@ __global__ void __launch_bounds__(BT)
           __syncthreads();
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
     // FP64 recompute every recomp_every moves of this chain, inside the
     // launch: S(k) <- exact, full energy, drift (as recompute_kernel; the
     // same exact_sk, so the same S bits).  Uniform branch (LDS values
-    // written before the barrier above).
+    // written before the last barrier).
     if (C.recomp_every > 0 && C.steps % C.recomp_every == 0) {
       double erec = 0.0, dev = 0.0;
       if (STREAM) {
