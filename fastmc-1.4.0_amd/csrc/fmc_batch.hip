@@ -170,16 +170,79 @@ struct Rng {
   }
 };
 
-// quaternion rotation of rotation (utility_pack.f:2870-2906) with the
+// Lane j's value of a wave-uniform computation (j uniform).
+__device__ __forceinline__ float lane_f(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_d(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), j));
+  const unsigned hi =
+      static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), j));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+__device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Draw i of the duni stream (gcmc.f:2786-2799) ahead of state R, on lane i
+// of a wave.  Draw i reads u(ir_i), u(jr_i) and overwrites u(ir_i); both
+// indices step down together from 97 and 33, so jr = ir + 33 (mod 97)
+// throughout and draw i reads an entry that another draw overwrites only 33
+// draws later: within kLook draws every read sees the table as the
+// sequential draws would.  c, cd, cm and the table entries are multiples of
+// 2^-24 below 1, so every float step is exact and c after i+1 steps is
+// (c - (i+1) cd) mod cm in integers of 2^-24.
+constexpr int kLook = 10;   // most draws one proposal takes
+struct Ahead {
+  int ir;    // u index of this lane's draw
+  float w;   // table entry the draw leaves
+  float c;   // c after the draw
+  float d;   // the draw
+};
+
+__device__ __forceinline__ Ahead rng_ahead(const Rng& R, int lane) {
+  Ahead A;
+  const int i = lane < kLook ? lane : kLook - 1;
+  int ir = R.ir - i;
+  if (ir <= 0) ir += 97;
+  int jr = R.jr - i;
+  if (jr <= 0) jr += 97;
+  A.ir = ir;
+  float w = R.u[ir - 1] - R.u[jr - 1];
+  if (w < 0.0f) w = w + 1.0f;
+  A.w = w;
+  const int cd = static_cast<int>(R.cd * 16777216.0f);
+  const int cm = static_cast<int>(R.cm * 16777216.0f);
+  int ci = (static_cast<int>(R.c * 16777216.0f) - (i + 1) * cd) % cm;
+  if (ci < 0) ci += cm;
+  A.c = static_cast<float>(ci) * (1.0f / 16777216.0f);
+  float d = w - A.c;
+  if (d < 0.0f) d = d + 1.0f;
+  A.d = d;
+  return A;
+}
+
+// the first n draws taken: their table entries written (one per lane) and
+// the state after them, on every lane
+__device__ __forceinline__ void rng_commit(Rng& R, const Ahead& A, int lane, int n) {
+  if (lane < n) R.u[A.ir - 1] = A.w;
+  if (n > 0) {
+    R.c = lane_f(A.c, n - 1);
+    R.ir = R.ir - n;
+    if (R.ir <= 0) R.ir += 97;
+    R.jr = R.jr - n;
+    if (R.jr <= 0) R.jr += 97;
+  }
+}
+
+// rotation matrix of rotation (utility_pack.f:2870-2906) from the
 // quaternion built as random_ins / random_rot do
-__device__ void rotate_sites(int n, double theta, double r2, double r3,
-                             double r4, double* x, double* y, double* z) {
+__device__ __forceinline__ void rot_matrix(double theta, double r2, double r3, double r4,
+                                           double (&Q)[9]) {
   const double b1 = 2.0 * r2 - 1.0, b2 = 2.0 * r3 - 1.0, b3 = 2.0 * r4 - 1.0;
   double st, q1;
   sincos(theta / 2.0, &st, &q1);
   const double nrm = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
   const double q2 = st * (b1 / nrm), q3 = st * (b2 / nrm), q4 = st * (b3 / nrm);
-  double Q[9];
   Q[0] = 2.0 * q2 * q2 + 2.0 * q1 * q1 - 1.0;
   Q[1] = 2.0 * q2 * q3 - 2.0 * q1 * q4;
   Q[2] = 2.0 * q2 * q4 + 2.0 * q1 * q3;
@@ -189,12 +252,16 @@ __device__ void rotate_sites(int n, double theta, double r2, double r3,
   Q[6] = 2.0 * q2 * q4 - 2.0 * q1 * q3;
   Q[7] = 2.0 * q3 * q4 + 2.0 * q1 * q2;
   Q[8] = 2.0 * q4 * q4 + 2.0 * q1 * q1 - 1.0;
-  for (int i = 0; i < n; ++i) {
-    const double tx = Q[0] * x[i] + Q[1] * y[i] + Q[2] * z[i];
-    const double ty = Q[3] * x[i] + Q[4] * y[i] + Q[5] * z[i];
-    const double tz = Q[6] * x[i] + Q[7] * y[i] + Q[8] * z[i];
-    x[i] = tx; y[i] = ty; z[i] = tz;
-  }
+}
+
+__device__ __forceinline__ void rot_apply(const double (&Q)[9], double& x, double& y,
+                                          double& z) {
+  const double tx = Q[0] * x + Q[1] * y + Q[2] * z;
+  const double ty = Q[3] * x + Q[4] * y + Q[5] * z;
+  const double tz = Q[6] * x + Q[7] * y + Q[8] * z;
+  x = tx;
+  y = ty;
+  z = tz;
 }
 
 // energy_eval (utility_pack.f:3049-3095)
@@ -380,11 +447,21 @@ __device__ __forceinline__ int un(int v) { return static_cast<signed char>((v >>
 // The serial lane: move selection, adaptive steps, draws and proposal
 // (gcmc.f:745-847, 855-877, 1010-1060, 1176-1248; utility_pack.f:2545-3047)
 // ---------------------------------------------------------------------------
-__device__ void propose(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D) {
+// The proposal as one wave computes it: every lane runs the same scalar
+// steps on the same values (wave-uniform, so no lane waits on another), lane
+// i holds draw i of the lookahead and lane s works on guest site s.  Lane 0
+// writes the chain's counters and the descriptor's scalars.  The
+// arithmetic and the draw order are those of the reference's sequential
+// form; the centre of mass is summed over the sites in order.
+__device__ void propose(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D, int lane) {
   const double delrmin = 0.1, minangle = kPi / 18.0;
+  const Ahead A = rng_ahead(R, lane);
+  // draw 0 picks the guest type when there is more than one; draw g0 + j is
+  // the j-th after it
+  const int g0 = uni_i(C.ntpguest > 1 ? 1 : 0);
   int ig = 0;
-  if (C.ntpguest > 1) ig = static_cast<int>(floor(R.next() * C.ntpguest));
-  const double randmov = R.next();
+  if (g0) ig = uni_i(static_cast<int>(floor(static_cast<double>(lane_f(A.d, 0)) * C.ntpguest)));
+  const double randmov = lane_f(A.d, g0);
   const double* cum = C.cum[ig];
   int kind = 0, sub = 0;
   if (randmov < cum[0]) kind = FMC_INSERT;
@@ -395,115 +472,145 @@ __device__ void propose(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D) {
   else if (randmov < cum[7]) { kind = FMC_DISPLACE; sub = 5; }
   else if (randmov < cum[8]) kind = 0;
   else kind = FMC_DISPLACE;
-  const int ns = Z.ns[ig];
-  const int nmols = C.nmol[ig];
-  if (nmols >= 1 && C.disp_count[ig] >= 1) {
-    if (C.disp_count[ig] % 100 == 0) {
-      if ((C.accept_disp[ig] / static_cast<double>(C.disp_count[ig])) > C.disp_ratio[ig])
-        C.delrdisp[ig] = C.delrdisp[ig] * 1.05;
-      else if (C.delrdisp[ig] > delrmin)
-        C.delrdisp[ig] = C.delrdisp[ig] * 0.95;
-    } else if (C.disp_count[ig] % 50 == 0) {
-      if ((C.accept_disp[ig] / static_cast<double>(C.disp_count[ig])) > C.disp_ratio[ig])
-        C.rotangle = C.rotangle * 1.05;
-      else if (C.rotangle > minangle)
-        C.rotangle = C.rotangle * 0.95;
+  const int ns = uni_i(Z.ns[ig]);
+  const int nmols = uni_i(C.nmol[ig]);
+  if (lane == 0) {
+    if (nmols >= 1 && C.disp_count[ig] >= 1) {
+      if (C.disp_count[ig] % 100 == 0) {
+        if ((C.accept_disp[ig] / static_cast<double>(C.disp_count[ig])) > C.disp_ratio[ig])
+          C.delrdisp[ig] = C.delrdisp[ig] * 1.05;
+        else if (C.delrdisp[ig] > delrmin)
+          C.delrdisp[ig] = C.delrdisp[ig] * 0.95;
+      } else if (C.disp_count[ig] % 50 == 0) {
+        if ((C.accept_disp[ig] / static_cast<double>(C.disp_count[ig])) > C.disp_ratio[ig])
+          C.rotangle = C.rotangle * 1.05;
+        else if (C.rotangle > minangle)
+          C.rotangle = C.rotangle * 0.95;
+      }
     }
-  }
-  if (nmols >= 1 && C.tran_count[ig] >= 1 && C.tran_count[ig] % 100 == 0) {
-    if ((C.accept_tran[ig] / static_cast<double>(C.tran_count[ig])) > C.tran_ratio[ig])
-      C.tran_delr[ig] = C.tran_delr[ig] * 1.05;
-    else if (C.tran_delr[ig] > delrmin)
-      C.tran_delr[ig] = C.tran_delr[ig] * 0.95;
-  }
-  if (nmols >= 1 && C.rota_count >= 1 && C.rota_count % 100 == 0) {
-    if ((C.accept_rota / static_cast<double>(C.rota_count)) > C.rota_ratio)
-      C.rota_rotangle = C.rota_rotangle * 1.05;
-    else if (C.rota_rotangle > minangle)
-      C.rota_rotangle = C.rota_rotangle * 0.95;
+    if (nmols >= 1 && C.tran_count[ig] >= 1 && C.tran_count[ig] % 100 == 0) {
+      if ((C.accept_tran[ig] / static_cast<double>(C.tran_count[ig])) > C.tran_ratio[ig])
+        C.tran_delr[ig] = C.tran_delr[ig] * 1.05;
+      else if (C.tran_delr[ig] > delrmin)
+        C.tran_delr[ig] = C.tran_delr[ig] * 0.95;
+    }
+    if (nmols >= 1 && C.rota_count >= 1 && C.rota_count % 100 == 0) {
+      if ((C.accept_rota / static_cast<double>(C.rota_count)) > C.rota_ratio)
+        C.rota_rotangle = C.rota_rotangle * 1.05;
+      else if (C.rota_rotangle > minangle)
+        C.rota_rotangle = C.rota_rotangle * 0.95;
+    }
   }
   if (nmols == 0) { kind = FMC_INSERT; sub = 0; }
-  D.kind = kind;
-  D.sub = sub;
-  D.ig = ig;
-  D.ns = ns;
-  D.imol = -1;
-  D.accepted = 0;
+  kind = uni_i(kind);
+  sub = uni_i(sub);
+  // guest atoms of every type (the capacity check and the descriptor)
+  int na = 0;
+  for (int g = 0; g < Z.ngt; ++g) {
+    if (lane == 0) D.gpre[g] = na;
+    na += C.nmol[g] * Z.ns[g];
+  }
+  na = uni_i(na);
+  if (lane == 0) {
+    D.gpre[Z.ngt] = na;
+    D.nguest = na;
+    D.kind = kind;
+    D.sub = sub;
+    D.ig = ig;
+    D.ns = ns;
+    D.imol = -1;
+    D.accepted = 0;
+  }
+  int taken = g0 + 1;   // draws taken (type, move)
+  const bool site = lane < ns;
   if (kind == FMC_INSERT) {
-    int used = 0;
-    for (int g = 0; g < Z.ngt; ++g) used += C.nmol[g] * Z.ns[g];
-    if (used + ns >= C.cap_atoms) {
-      C.fault = 1;
-      D.kind = 0;
+    if (na + ns >= C.cap_atoms) {
+      if (lane == 0) {
+        C.fault = 1;
+        D.kind = 0;
+        D.np = 0;
+      }
+      rng_commit(R, A, lane, taken);
       return;
     }
-    const double ra = R.next(), rb = R.next(), rc = R.next();
-    const double r1 = R.next(), r2 = R.next(), r3 = R.next(), r4 = R.next();
+    const double ra = lane_f(A.d, g0 + 1), rb = lane_f(A.d, g0 + 2), rc = lane_f(A.d, g0 + 3);
+    const double r1 = lane_f(A.d, g0 + 4), r2 = lane_f(A.d, g0 + 5),
+                 r3 = lane_f(A.d, g0 + 6), r4 = lane_f(A.d, g0 + 7);
+    taken += 7;
     const double* cell = Z.cell;
     const double xc = ra * cell[0] + rb * cell[3] + rc * cell[6];
     const double yc = ra * cell[1] + rb * cell[4] + rc * cell[7];
     const double zc = ra * cell[2] + rb * cell[5] + rc * cell[8];
-    for (int s = 0; s < ns; ++s) {
-      D.nx[s] = Z.tx[ig][s];
-      D.ny[s] = Z.ty[ig][s];
-      D.nz[s] = Z.tz[ig][s];
+    double Q[9];
+    rot_matrix(r1 * 2.0 * kPi, r2, r3, r4, Q);
+    if (site) {
+      double x = Z.tx[ig][lane], y = Z.ty[ig][lane], z = Z.tz[ig][lane];
+      rot_apply(Q, x, y, z);
+      D.nx[lane] = x + xc;
+      D.ny[lane] = y + yc;
+      D.nz[lane] = z + zc;
     }
-    rotate_sites(ns, r1 * 2.0 * kPi, r2, r3, r4, D.nx, D.ny, D.nz);
-    for (int s = 0; s < ns; ++s) {
-      D.nx[s] = D.nx[s] + xc;
-      D.ny[s] = D.ny[s] + yc;
-      D.nz[s] = D.nz[s] + zc;
-    }
-    D.slot = nmols * ns;
+    if (lane == 0) D.slot = nmols * ns;
   } else if (kind == FMC_DELETE) {
-    D.imol = static_cast<int>(floor(R.next() * nmols));
-    for (int s = 0; s < ns; ++s) {
-      D.ox[s] = C.gx[ig][D.imol * ns + s];
-      D.oy[s] = C.gy[ig][D.imol * ns + s];
-      D.oz[s] = C.gz[ig][D.imol * ns + s];
+    const int imol = uni_i(static_cast<int>(floor(static_cast<double>(lane_f(A.d, g0 + 1)) * nmols)));
+    taken += 1;
+    if (lane == 0) D.imol = imol;
+    if (site) {
+      D.ox[lane] = gload(C.gx[ig] + imol * ns + lane);
+      D.oy[lane] = gload(C.gy[ig] + imol * ns + lane);
+      D.oz[lane] = gload(C.gz[ig] + imol * ns + lane);
     }
   } else if (kind == FMC_DISPLACE) {
-    double ra, rb, rc, r1, r2, r3, r4, dd, da;
+    double ra, rb, rc, r1, r2, r3, r4, dd, da, rm;
     if (sub == 4) {
-      C.tran_count[ig]++;
-      ra = R.next(); rb = R.next(); rc = R.next();
+      if (lane == 0) C.tran_count[ig]++;
+      ra = lane_f(A.d, g0 + 1); rb = lane_f(A.d, g0 + 2); rc = lane_f(A.d, g0 + 3);
       r1 = r2 = r3 = r4 = 0.0;
       dd = C.tran_delr[ig]; da = 0.0;
+      rm = lane_f(A.d, g0 + 4);
+      taken += 4;
     } else if (sub == 5) {
-      C.rota_count++;
+      if (lane == 0) C.rota_count++;
       ra = rb = rc = 0.5;
-      r1 = R.next(); r2 = R.next(); r3 = R.next(); r4 = R.next();
+      r1 = lane_f(A.d, g0 + 1); r2 = lane_f(A.d, g0 + 2);
+      r3 = lane_f(A.d, g0 + 3); r4 = lane_f(A.d, g0 + 4);
       dd = 0.0; da = C.rota_rotangle;
+      rm = lane_f(A.d, g0 + 5);
+      taken += 5;
     } else {
-      C.disp_count[ig]++;
-      ra = R.next(); rb = R.next(); rc = R.next();
-      r1 = R.next(); r2 = R.next(); r3 = R.next(); r4 = R.next();
+      if (lane == 0) C.disp_count[ig]++;
+      ra = lane_f(A.d, g0 + 1); rb = lane_f(A.d, g0 + 2); rc = lane_f(A.d, g0 + 3);
+      r1 = lane_f(A.d, g0 + 4); r2 = lane_f(A.d, g0 + 5);
+      r3 = lane_f(A.d, g0 + 6); r4 = lane_f(A.d, g0 + 7);
       dd = C.delrdisp[ig]; da = C.rotangle;
+      rm = lane_f(A.d, g0 + 8);
+      taken += 8;
     }
-    D.imol = static_cast<int>(floor(R.next() * nmols));
-    double* x = D.nx;
-    double* y = D.ny;
-    double* z = D.nz;
-    for (int s = 0; s < ns; ++s) {
-      x[s] = D.ox[s] = C.gx[ig][D.imol * ns + s];
-      y[s] = D.oy[s] = C.gy[ig][D.imol * ns + s];
-      z[s] = D.oz[s] = C.gz[ig][D.imol * ns + s];
+    const int imol = uni_i(static_cast<int>(floor(rm * nmols)));
+    if (lane == 0) D.imol = imol;
+    double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
+    if (site) {
+      x = gload(C.gx[ig] + imol * ns + lane);
+      y = gload(C.gy[ig] + imol * ns + lane);
+      z = gload(C.gz[ig] + imol * ns + lane);
+      m = Z.gmass[ig][lane];
+      D.ox[lane] = x;
+      D.oy[lane] = y;
+      D.oz[lane] = z;
     }
     const double h = dd * 0.5;
     const double ta = (2.0 * ra - 1.0) * h, tb = (2.0 * rb - 1.0) * h,
                  tc = (2.0 * rc - 1.0) * h;
+    x = ta + x;
+    y = tb + y;
+    z = tc + z;
     double w = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     for (int s = 0; s < ns; ++s) {
-      x[s] = ta + x[s];
-      y[s] = tb + y[s];
-      z[s] = tc + z[s];
-    }
-    for (int s = 0; s < ns; ++s) {
-      const double m = Z.gmass[ig][s];
-      w = w + m;
-      cx = cx + x[s] * m;
-      cy = cy + y[s] * m;
-      cz = cz + z[s] * m;
+      const double ms = lane_d(m, s);
+      w = w + ms;
+      cx = cx + lane_d(x, s) * ms;
+      cy = cy + lane_d(y, s) * ms;
+      cz = cz + lane_d(z, s) * ms;
     }
     cx = cx / w; cy = cy / w; cz = cz / w;
     const double* rcl = Z.rcell;
@@ -514,39 +621,34 @@ __device__ void propose(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D) {
     const double xs = sx - floor(sx), ys = sy - floor(sy), zs = sz - floor(sz);
     const double X = (cell[0] * xs + cell[3] * ys + cell[6] * zs);
     const double Yv = (cell[1] * xs + cell[4] * ys + cell[7] * zs);
-    const double Z = (cell[2] * xs + cell[5] * ys + cell[8] * zs);
-    for (int s = 0; s < ns; ++s) {
-      x[s] = x[s] + (X - cx);
-      y[s] = y[s] + (Yv - cy);
-      z[s] = z[s] + (Z - cz);
-    }
-    for (int s = 0; s < ns; ++s) {
-      x[s] = x[s] - X;
-      y[s] = y[s] - Yv;
-      z[s] = z[s] - Z;
-    }
-    rotate_sites(ns, r1 * da, r2, r3, r4, x, y, z);
-    for (int s = 0; s < ns; ++s) {
-      x[s] = x[s] + X;
-      y[s] = y[s] + Yv;
-      z[s] = z[s] + Z;
+    const double Zv = (cell[2] * xs + cell[5] * ys + cell[8] * zs);
+    x = x + (X - cx);
+    y = y + (Yv - cy);
+    z = z + (Zv - cz);
+    x = x - X;
+    y = y - Yv;
+    z = z - Zv;
+    double Q[9];
+    rot_matrix(r1 * da, r2, r3, r4, Q);
+    rot_apply(Q, x, y, z);
+    if (site) {
+      D.nx[lane] = x + X;
+      D.ny[lane] = y + Yv;
+      D.nz[lane] = z + Zv;
     }
   }
+  // trial positions: the old sites, then the new ones
   int np = 0;
   if (kind != 0) {
-    if (kind != FMC_INSERT)
-      for (int s = 0; s < ns; ++s) { D.pset[np] = 0; D.psite[np] = s; ++np; }
-    if (kind != FMC_DELETE)
-      for (int s = 0; s < ns; ++s) { D.pset[np] = 1; D.psite[np] = s; ++np; }
+    const int nold = kind != FMC_INSERT ? ns : 0;
+    np = nold + (kind != FMC_DELETE ? ns : 0);
+    if (lane < np) {
+      D.pset[lane] = lane < nold ? 0 : 1;
+      D.psite[lane] = lane < nold ? lane : lane - nold;
+    }
   }
-  D.np = np;
-  int na = 0;
-  for (int g = 0; g < Z.ngt; ++g) {
-    D.gpre[g] = na;
-    na += C.nmol[g] * Z.ns[g];
-  }
-  D.gpre[Z.ngt] = na;
-  D.nguest = na;
+  if (lane == 0) D.np = np;
+  rng_commit(R, A, lane, taken);
 }
 
 // Production count at which snapshot k of the running sums is taken.
@@ -903,7 +1005,7 @@ __global__ void __launch_bounds__(BT)
       sim[j] = in ? C.sim[k] : 0.0;
     }
   }
-  Rng R;
+  Rng R = {};
   // time budget (constant-rate counter): thread 0 raises stop_flag after
   // the move that crosses it; the flag is read after a barrier
   const long long t_start = (tid == 0 && budget_ticks > 0)
@@ -932,11 +1034,22 @@ __global__ void __launch_bounds__(BT)
     // a prepared move is in place already (written before the last
     // barrier); only a fresh proposal needs one
     const bool fresh = !spec_hit;
-    if (tid == 0) {
-      if (fresh) propose(Z, C, R, Dbuf[dsel]);
-      cur_c = R.c;
-      cur_ir = R.ir;
-      cur_jr = R.jr;
+    if (tid < kWave) {
+      if (fresh) {
+        // wave 0 proposes; its lanes take thread 0's duni state
+        R.u = rng_u[rsel];
+        R.c = lane_f(R.c, 0);
+        R.ir = __builtin_amdgcn_readlane(R.ir, 0);
+        R.jr = __builtin_amdgcn_readlane(R.jr, 0);
+        R.cd = C.cd;
+        R.cm = C.cm;
+        propose(Z, C, R, Dbuf[dsel], tid);
+      }
+      if (tid == 0) {
+        cur_c = R.c;
+        cur_ir = R.ir;
+        cur_jr = R.jr;
+      }
     }
     if (fresh) __syncthreads();
     MoveDesc& D = Dbuf[dsel];
@@ -1066,7 +1179,8 @@ __global__ void __launch_bounds__(BT)
         for (int i = lane; i < 97; i += kWave) rng_u[rsel ^ 1][i] = rng_u[rsel][i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
-        if (lane == 0) {
+        {
+          // every lane holds the same duni state
           Rng Rs;
           Rs.u = rng_u[rsel ^ 1];
           Rs.c = cur_c;
@@ -1078,13 +1192,17 @@ __global__ void __launch_bounds__(BT)
           // rejected move with a parallel phase takes one), the attempt,
           // the step and the production sample
           (void)Rs.next();
-          Cn.attempted[move_slot(D)]++;
-          Cn.steps++;
-          (void)production_sample(Cn);
-          propose(Z, Cn, Rs, Dbuf[dsel ^ 1]);
-          spec_c = Rs.c;
-          spec_ir = Rs.ir;
-          spec_jr = Rs.jr;
+          if (lane == 0) {
+            Cn.attempted[move_slot(D)]++;
+            Cn.steps++;
+            (void)production_sample(Cn);
+          }
+          propose(Z, Cn, Rs, Dbuf[dsel ^ 1], lane);
+          if (lane == 0) {
+            spec_c = Rs.c;
+            spec_ir = Rs.ir;
+            spec_jr = Rs.jr;
+          }
         }
       }
       // ---- real space: framework candidates of every position's cell and
@@ -1513,6 +1631,41 @@ __global__ void __launch_bounds__(256) grid_fill_kernel(GridParams P) {
   }
 }
 
+// Candidates of a cell ordered by their minimum-image distance from the
+// cell's centre (ties in atom order).  A trial position lies in the cell, so
+// the lanes of a wave then meet pairs of similar length, and their erfc and
+// vdW table reads fall on a few cache lines instead of one per lane.  Only
+// the order of the pair sum changes.  A cell with more than kSortCap
+// candidates keeps atom order.
+constexpr int kSortCap = 1024;
+__global__ void __launch_bounds__(256) grid_sort_kernel(GridParams P) {
+  __shared__ Rec r[kSortCap];
+  __shared__ int ty[kSortCap];
+  __shared__ double key[kSortCap];
+  const int c = blockIdx.x;
+  const int n = P.count[c];
+  if (n < 2 || n > kSortCap || n > P.cap) return;
+  double cx, cy, cz;
+  grid_center(P, c, cx, cy, cz);
+  Rec* rec = P.rec + static_cast<size_t>(c) * P.cap;
+  int* rt = P.rtype + static_cast<size_t>(c) * P.cap;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    r[i] = rec[i];
+    ty[i] = rt[i];
+    double dx = r[i].x - cx, dy = r[i].y - cy, dz = r[i].z - cz;
+    min_image(P.S, dx, dy, dz);
+    key[i] = dx * dx + dy * dy + dz * dz;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 256) {
+    const double ki = key[i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) rank += (key[j] < ki || (key[j] == ki && j < i)) ? 1 : 0;
+    rec[rank] = r[i];
+    rt[rank] = ty[i];
+  }
+}
+
 // Periodic FP64 full recompute of a chain (no reference counterpart; the
 // reference only accumulates estep, gcmc.f:981,1151,1368).  It replaces the
 // running S(k) by S_fw(k) + the guests' exact sum (ewald.f:368-630 for the
@@ -1938,6 +2091,8 @@ int build_system(fmc_batch* b, const fmc_ctx* ctx, uint64_t key,
     G.rtype = rt;
     G.cap = cap;
     hipLaunchKernelGGL(grid_fill_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
+    BCHECK(hipGetLastError());
+    hipLaunchKernelGGL(grid_sort_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
     BCHECK(hipGetLastError());
     BCHECK(hipStreamSynchronize(b->setup));
     Y.grec = rec;
