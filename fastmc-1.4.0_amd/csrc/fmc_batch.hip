@@ -421,6 +421,47 @@ __device__ __forceinline__ void trial_ds(int np, const Cplx<T>* xy, int nlm, int
   di = ai;
 }
 
+// trial_ds for the k-vectors a thread owns, kpt of its KPT slots (kpt the
+// same on every thread), positions outer: the LDS reads of one position for
+// a group of four k issue together and wait once, rather than once per
+// (k, position).  Each k accumulates over the positions in the same order
+// as trial_ds, so the sums are the same.  A slot of a group beyond the
+// thread's k-vectors carries index 0 and its sum is never used.
+template <typename T, int KPT>
+__device__ __forceinline__ void trial_ds_tile(int np, const Cplx<T>* xy, int nlm,
+                                              const Cplx<T>* zt, int zoff, int kpt,
+                                              const int (&kv)[KPT], T (&dr)[KPT],
+                                              T (&di)[KPT]) {
+#pragma clang fp contract(off)
+  constexpr int G = 4;
+  constexpr int NG = (KPT + G - 1) / G;
+#pragma unroll
+  for (int j = 0; j < KPT; ++j) {
+    dr[j] = 0;
+    di[j] = 0;
+  }
+  for (int p = 0; p < np; ++p) {
+    const Cplx<T>* xp = xy + p * nlm;
+    const Cplx<T>* zp = zt + p * kZRow + zoff;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      if (g * G < kpt) {
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj) {
+          const int j = g * G + jj;
+          if (j >= KPT) break;
+          const Cplx<T> a = xp[kv[j] & 0xffff];
+          const Cplx<T> c = zp[static_cast<signed char>((kv[j] >> 16) & 0xff)];
+          dr[j] = fmat(a.x, c.x, dr[j]);
+          dr[j] = fmat(-a.y, c.y, dr[j]);
+          di[j] = fmat(a.y, c.x, di[j]);
+          di[j] = fmat(a.x, c.y, di[j]);
+        }
+      }
+    }
+  }
+}
+
 // a_k (|S + dS|^2 - |S|^2) = a_k (2 S.dS + |dS|^2): no cancellation of |S|^2
 __device__ __forceinline__ double de_k(double ak, double s0, double s1, double dr, double di) {
 #pragma clang fp contract(off)
@@ -1150,16 +1191,18 @@ __global__ void __launch_bounds__(BT)
       } else {
         // fully unrolled (predicated, no early exit) so that sre/sim/dre/dim
         // keep static register indices
+        int kvj[KPT];
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
           const int k = tid + j * BT;
-          dre[j] = 0;
-          dim[j] = 0;
+          kvj[j] = (j < kpt && k < nk) ? kv_l[k] : 0;
+        }
+        trial_ds_tile<T, KPT>(np, xy_l, nlm, zt, Z.kmax[2], __builtin_amdgcn_readfirstlane(kpt),
+                              kvj, dre, dim);
+#pragma unroll
+        for (int j = 0; j < KPT; ++j) {
+          const int k = tid + j * BT;
           if (j < kpt && k < nk) {
-            const int v = kv_l[k];
-            const int lm = v & 0xffff;
-            const int n = static_cast<signed char>((v >> 16) & 0xff);
-            trial_ds<T>(np, xy_l, nlm, lm, zt, n + Z.kmax[2], dre[j], dim[j]);
             const double dr = dre[j], di = dim[j];
             e1 += de_k(ak_l[k], sre[j], sim[j], dr, di);
           }
