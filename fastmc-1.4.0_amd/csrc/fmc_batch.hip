@@ -1674,41 +1674,6 @@ __global__ void __launch_bounds__(256) grid_fill_kernel(GridParams P) {
   }
 }
 
-// Candidates of a cell ordered by their minimum-image distance from the
-// cell's centre (ties in atom order).  A trial position lies in the cell, so
-// the lanes of a wave then meet pairs of similar length, and their erfc and
-// vdW table reads fall on a few cache lines instead of one per lane.  Only
-// the order of the pair sum changes.  A cell with more than kSortCap
-// candidates keeps atom order.
-constexpr int kSortCap = 1024;
-__global__ void __launch_bounds__(256) grid_sort_kernel(GridParams P) {
-  __shared__ Rec r[kSortCap];
-  __shared__ int ty[kSortCap];
-  __shared__ double key[kSortCap];
-  const int c = blockIdx.x;
-  const int n = P.count[c];
-  if (n < 2 || n > kSortCap || n > P.cap) return;
-  double cx, cy, cz;
-  grid_center(P, c, cx, cy, cz);
-  Rec* rec = P.rec + static_cast<size_t>(c) * P.cap;
-  int* rt = P.rtype + static_cast<size_t>(c) * P.cap;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    r[i] = rec[i];
-    ty[i] = rt[i];
-    double dx = r[i].x - cx, dy = r[i].y - cy, dz = r[i].z - cz;
-    min_image(P.S, dx, dy, dz);
-    key[i] = dx * dx + dy * dy + dz * dz;
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < n; i += 256) {
-    const double ki = key[i];
-    int rank = 0;
-    for (int j = 0; j < n; ++j) rank += (key[j] < ki || (key[j] == ki && j < i)) ? 1 : 0;
-    rec[rank] = r[i];
-    rt[rank] = ty[i];
-  }
-}
-
 // Periodic FP64 full recompute of a chain (no reference counterpart; the
 // reference only accumulates estep, gcmc.f:981,1151,1368).  It replaces the
 // running S(k) by S_fw(k) + the guests' exact sum (ewald.f:368-630 for the
@@ -2134,8 +2099,6 @@ int build_system(fmc_batch* b, const fmc_ctx* ctx, uint64_t key,
     G.rtype = rt;
     G.cap = cap;
     hipLaunchKernelGGL(grid_fill_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
-    BCHECK(hipGetLastError());
-    hipLaunchKernelGGL(grid_sort_kernel, dim3(ncell), dim3(256), 0, b->setup, G);
     BCHECK(hipGetLastError());
     BCHECK(hipStreamSynchronize(b->setup));
     Y.grec = rec;
