@@ -2667,6 +2667,22 @@ struct CkHeader {
 constexpr char kCkMagic[8] = {'F', 'M', 'C', 'C', 'H', 'K', 0, 0};
 constexpr int kCkVersion = 1;
 
+// A duni state the generator can reach (gcmc.f:2760-2799): the table and c
+// multiples of 2^-24 in [0, 1) and [0, cm), cd and cm its constants, and the
+// two indices 64 apart (mod 97) as they start (97, 33) and step together.
+// The kernel's lookahead draws rely on the last two.
+bool rng_state_ok(const BChain& C) {
+  const float q = 16777216.0f;
+  auto grid = [q](float v) { return v >= 0.0f && v < 1.0f && v * q == std::floor(v * q); };
+  if (C.cd != 7654321.0f / q || C.cm != 16777213.0f / q) return false;
+  if (!grid(C.c) || !(C.c < C.cm)) return false;
+  if (C.ir < 1 || C.ir > 97 || C.jr < 1 || C.jr > 97 || (C.ir - C.jr + 97) % 97 != 64)
+    return false;
+  for (int i = 0; i < 97; ++i)
+    if (!grid(C.u[i])) return false;
+  return true;
+}
+
 long long ck_size(const BChain& C, int nk, const int* ns) {
   long long n = sizeof(CkHeader) + sizeof(BChain) + 2LL * nk * sizeof(double) +
                 kWinDoubles * sizeof(double);
@@ -2749,7 +2765,8 @@ extern "C" int fmc_batch_restore_chain(fmc_batch* b, const fmc_ctx* ctx, const v
   if (std::memcmp(h.magic, kCkMagic, sizeof(h.magic)) != 0 || h.version != kCkVersion ||
       h.chain_bytes != static_cast<int>(sizeof(BChain)) || h.key != key ||
       h.nk != ctx->nk || h.ntpguest != static_cast<int>(ctx->sys.guests.size()) ||
-      h.ntpguest != C.ntpguest || h.cap_atoms != C.cap_atoms || C.cap_atoms < 1)
+      h.ntpguest != C.ntpguest || h.cap_atoms != C.cap_atoms || C.cap_atoms < 1 ||
+      !rng_state_ok(C))
     return FMC_ERR_ARG;
   for (int t = 0; t < h.ntpguest; ++t)
     if (h.ns[t] != ctx->sys.guests[t].ns || C.nmol[t] < 0 ||

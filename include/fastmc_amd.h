@@ -221,7 +221,9 @@ int fmc_batch_checkpoint(fmc_batch *b, int chain_id, void *buf, long long cap,
                          long long *len);
 /* add a chain from a checkpoint; ctx must describe the system the chain was
  * admitted with (framework, cell, Ewald and force-field set-up; its guest
- * molecules are ignored), else FMC_ERR_ARG */
+ * molecules are ignored), and the duni state must be one the generator
+ * reaches (table and carry on the 2^-24 grid, indices 64 apart), else
+ * FMC_ERR_ARG */
 int fmc_batch_restore_chain(fmc_batch *b, const fmc_ctx *ctx, const void *buf,
                             long long len, int *chain_id);
 /* a chain's guest molecules of type gtype: *nmol molecules, sites

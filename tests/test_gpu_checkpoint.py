@@ -131,6 +131,19 @@ def test_checkpoint_refuses_a_foreign_system(fmc):
     for bad_ctx, bad_blob in ((ctx2, blob), (ctx, blob[:-8]), (ctx, b"x" + blob[1:])):
         with pytest.raises(FmcError):
             b.restore_chain(bad_ctx, bad_blob)
+    # a duni state the generator cannot reach: the constants cd, cm sit
+    # between the carry c and the indices ir, jr
+    import struct
+    at = blob.find(struct.pack("<ff", 7654321.0 / 16777216.0, 16777213.0 / 16777216.0))
+    assert at > 0
+    ir, jr = struct.unpack_from("<ii", blob, at + 8)
+    assert (ir - jr) % 97 == 64
+    for off, val in ((at + 8, struct.pack("<i", ir % 97 + 1)),           # indices not 64 apart
+                     (at - 4, struct.pack("<f", 0.1))):                  # carry off the 2^-24 grid
+        bad = bytearray(blob)
+        bad[off:off + 4] = val
+        with pytest.raises(FmcError):
+            b.restore_chain(ctx, bytes(bad))
     # the failed restores left the slots free
     c3 = b.restore_chain(ctx, blob)
     c4 = b.restore_chain(ctx, blob)
