@@ -9,7 +9,8 @@
 //     (k = tid + j*BT), so the reciprocal update reads no memory;
 //   * the trial's structure-factor change dS(k) stays in registers and is
 //     added to S(k) when the move is accepted;
-//   * per-site phase tables e^{i 2 pi l s} (ewald.f:97-137) sit in LDS;
+//   * per-position tables q X[l] Y[m] and Z[n] of the phase factors
+//     e^{i 2 pi l s} (ewald.f:97-137) sit in LDS;
 //   * real space walks a per-framework candidate list: the framework is cut
 //     into ~1.5 A cells and each cell lists every atom that can lie within
 //     rcut of a point in it (a superset, so the pair set -- and therefore the
@@ -954,6 +955,85 @@ __device__ __forceinline__ void recompute_finish(const SysQ& Z, BChain& C, doubl
   C.n_recompute++;
 }
 
+// The per-move tables of a trial's positions, one position per wave (waves
+// w < nw of the caller's range take positions p = w, w + nw, ...): its
+// phase factors e^{i 2 pi l f} for every dimension and l (ewald.f:97-137,
+// one sincospi per lane, into the wave's own rows of tab), then, read back
+// within the wave, its q X[l] Y[m] row and signed Z[n] row for the
+// reciprocal sum, its pair-table row, and its coordinates, charge factors
+// and candidate-list start for real space.  Positions are independent, so
+// no work-group barrier is needed inside; the caller's next barrier
+// publishes the tables.
+template <typename T>
+__device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveDesc& D,
+                                             int wave, int nw, int lane,
+                                             Cplx<double> (*tab)[3][kBTab], Cplx<T>* xy_l,
+                                             Cplx<T>* zt, int nlm) {
+  const int np = D.np, ig = D.ig, kind = D.kind;
+  const int ntp = Y.S.ntpatm;
+  const int w2 = 2 * Z.kmax[1] + 1, k3 = Z.kmax[2], nz = 2 * k3 + 1;
+  const float rw2 = 1.0f / w2;
+  for (int p = wave; p < np; p += nw) {
+    const int set = D.pset[p], s = D.psite[p];
+    const double x = set ? D.nx[s] : D.ox[s];
+    const double y = set ? D.ny[s] : D.oy[s];
+    const double z = set ? D.nz[s] : D.oz[s];
+    const double q = Z.gq[ig][s];
+    const double pq = set ? q : -q;
+    const int ts = Z.gtype[ig][s];
+    if (lane < 3 * kBTab) {
+      const int d = lane / kBTab, l = lane - d * kBTab;
+      if (l <= Z.kmax[d]) {
+        double f = Z.rcell[d] * x + Z.rcell[3 + d] * y + Z.rcell[6 + d] * z;
+        f = f - floor(f);
+        double ph = l * f;
+        ph = ph - rint(ph);
+        double sv, cv;
+        sincospi(2.0 * ph, &sv, &cv);
+        tab[p][d][l] = {cv, sv};
+      }
+    }
+    if (lane == 0) {
+      D.plo[p] = cell_of(Z, x, y, z) * Y.gcap;
+      D.ppx[p] = x;
+      D.ppy[p] = y;
+      D.ppz[p] = z;
+      D.pce[p] = q / Z.epsq * kR4pie0;
+      D.pq[p] = pq;
+      D.pts[p] = ts;
+      D.psg[p] = (kind == FMC_DISPLACE && set == 0) ? -1 : 1;
+    }
+    if (ntp <= kLdsTypes)
+      for (int tj = lane; tj < ntp; tj += kWave)
+        D.ptab[p][tj] = static_cast<short>(Y.S.pairtab[ts * ntp + tj]);
+    // the phase factors of this position, written by this wave, read back
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // X[l] * Y[m] for every (l, m) (m < 0 conjugates Y)
+    Cplx<T>* xr = xy_l + p * nlm;
+    for (int r = lane; r < nlm; r += kWave) {
+      const int l = div_rc(r, w2, rw2), m = r - l * w2 - Z.kmax[1];
+      const Cplx<double> a = tab[p][0][l];
+      const Cplx<double> b = tab[p][1][m < 0 ? -m : m];
+      double ox, oy;
+      if (m >= 0) {
+        ox = a.x * b.x - a.y * b.y;
+        oy = a.y * b.x + b.y * a.x;
+      } else {
+        ox = a.x * b.x + a.y * b.y;
+        oy = a.y * b.x - b.y * a.x;
+      }
+      xr[r] = {static_cast<T>(pq * ox), static_cast<T>(pq * oy)};
+    }
+    for (int i = lane; i < nz; i += kWave) {
+      const int n = i - k3;
+      const Cplx<double> c = tab[p][2][n < 0 ? -n : n];
+      zt[p * kZRow + i] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The persistent kernel
 // ---------------------------------------------------------------------------
@@ -1093,86 +1173,23 @@ __global__ void __launch_bounds__(BT)
       }
     }
     if (fresh) __syncthreads();
-    MoveDesc& D = Dbuf[dsel];
+    // dsel is written only by thread 0's decision, after the reduction barrier
+    const int dcur = dsel;
+    MoveDesc& D = Dbuf[dcur];
     const int kind = D.kind;
     const int ns = D.ns;
     const int ig = D.ig;
     double e1 = 0.0, e2 = 0.0, ev = 0.0;
     if (kind != 0) {
       const int np = D.np;
-      // phase factors e^{i 2 pi l f} of every position, dimension and l
-      // (ewald.f:97-137), one sincospi each, in parallel; candidate-list
-      // bounds of every position's cell by the last wave
-      {
-        const int nl = kBTab;
-        for (int t = tid; t < np * 3 * nl; t += BT) {
-          const int p = t / (3 * nl), r = t - p * 3 * nl;
-          const int d = r / nl, l = r - d * nl;
-          if (l > Z.kmax[d]) continue;
-          const int set = D.pset[p], s = D.psite[p];
-          const double x = set ? D.nx[s] : D.ox[s];
-          const double y = set ? D.ny[s] : D.oy[s];
-          const double z = set ? D.nz[s] : D.oz[s];
-          double f = Z.rcell[d] * x + Z.rcell[3 + d] * y + Z.rcell[6 + d] * z;
-          f = f - floor(f);
-          double ph = l * f;
-          ph = ph - rint(ph);
-          double sv, cv;
-          sincospi(2.0 * ph, &sv, &cv);
-          tab[p][d][l] = {cv, sv};
-        }
-        if (tid >= BT - kWave && tid < BT - kWave + np) {
-          const int p = tid - (BT - kWave);
-          const int set = D.pset[p], s = D.psite[p];
-          const int c = set ? cell_of(Z, D.nx[s], D.ny[s], D.nz[s])
-                            : cell_of(Z, D.ox[s], D.oy[s], D.oz[s]);
-          D.plo[p] = c * Y.gcap;
-          D.ppx[p] = set ? D.nx[s] : D.ox[s];
-          D.ppy[p] = set ? D.ny[s] : D.oy[s];
-          D.ppz[p] = set ? D.nz[s] : D.oz[s];
-          D.pce[p] = Z.gq[ig][s] / Z.epsq * kR4pie0;
-          D.pq[p] = set ? Z.gq[ig][s] : -Z.gq[ig][s];
-          D.pts[p] = Z.gtype[ig][s];
-          D.psg[p] = (kind == FMC_DISPLACE && set == 0) ? -1 : 1;
-        }
+      // the positions' tables: built by every wave for a fresh proposal;
+      // a prepared one had them built while the previous move was decided
+      if (fresh) {
+        build_tables<T>(Z, Y, D, tid / kWave, BT / kWave, tid % kWave, tab, xy_l, zt, nlm);
+        __syncthreads();
       }
-      __syncthreads();
-      // pairtab rows of the positions' site types
       const int ntp = Y.S.ntpatm;
-      const float rntp = 1.0f / ntp;
       const bool ldsrow = ntp <= kLdsTypes;
-      if (ldsrow)
-        for (int t = tid; t < np * ntp; t += BT) {
-          const int p = div_rc(t, ntp, rntp), tj = t - p * ntp;
-          D.ptab[p][tj] = static_cast<short>(Y.S.pairtab[D.pts[p] * ntp + tj]);
-        }
-      // X[l] * Y[m] for every (l, m) and position (m < 0 conjugates Y)
-      {
-        const int w2 = 2 * Z.kmax[1] + 1;
-        const float rnlm = 1.0f / nlm, rw2 = 1.0f / w2;
-        for (int t = tid; t < np * nlm; t += BT) {
-          const int p = div_rc(t, nlm, rnlm), r = t - p * nlm;
-          const int l = div_rc(r, w2, rw2), m = r - l * w2 - Z.kmax[1];
-          const Cplx<double> a = tab[p][0][l];
-          const Cplx<double> b = tab[p][1][m < 0 ? -m : m];
-          double ox, oy;
-          if (m >= 0) {
-            ox = a.x * b.x - a.y * b.y;
-            oy = a.y * b.x + b.y * a.x;
-          } else {
-            ox = a.x * b.x + a.y * b.y;
-            oy = a.y * b.x - b.y * a.x;
-          }
-          xy_l[t] = {static_cast<T>(D.pq[p] * ox), static_cast<T>(D.pq[p] * oy)};
-        }
-        const int k3 = Z.kmax[2], nz = 2 * k3 + 1;
-        for (int t = tid; t < np * nz; t += BT) {
-          const int p = div_rc(t, nz, 1.0f / nz), n = t - p * nz - k3;
-          const Cplx<double> c = tab[p][2][n < 0 ? -n : n];
-          zt[p * kZRow + n + k3] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
-        }
-      }
-      __syncthreads();
       // ---- reciprocal space (ewald1_guest, ewald.f:141-319) ----
       // dS(k) = sum_new q e^{ik.r} - sum_old q e^{ik.r}
       if (STREAM) {
@@ -1372,6 +1389,15 @@ __global__ void __launch_bounds__(BT)
       red[2][tid / kWave] = ev;
     }
     __syncthreads();
+    // while thread 0 decides, waves 1.. build the tables of the prepared
+    // next move (this move's are read no more); the next move uses them if
+    // it adopts the prepared proposal, and rebuilds them otherwise
+    if (kind != 0 && tid >= kWave) {
+      MoveDesc& Dn = Dbuf[dcur ^ 1];
+      if (Dn.kind != 0)
+        build_tables<T>(Z, Y, Dn, tid / kWave - 1, BT / kWave - 1, tid % kWave, tab, xy_l, zt,
+                        nlm);
+    }
     if (tid == 0) {
       double a = 0.0, b = 0.0, c = 0.0;
       for (int w = 0; w < BT / kWave; ++w) {
