@@ -16,7 +16,9 @@
 //     rcut of a point in it (a superset, so the pair set -- and therefore the
 //     energy -- is exactly that of the reference's all-atom scan), plus all
 //     guest atoms;
-//   * one lane runs the serial part (RNG draws, proposal, acceptance).
+//   * the serial part runs wave-wide (RNG draws and proposal: every lane
+//     the same scalar steps, lane i draw i, lane s site s) or on one lane
+//     (acceptance and bookkeeping).
 // Precision FP64 reproduces the oracle's accept/reject stream; MIXED does
 // the per-site phase products and the pair interpolation in FP32 with FP64
 // coordinates, S(k), energies and acceptance.
@@ -372,26 +374,6 @@ struct Cplx {
   T x, y;
 };
 
-// e^{i k.r} * q for one site from the LDS phase tables, precision T
-template <typename T>
-__device__ __forceinline__ Cplx<T> sfac(const Cplx<T>* X, const Cplx<T>* Y,
-                                        const Cplx<T>* Z, int l, int m, int n,
-                                        T q) {
-  const Cplx<T> a = X[l];
-  const Cplx<T> b = Y[m < 0 ? -m : m];
-  const Cplx<T> c = Z[n < 0 ? -n : n];
-  T clm, slm;
-  if (m >= 0) {
-    clm = a.x * b.x - a.y * b.y;
-    slm = a.y * b.x + b.y * a.x;
-  } else {
-    clm = a.x * b.x + a.y * b.y;
-    slm = a.y * b.x - b.y * a.x;
-  }
-  if (n >= 0) return {q * (clm * c.x - slm * c.y), q * (slm * c.x + clm * c.y)};
-  return {q * (clm * c.x + slm * c.y), q * (slm * c.x - clm * c.y)};
-}
-
 __device__ __forceinline__ float fmat(float a, float b, float c) {
   return __builtin_fmaf(a, b, c);
 }
@@ -486,7 +468,7 @@ __device__ __forceinline__ int um(int v) { return static_cast<signed char>((v >>
 __device__ __forceinline__ int un(int v) { return static_cast<signed char>((v >> 16) & 0xff); }
 
 // ---------------------------------------------------------------------------
-// The serial lane: move selection, adaptive steps, draws and proposal
+// The proposal: move selection, adaptive steps, draws and trial positions
 // (gcmc.f:745-847, 855-877, 1010-1060, 1176-1248; utility_pack.f:2545-3047)
 // ---------------------------------------------------------------------------
 // The proposal as one wave computes it: every lane runs the same scalar
