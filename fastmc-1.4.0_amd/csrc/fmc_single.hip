@@ -235,6 +235,26 @@ struct RecipFullParams {
   double2* part;   // [nchunk][nk]
 };
 
+// A molecule's sites written into the resident guest arrays from kernel
+// arguments (an accepted insert or displacement), so a commit enqueues no
+// host-to-device copy; stream order puts it before the next trial.
+struct SitesArgs {
+  double* gx;
+  double* gy;
+  double* gz;
+  int lo, ns;
+  double x[kMaxSites], y[kMaxSites], z[kMaxSites];
+};
+
+__global__ void put_sites_kernel(SitesArgs A) {
+  const int s = threadIdx.x;
+  if (s < A.ns) {
+    A.gx[A.lo + s] = A.x[s];
+    A.gy[A.lo + s] = A.y[s];
+    A.gz[A.lo + s] = A.z[s];
+  }
+}
+
 __global__ void __launch_bounds__(kBS) recip_partial_kernel(RecipFullParams P) {
   __shared__ double4 at[kBS];
   const int c = blockIdx.y;
@@ -858,12 +878,12 @@ extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
   P.nb_real = std::max(1, std::min(c->nb_real_max, (ntot + kBS - 1) / kBS));
   P.partial = c->d_partial;
   P.counter = c->d_counter;
-  P.out = c->d_out;
+  // the last work-group writes the three sums straight into the pinned
+  // host buffer (device-addressable), visible once the stream has drained
+  P.out = c->h_out;
   hipLaunchKernelGGL(trial_kernel, dim3(P.nb_recip + P.nb_real), dim3(kBS), 0,
                      c->stream, P);
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, 3 * sizeof(double),
-                          hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   out[0] = c->h_out[0];
   out[1] = c->h_out[1];
@@ -915,7 +935,23 @@ extern "C" int fmc_commit(fmc_ctx* c) {
     lo = m * ns;
     hi = (m + 1) * ns;
   }
-  if (hi > lo) {
+  if (c->pending != FMC_DELETE) {
+    // ns sites from kernel arguments
+    SitesArgs A;
+    A.gx = c->d_gx[gt];
+    A.gy = c->d_gy[gt];
+    A.gz = c->d_gz[gt];
+    A.lo = lo;
+    A.ns = ns;
+    for (int s = 0; s < kMaxSites; ++s) {
+      A.x[s] = s < ns ? c->pend_x[s] : 0.0;
+      A.y[s] = s < ns ? c->pend_y[s] : 0.0;
+      A.z[s] = s < ns ? c->pend_z[s] : 0.0;
+    }
+    hipLaunchKernelGGL(put_sites_kernel, dim3(1), dim3(kMaxSites), 0, c->stream, A);
+    HIPCHECK(hipGetLastError());
+  } else if (hi > lo) {
+    // a delete shifts the successors: the range from the host mirror
     const size_t nb = (hi - lo) * sizeof(double);
     HIPCHECK(hipMemcpyAsync(c->d_gx[gt] + lo, c->hx[gt].data() + lo, nb,
                             hipMemcpyHostToDevice, c->stream));
@@ -923,8 +959,8 @@ extern "C" int fmc_commit(fmc_ctx* c) {
                             hipMemcpyHostToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->d_gz[gt] + lo, c->hz[gt].data() + lo, nb,
                             hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
   }
-  HIPCHECK(hipStreamSynchronize(c->stream));
   c->cur = 1 - c->cur;  // S(k) <- S_new
   c->pending = 0;
   return FMC_OK;
