@@ -255,6 +255,30 @@ __global__ void put_sites_kernel(SitesArgs A) {
   }
 }
 
+// An accepted delete: the successors of the deleted molecule move down by
+// ns sites (gcmc.f:1127-1131) in the resident arrays, [lo, hi) <- [lo + ns,
+// hi + ns), by one work-group: each pass reads its span before a barrier
+// and writes it after, so no read sees a slot already overwritten.
+__global__ void __launch_bounds__(kBS) shift_sites_kernel(double* gx, double* gy, double* gz,
+                                                          int lo, int hi, int ns) {
+  for (int base = lo; base < hi; base += kBS) {
+    const int i = base + threadIdx.x;
+    double x = 0.0, y = 0.0, z = 0.0;
+    if (i < hi) {
+      x = gx[i + ns];
+      y = gy[i + ns];
+      z = gz[i + ns];
+    }
+    __syncthreads();
+    if (i < hi) {
+      gx[i] = x;
+      gy[i] = y;
+      gz[i] = z;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void __launch_bounds__(kBS) recip_partial_kernel(RecipFullParams P) {
   __shared__ double4 at[kBS];
   const int c = blockIdx.y;
@@ -951,15 +975,10 @@ extern "C" int fmc_commit(fmc_ctx* c) {
     hipLaunchKernelGGL(put_sites_kernel, dim3(1), dim3(kMaxSites), 0, c->stream, A);
     HIPCHECK(hipGetLastError());
   } else if (hi > lo) {
-    // a delete shifts the successors: the range from the host mirror
-    const size_t nb = (hi - lo) * sizeof(double);
-    HIPCHECK(hipMemcpyAsync(c->d_gx[gt] + lo, c->hx[gt].data() + lo, nb,
-                            hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipMemcpyAsync(c->d_gy[gt] + lo, c->hy[gt].data() + lo, nb,
-                            hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipMemcpyAsync(c->d_gz[gt] + lo, c->hz[gt].data() + lo, nb,
-                            hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    // a delete shifts the successors down on the device
+    hipLaunchKernelGGL(shift_sites_kernel, dim3(1), dim3(kBS), 0, c->stream, c->d_gx[gt],
+                       c->d_gy[gt], c->d_gz[gt], lo, hi, ns);
+    HIPCHECK(hipGetLastError());
   }
   c->cur = 1 - c->cur;  // S(k) <- S_new
   c->pending = 0;
