@@ -72,6 +72,33 @@ def test_batch_fp64_matches_oracle_chain(fmc, oracle, tile):
         assert not st["capacity_fault"]
 
 
+@pytest.mark.parametrize("kmax", [12, 14])
+def test_batch_fp64_large_nk_register_tiles(fmc, oracle, kmax):
+    """The register tiles of 12 and 14 k-vectors per thread at 512 threads
+    (cubic cell, Nk 4192 and about 6650; at 14 the grouped reciprocal sum
+    ends in a partial group of two) reproduce the oracle's trajectory."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd import synth
+    rng = np.random.default_rng(31 + kmax)
+    fw = synth.make_framework(rng, (26.0, 26.0, 26.0), 260)
+    w = synth.Workload("porous", fw, ["co2"], [20.0], 298.0, (kmax, kmax, kmax))
+    A = w.system_arrays()
+    n, seed = 400, (7, 11, 13, 17)
+    ctx = fmc.Context.from_arrays(A)
+    nk = ctx.info()["nk"]
+    assert (4096 < nk <= 6144) if kmax == 12 else (6144 < nk <= 7168), nk
+    b = Batch(1)
+    cid = b.add_chain(ctx, 298.0, [20.0e5], seed, delr=A["delr"])
+    b.enable_log(cid, n)
+    b.set_tile(512)
+    b.run(n)
+    log = b.get_log(cid, n)
+    b.close()
+    ref = _oracle_recs(oracle, A, 298.0, [20.0e5], seed, None, n)[0]
+    worst = _compare(log, ref)
+    print(f"kmax {kmax}, Nk {nk}: {int(ref[:, 2].sum())} accepted, worst estep rel {worst:.2e}")
+
+
 def test_batch_mixed_precision_error(fmc, oracle):
     """Mixed precision on the same systems: per-move estep within 1e-6
     relative of FP64 while the trajectories agree; loadings evolve
