@@ -134,6 +134,42 @@ def test_batch_mixed_precision_error(fmc, oracle):
     assert floored.max() < 1e-2
 
 
+@pytest.mark.parametrize("kmax", [12, 14])
+def test_batch_mixed_large_nk_register_tiles(fmc, oracle, kmax):
+    """Mixed precision on the 512x12 and 512x14 register tiles: the same
+    decisions as the FP64 oracle while the trajectories agree, and the
+    estep error bars of the small-system test."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd.capi import PREC_MIXED
+    from fastmc_amd import synth
+    rng = np.random.default_rng(31 + kmax)
+    fw = synth.make_framework(rng, (26.0, 26.0, 26.0), 260)
+    w = synth.Workload("porous", fw, ["co2"], [20.0], 298.0, (kmax, kmax, kmax))
+    A = w.system_arrays()
+    n, seed = 400, (7, 11, 13, 17)
+    ref = _oracle_recs(oracle, A, 298.0, [20.0e5], seed, None, n)[0]
+    b = Batch(1, PREC_MIXED)
+    ctx = fmc.Context.from_arrays(A)
+    cid = b.add_chain(ctx, 298.0, [20.0e5], seed, delr=A["delr"])
+    b.enable_log(cid, n)
+    b.set_tile(512)
+    b.run(n)
+    log = b.get_log(cid, n)
+    b.close()
+    same = np.all(log[:, :4] == ref[:, :4], axis=1)
+    first_diff = np.argmin(same) if not same.all() else n
+    assert first_diff > 50
+    kT = 1.9872041e-3 * 298.0
+    a, m = ref[:first_diff, 4], log[:first_diff, 4]
+    keep = np.abs(a) < 50 * kT
+    err = np.abs(m - a)[keep]
+    floored = err / np.maximum(np.abs(a[keep]), 1e-3 * kT)
+    print(f"kmax {kmax}: identical for {first_diff} moves; max err {err.max() / kT:.2e} kT, "
+          f"median {np.median(err) / kT:.2e} kT; floored rel max {floored.max():.2e}")
+    assert err.max() < 1e-3 * kT
+    assert floored.max() < 1e-2
+
+
 def test_batch_recompute_matches_oracle_energy(fmc, oracle):
     """FP64 recompute after 1500 moves: the full guest energy equals the
     oracle's single_point of the same (identical-trajectory) configuration,
