@@ -42,6 +42,7 @@ GUEST_SITES = {
     "Ow": (15.9994, 0.1550, 3.1536),  # water oxygen
     "Hw": (1.00794, 0.0, 0.0),
     "Mw": (0.0, 0.0, 0.0),
+    "Ar": (39.948, 0.2381, 3.405),   # argon, one uncharged LJ site
 }
 
 
@@ -72,11 +73,14 @@ def guest_template(name: str):
                 ("Hw", 1.00794, 0.52, hx, 0.0, hz),
                 ("Hw", 1.00794, 0.52, -hx, 0.0, hz),
                 ("Mw", 0.0, -1.04, 0.0, 0.0, 0.15)]
+    if name == "ar":
+        return [("Ar", 39.948, 0.0, 0.0, 0.0, 0.0)]
     raise ValueError(name)
 
 
 GUEST_TITLE = {"co2": "CO2 rigid 3-site", "n2": "N2 3-site with centre charge",
-               "ch4": "CH4 rigid 5-site", "h2o": "H2O rigid 4-site"}
+               "ch4": "CH4 rigid 5-site", "h2o": "H2O rigid 4-site",
+               "ar": "Ar single LJ site"}
 
 
 @dataclasses.dataclass

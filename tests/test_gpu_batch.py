@@ -99,6 +99,61 @@ def test_batch_fp64_large_nk_register_tiles(fmc, oracle, kmax):
     print(f"kmax {kmax}, Nk {nk}: {int(ref[:, 2].sum())} accepted, worst estep rel {worst:.2e}")
 
 
+def test_single_site_uncharged_guest(fmc, oracle):
+    """A one-site uncharged guest (argon) beside CO2: one trial position per
+    set, a centre of mass over one site and zero charges in every Ewald
+    term.  The batch chain (FP64) follows the oracle's trajectory and the
+    single-chain drop-in matches its trial components.
+
+    Rotating a one-site guest changes no energy: estep is rounding noise of
+    order 1e-14 in both codes, and its sign decides whether the acceptance
+    draw is taken (gcmc.f:1330-1338), so the streams may part there.  With
+    rotate-only moves of argon on, the trajectories agree up to the first
+    such move, where both esteps are below 1e-12 kcal/mol; with them off
+    they agree throughout."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd import synth
+    rng = np.random.default_rng(21)
+    fw = synth.make_framework(rng, (26.0, 26.0, 26.0), 260)
+    w = synth.Workload("porous", fw, ["co2", "ar"], [4.0, 8.0], 298.0, (8, 8, 8))
+    A = w.system_arrays()
+    ctx = fmc.Context.from_arrays(A)
+    seed = (3, 5, 7, 11)
+    co2 = [1, 1, 1, 0, 0, 0, 0.5, 0.5, 0]
+    for ar, n in (([1, 1, 1, 0, 0, 0, 0.5, 0.5, 0], 300), ([1, 1, 1, 0, 0, 0, 0.5, 0, 0], 900)):
+        freq = [co2, ar]
+        ref = _oracle_recs(oracle, A, 298.0, [4.0e5, 8.0e5], seed, freq, n)[0]
+        assert set(ref[:, 1].astype(int)) == {0, 1} and ref[ref[:, 1] == 1, 2].sum() > 0
+        b = Batch(1)
+        cid = b.add_chain(ctx, 298.0, [4.0e5, 8.0e5], seed, freq=freq, delr=A["delr"])
+        b.enable_log(cid, n)
+        b.run(n)
+        log = b.get_log(cid, n)
+        b.close()
+        tie = np.nonzero((ref[:, 0] == 5) & (ref[:, 1] == 1))[0]
+        if ar[7] > 0:
+            assert len(tie) > 0
+            t = tie[0]
+            _compare(log[:t], ref[:t])
+            assert np.array_equal(log[t, :2], ref[t, :2])
+            assert abs(log[t, 4]) < 1e-12 and abs(ref[t, 4]) < 1e-12, (log[t, 4], ref[t, 4])
+        else:
+            assert len(tie) == 0
+            _compare(log, ref)
+    # the drop-in: insert, displace and delete one argon atom
+    orc = oracle.OracleSystem(A["imcon"], A["cell"], A["alpha"], A["kmax"], A["rcut"],
+                              A["delr"], A["ntpatm"], A["vdw"], A["fw"], A["guests"])
+    x, y, z = np.array([7.3]), np.array([5.1]), np.array([9.2])
+    for kind, m, xyz in ((1, -1, (x, y, z)), (3, 0, (x + 0.4, y - 0.3, z + 0.2)), (2, 0, None)):
+        args = xyz if xyz is not None else (None, None, None)
+        a = ctx.trial(kind, 1, m, *args)
+        r = orc.trial(kind, 1, m, *args)
+        assert np.all(np.abs(a - r) <= 1e-10 * np.maximum(np.abs(r), 1.0)), (kind, a, r)
+        assert a[1] == 0.0   # no charge: no real-space Coulomb term
+        ctx.commit()
+        orc.commit()
+
+
 def test_batch_mixed_precision_error(fmc, oracle):
     """Mixed precision on the same systems: per-move estep within 1e-6
     relative of FP64 while the trajectories agree; loadings evolve
