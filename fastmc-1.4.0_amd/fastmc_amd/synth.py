@@ -43,6 +43,8 @@ GUEST_SITES = {
     "Hw": (1.00794, 0.0, 0.0),
     "Mw": (0.0, 0.0, 0.0),
     "Ar": (39.948, 0.2381, 3.405),   # argon, one uncharged LJ site
+    "Ce": (12.0107, 0.0660, 3.50),   # ethane carbon
+    "He": (1.00794, 0.0300, 2.50),   # ethane hydrogen
 }
 
 
@@ -75,12 +77,26 @@ def guest_template(name: str):
                 ("Mw", 0.0, -1.04, 0.0, 0.0, 0.15)]
     if name == "ar":
         return [("Ar", 39.948, 0.0, 0.0, 0.0, 0.0)]
+    if name == "c2h6":
+        # rigid staggered ethane, all 8 atoms: C-C 1.54 A, C-H 1.09 A,
+        # tetrahedral angles, q_C = -0.18 e, q_H = +0.06 e
+        cc, ch = 1.54, 1.09
+        th = math.acos(-1.0 / 3.0)
+        rho, dz = ch * math.sin(th), ch * math.cos(th)
+        sites = [("Ce", 12.0107, -0.18, 0.0, 0.0, cc / 2), ("Ce", 12.0107, -0.18, 0.0, 0.0, -cc / 2)]
+        for k in range(3):
+            a = 2.0 * math.pi * k / 3.0
+            sites.append(("He", 1.00794, 0.06, rho * math.cos(a), rho * math.sin(a), cc / 2 - dz))
+        for k in range(3):
+            a = 2.0 * math.pi * k / 3.0 + math.pi / 3.0
+            sites.append(("He", 1.00794, 0.06, rho * math.cos(a), rho * math.sin(a), -cc / 2 + dz))
+        return sites
     raise ValueError(name)
 
 
 GUEST_TITLE = {"co2": "CO2 rigid 3-site", "n2": "N2 3-site with centre charge",
                "ch4": "CH4 rigid 5-site", "h2o": "H2O rigid 4-site",
-               "ar": "Ar single LJ site"}
+               "ar": "Ar single LJ site", "c2h6": "C2H6 rigid 8-site"}
 
 
 @dataclasses.dataclass

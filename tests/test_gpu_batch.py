@@ -154,6 +154,57 @@ def test_single_site_uncharged_guest(fmc, oracle):
         orc.commit()
 
 
+def test_eight_site_guest(fmc, oracle):
+    """The largest rigid guest the device takes (8 sites: all-atom ethane)
+    beside CO2: 16 trial positions for a displacement, several positions
+    per table-building wave.  FP64 batch chain against the oracle's
+    trajectory, mixed precision within the error bars of the small-system
+    test, and the drop-in's components for one insert, displace, delete."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd.capi import PREC_MIXED
+    from fastmc_amd import synth
+    rng = np.random.default_rng(8)
+    fw = synth.make_framework(rng, (26.0, 26.0, 26.0), 260)
+    w = synth.Workload("porous", fw, ["co2", "c2h6"], [4.0, 2.0], 298.0, (8, 8, 8))
+    A = w.system_arrays()
+    ctx = fmc.Context.from_arrays(A)
+    assert list(ctx.nsites) == [3, 8]
+    n, seed = 900, (9, 8, 7, 6)
+    freq = [[1, 1, 1, 0, 0, 0, 0.5, 0.5, 0]] * 2
+    ref = _oracle_recs(oracle, A, 298.0, [4.0e5, 2.0e5], seed, freq, n)[0]
+    assert ref[(ref[:, 1] == 1) & (ref[:, 0] == 3), 2].sum() > 0   # accepted ethane displacements
+    logs = {}
+    for prec in (0, PREC_MIXED):
+        b = Batch(1, prec) if prec else Batch(1)
+        cid = b.add_chain(ctx, 298.0, [4.0e5, 2.0e5], seed, freq=freq, delr=A["delr"])
+        b.enable_log(cid, n)
+        b.run(n)
+        logs[prec] = b.get_log(cid, n)
+        b.close()
+    _compare(logs[0], ref)
+    m = logs[PREC_MIXED]
+    same = np.all(m[:, :4] == ref[:, :4], axis=1)
+    first_diff = np.argmin(same) if not same.all() else n
+    assert first_diff > 50
+    kT = 1.9872041e-3 * 298.0
+    a = ref[:first_diff, 4]
+    keep = np.abs(a) < 50 * kT
+    err = np.abs(m[:first_diff, 4] - a)[keep]
+    assert err.max() < 1e-3 * kT
+    # the drop-in
+    orc = oracle.OracleSystem(A["imcon"], A["cell"], A["alpha"], A["kmax"], A["rcut"],
+                              A["delr"], A["ntpatm"], A["vdw"], A["fw"], A["guests"])
+    g = A["guests"][1]
+    x, y, z = [np.asarray(g[k]) + 7.3 for k in "xyz"]
+    for kind, mol, xyz in ((1, -1, (x, y, z)), (3, 0, (x + 0.4, y - 0.3, z + 0.2)), (2, 0, None)):
+        args = xyz if xyz is not None else (None, None, None)
+        r1 = ctx.trial(kind, 1, mol, *args)
+        r2 = orc.trial(kind, 1, mol, *args)
+        assert np.all(np.abs(r1 - r2) <= 1e-10 * np.maximum(np.abs(r2), 1.0)), (kind, r1, r2)
+        ctx.commit()
+        orc.commit()
+
+
 def test_batch_mixed_precision_error(fmc, oracle):
     """Mixed precision on the same systems: per-move estep within 1e-6
     relative of FP64 while the trajectories agree; loadings evolve
