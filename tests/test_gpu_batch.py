@@ -205,6 +205,39 @@ def test_eight_site_guest(fmc, oracle):
         orc.commit()
 
 
+def test_four_guest_types_one_at_zero_pressure(fmc, oracle):
+    """Four guest types, the device maximum (CO2, N2, CH4, argon), one of
+    them at zero pressure: the guest-type draw, the per-type atom offsets
+    and the 4x4 tail form at full size, and energy_eval's zero-pressure
+    insert (utility_pack.f:3049-3095: a zero prefactor, never accepted),
+    where no rejection bound applies; with no molecule of that type every
+    move drawn for it becomes such an insert (gcmc.f:837-847).  FP64 batch
+    chain against the oracle's trajectory."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd import synth
+    rng = np.random.default_rng(44)
+    fw = synth.make_framework(rng, (26.0, 26.0, 26.0), 260)
+    w = synth.Workload("porous", fw, ["co2", "n2", "ch4", "ar"], [4.0, 6.0, 3.0, 0.0],
+                       298.0, (8, 8, 8))
+    A = w.system_arrays()
+    ctx = fmc.Context.from_arrays(A)
+    press = [4.0e5, 6.0e5, 3.0e5, 0.0]
+    n, seed = 1200, (4, 3, 2, 1)
+    freq = [[1, 1, 1, 0, 0, 0, 0.5, 0.5, 0]] * 3 + [[1, 1, 1, 0, 0, 0, 0.5, 0, 0]]
+    ref = _oracle_recs(oracle, A, 298.0, press, seed, freq, n)[0]
+    assert set(ref[:, 1].astype(int)) == {0, 1, 2, 3}
+    # the zero-pressure guest is tried and never inserted
+    ar = ref[:, 1] == 3
+    assert ar.sum() > 0 and not np.any(ref[ar & (ref[:, 0] == 1), 2])
+    b = Batch(1)
+    cid = b.add_chain(ctx, 298.0, press, seed, freq=freq, delr=A["delr"])
+    b.enable_log(cid, n)
+    b.run(n)
+    log = b.get_log(cid, n)
+    b.close()
+    _compare(log, ref)
+
+
 def test_batch_mixed_precision_error(fmc, oracle):
     """Mixed precision on the same systems: per-move estep within 1e-6
     relative of FP64 while the trajectories agree; loadings evolve
