@@ -99,13 +99,19 @@ class Server:
     def _retire(self, now: float, cv: threading.Condition):
         for cid in self.batch.poll():
             with cv:
-                req, t_adm, need = self._live.pop(cid)
+                req, t_adm, need = self._live[cid]
             st = self.batch.stats(cid)
             rep = None
             if st["win_len"] > 0 and not st["capacity_fault"]:
                 rep = self.batch.report(cid, req.temp)
-            self.batch.remove_chain(cid)
+            # the library's slot and the _live entry go together, under the
+            # lock: the admission thread counts slots from _live, so it never
+            # adds a chain while this one still holds a library slot, and a
+            # chain it adds into the freed slot is recorded after this entry
+            # is gone
             with cv:
+                self.batch.remove_chain(cid)
+                del self._live[cid]
                 self._used -= need
                 self.moves_done += st["steps"]
                 self.results[req.rid] = Result(req.rid, req.arrival, t_adm, now, st, need,
