@@ -99,6 +99,9 @@ class Server:
     def _retire(self, now: float, cv: threading.Condition):
         for cid in self.batch.poll():
             with cv:
+                # a chain runs from its add_chain on; one that finished in its
+                # first slice may not be recorded by the admission thread yet
+                cv.wait_for(lambda: cid in self._live)
                 req, t_adm, need = self._live[cid]
             st = self.batch.stats(cid)
             rep = None
