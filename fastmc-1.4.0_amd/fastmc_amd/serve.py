@@ -216,6 +216,13 @@ class Server:
                         state["stop"] = True
                         cv.notify_all()
                     break
+        except BaseException:
+            # a failed slice: the admission thread may be waiting for a slot
+            # that will not come; stop it before joining
+            with cv:
+                state["stop"] = True
+                cv.notify_all()
+            raise
         finally:
             th.join()
         if state["error"] is not None:

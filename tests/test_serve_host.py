@@ -78,3 +78,47 @@ def test_retire_waits_for_a_chain_the_admission_has_not_recorded():
     srv._retire(1.0, cv)
     t.join()
     assert srv._live == {} and srv._used == 0 and srv.results[9].stats["steps"] == 3
+
+
+def test_a_failed_slice_ends_the_run_instead_of_hanging(monkeypatch):
+    """When a slice fails, run() stops the admission thread (which may be
+    waiting for a slot) and raises the error."""
+    from fastmc_amd import serve
+
+    class Ctx:
+        def close(self):
+            pass
+
+    monkeypatch.setattr(serve.Context, "from_arrays", staticmethod(lambda arrays: Ctx()))
+    monkeypatch.setattr(serve, "footprint", lambda ctx, precision: 1)
+
+    class Batch:
+        def add_chain(self, ctx, temp, press, seed, **kw):
+            return 0
+
+        def launch(self, n):
+            raise RuntimeError("slice failed")
+
+    srv = serve.Server.__new__(serve.Server)
+    srv.slots, srv.precision, srv.slice_moves, srv.slice_ms = 1, 0, 10, 0.0
+    srv.hbm_budget, srv.clock, srv.results = 1 << 30, serve.time.perf_counter, {}
+    srv.batch = Batch()
+    srv._live, srv._used, srv._inflight = {}, 0, 0
+    srv.slices = srv.moves_done = 0
+    srv.suspended, srv.unstarted = [], []
+    reqs = [types.SimpleNamespace(rid=i, arrival=0.0, arrays=None, temp=298.0,
+                                  press_pa=[1e5], seed=(1, 2, 3, 4), delr=1.0,
+                                  eq_steps=0, moves=100) for i in range(3)]
+    out = {}
+
+    def go():
+        try:
+            srv.run(reqs, realtime=False)
+        except RuntimeError as e:
+            out["error"] = str(e)
+
+    t = threading.Thread(target=go, daemon=True)
+    t.start()
+    t.join(timeout=20)
+    assert not t.is_alive(), "run() hung after a failed slice"
+    assert out.get("error") == "slice failed"
