@@ -238,6 +238,35 @@ def test_four_guest_types_one_at_zero_pressure(fmc, oracle):
     _compare(log, ref)
 
 
+def test_bench_workload_chains_fp64(fmc, oracle):
+    """Three chains of the 256-chain bench workload (orthorhombic cells of
+    unequal sides, a kmax per axis, 471 to 5445 framework atoms, CO2 at
+    10 bar) in one FP64 batch: each follows the oracle's trajectory over
+    2000 moves from an empty framework."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd import synth
+    ws = synth.c3()
+    picks = [255, 151, 183]   # frameworks 31, 18, 22 at the highest pressure
+    n = 2000                  # the first insertions are accepted after 270-960 moves
+    b = Batch(len(picks))
+    ids, refs = [], []
+    for i in picks:
+        w = ws[i]
+        A = w.system_arrays()
+        ctx = fmc.Context.from_arrays(A)
+        press = [p * 1e5 for p in w.press_bar]
+        cid = b.add_chain(ctx, w.temp, press, w.seed, delr=w.delr)
+        b.enable_log(cid, n)
+        ids.append(cid)
+        refs.append(_oracle_recs(oracle, A, w.temp, press, w.seed, None, n)[0])
+    b.run(n)
+    for cid, ref, i in zip(ids, refs, picks):
+        worst = _compare(b.get_log(cid, n), ref)
+        assert ref[:, 2].sum() > 0
+        print(f"chain {i}: {int(ref[:, 2].sum())} accepted, worst estep rel {worst:.2e}")
+    b.close()
+
+
 def test_batch_mixed_precision_error(fmc, oracle):
     """Mixed precision on the same systems: per-move estep within 1e-6
     relative of FP64 while the trajectories agree; loadings evolve
