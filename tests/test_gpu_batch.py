@@ -267,6 +267,51 @@ def test_bench_workload_chains_fp64(fmc, oracle):
     b.close()
 
 
+def test_serving_requests_fp64(fmc, oracle):
+    """Three generated serving requests (1-3 of CO2, N2, CH4; 894 to 9875
+    framework atoms; their own temperatures and pressures) in one FP64
+    batch: each makes the oracle's decisions over 1500 moves.  Energies use
+    the bar of the large-cell stress test: each component within 1e-10 of
+    max(its magnitude, 100 internal units), and estep within 1e-10 of its
+    components' summed magnitude, since estep can be the small difference
+    of components of ~10^3 internal units (here -0.003 kcal/mol from
+    -1549 + 1549), where summation order alone moves it by ~1e-12 kcal/mol."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd import synth
+    rng = np.random.default_rng(77)
+    ws = [synth.c4_request(rng, i) for i in range(8)]
+    picks = [3, 5, 7]
+    n = 1500
+    b = Batch(len(picks))
+    ids, refs = [], []
+    for i in picks:
+        w = ws[i]
+        A = w.system_arrays()
+        ctx = fmc.Context.from_arrays(A)
+        press = [p * 1e5 for p in w.press_bar]
+        cid = b.add_chain(ctx, w.temp, press, w.seed, delr=w.delr)
+        b.enable_log(cid, n)
+        ids.append(cid)
+        s = oracle.OracleSystem(A["imcon"], A["cell"], A["alpha"], A["kmax"], A["rcut"],
+                                A["delr"], A["ntpatm"], A["vdw"], A["fw"], A["guests"])
+        recs = oracle.OracleChain(s, w.temp, press, w.seed, delr=w.delr).run(n)
+        refs.append(np.array([[r.kind, r.iguest, r.accepted, r.nmol_after, r.estep,
+                               r.e1, r.e2, r.vdw] for r in recs]))
+    b.run(n)
+    for cid, ref, i in zip(ids, refs, picks):
+        log = b.get_log(cid, n)
+        assert np.array_equal(log[:, :4], ref[:, :4]), i
+        comp = np.abs(log[:, 5:8] - ref[:, 5:8]) / np.maximum(np.abs(ref[:, 5:8]), 100.0)
+        scale = np.abs(ref[:, 5:8]).sum(axis=1) / 418.4
+        rel = np.abs(log[:, 4] - ref[:, 4]) / np.maximum(np.maximum(np.abs(ref[:, 4]), scale), 1e-3)
+        print(f"request {i} ({len(ws[i].guests)} species, {ws[i].fw.n} atoms, "
+              f"{ws[i].temp:.1f} K): {int(ref[:, 2].sum())} accepted, worst component "
+              f"{comp.max():.2e}, worst estep {rel.max():.2e}")
+        assert comp.max() < 1e-10 and rel.max() < 1e-10, i
+    assert sum(r[:, 2].sum() for r in refs) > 100
+    b.close()
+
+
 def test_batch_mixed_precision_error(fmc, oracle):
     """Mixed precision on the same systems: per-move estep within 1e-6
     relative of FP64 while the trajectories agree; loadings evolve
