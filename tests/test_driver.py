@@ -24,6 +24,14 @@ def _workload(name):
     import numpy as np_
     if name == "c1":
         return synth.c1(), [], 2000, 1000
+    if name == "ar":
+        # a one-site uncharged guest (argon) beside CO2, in the FIELD as a
+        # one-atom molecule; the default move mix (a displacement also
+        # translates, so no move leaves the energy exactly unchanged)
+        rng = np_.random.default_rng(21)
+        fw = synth.make_framework(rng, (26.0, 26.0, 26.0), 260)
+        w = synth.Workload("ar", fw, ["co2", "ar"], [4.0, 8.0], 298.0, (8, 8, 8))
+        return w, [], 1200, 400
     # a porous two-guest system with translate/rotate-only moves
     rng = np_.random.default_rng(21)
     fw = synth.make_framework(rng, (26.0, 26.0, 26.0), 260)
@@ -56,7 +64,7 @@ def oracle_driver(oracle):
     return DRV_ORACLE
 
 
-@pytest.mark.parametrize("name", ["c1", "mix"])
+@pytest.mark.parametrize("name", ["c1", "mix", "ar"])
 def test_driver_matches_oracle_chain(oracle_driver, oracle, tmp_path, name):
     from fastmc_amd import inputs
     _write(tmp_path, name)
@@ -95,7 +103,7 @@ def test_output_reports_energy_and_loading(oracle_driver, tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["c1", "mix"])
+@pytest.mark.parametrize("name", ["c1", "mix", "ar"])
 def test_gpu_driver_matches_oracle_driver(oracle_driver, tmp_path, name):
     if not os.path.exists(DRV_GPU):
         pytest.fail(f"{DRV_GPU} missing: build first")
