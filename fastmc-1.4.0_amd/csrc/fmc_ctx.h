@@ -35,7 +35,10 @@ struct fmc_ctx {
   double* d_partial = nullptr;
   unsigned* d_counter = nullptr;
   double* d_out = nullptr;
+  // pinned, coherent: [0..2] a trial's sums, [3] (as an integer) the
+  // sequence number its kernel publishes once they are written
   double* h_out = nullptr;
+  unsigned long long trial_seq = 0;
   hipStream_t stream = nullptr;
   int nb_recip_max = 64, nb_real_max = 64;
   // pending trial

@@ -63,7 +63,8 @@ struct TrialParams {
   int nb_recip, nb_real;
   double* partial;  // [3][nb_recip+nb_real]
   unsigned* counter;
-  double* out;
+  double* out;            // pinned host memory: three sums, then the sequence word
+  unsigned long long seq;  // published in out[3] after the sums
 };
 
 __device__ __forceinline__ int unpack_l(int v) { return static_cast<signed char>(v & 0xff); }
@@ -119,6 +120,10 @@ __device__ void finish_trial(const TrialParams& P, double* red) {
     P.out[1] = v[1];
     P.out[2] = v[2];
     *P.counter = 0u;
+    // the sums reach host memory before the sequence word (release at
+    // system scope), so the host may read them before the kernel retires
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(P.out + 3), P.seq,
+                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -723,7 +728,9 @@ int fmc_ctx::upload_state() {
   if ((r = dalloc(&d_counter, 1))) return r;
   HIPCHECK(hipMemset(d_counter, 0, sizeof(unsigned)));
   if ((r = dalloc(&d_out, 8))) return r;
-  HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_out), 8 * sizeof(double)));
+  HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_out), 8 * sizeof(double),
+                         hipHostMallocCoherent));
+  std::memset(h_out, 0, 8 * sizeof(double));
   HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   return FMC_OK;
 }
@@ -737,6 +744,7 @@ void fmc_ctx::release() {
   for (int b = 0; b < 2; ++b) { f(d_sre[b]); f(d_sim[b]); }
   for (int t = 0; t < kMaxGuestTypes; ++t) { f(d_gx[t]); f(d_gy[t]); f(d_gz[t]); }
   f(d_rpart); f(d_epart); f(d_partial); f(d_counter); f(d_out);
+  if (stream) (void)hipStreamSynchronize(stream);
   if (h_out) (void)hipHostFree(h_out);
   if (stream) (void)hipStreamDestroy(stream);
   d_erc = d_vvv = d_fx = d_fy = d_fz = d_fq = d_ak = nullptr;
@@ -829,6 +837,9 @@ extern "C" int fmc_get_sk(const fmc_ctx* c, double* re, double* im) {
   // building the device state on first use changes nothing observable
   int r;
   if ((r = ensure_device(const_cast<fmc_ctx*>(c)))) return r;
+  // a trial returns before its kernel retires and a commit does not wait:
+  // drain the stream before copying on another one
+  HIPCHECK(hipStreamSynchronize(c->stream));
   HIPCHECK(hipMemcpy(re, c->d_sre[c->cur], c->nk * sizeof(double), hipMemcpyDeviceToHost));
   HIPCHECK(hipMemcpy(im, c->d_sim[c->cur], c->nk * sizeof(double), hipMemcpyDeviceToHost));
   return FMC_OK;
@@ -844,6 +855,27 @@ extern "C" int fmc_recip_full(fmc_ctx* c, int all_atoms, double* energy) {
   if ((r = ensure_device(c))) return r;
   c->pending = 0;
   return recip_full_impl(c, all_atoms != 0, all_atoms != 0, energy);
+}
+
+// Wait for a trial's sums: spin on the sequence word the last work-group
+// publishes, which lands before the kernel retires. The stream is queried
+// every few thousand reads, so a kernel that ends (or fails) without
+// publishing is reported instead of waited on.
+static int wait_trial(fmc_ctx* c, unsigned long long seq) {
+  auto* word = reinterpret_cast<unsigned long long*>(c->h_out + 3);
+  for (unsigned spin = 1;; ++spin) {
+    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return FMC_OK;
+    __builtin_ia32_pause();
+    if ((spin & 4095) == 0) {
+      const hipError_t q = hipStreamQuery(c->stream);
+      if (q == hipSuccess)
+        return __atomic_load_n(word, __ATOMIC_ACQUIRE) == seq ? FMC_OK : FMC_ERR_HIP;
+      if (q != hipErrorNotReady) {
+        std::fprintf(stderr, "fastmc_amd: HIP error %s in a trial\n", hipGetErrorString(q));
+        return FMC_ERR_HIP;
+      }
+    }
+  }
 }
 
 extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
@@ -903,12 +935,16 @@ extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
   P.partial = c->d_partial;
   P.counter = c->d_counter;
   // the last work-group writes the three sums straight into the pinned
-  // host buffer (device-addressable), visible once the stream has drained
+  // host buffer, then the trial's sequence number
   P.out = c->h_out;
+  P.seq = ++c->trial_seq;
   hipLaunchKernelGGL(trial_kernel, dim3(P.nb_recip + P.nb_real), dim3(kBS), 0,
                      c->stream, P);
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipStreamSynchronize(c->stream));
+  {
+    const int rw = wait_trial(c, P.seq);
+    if (rw) return rw;
+  }
   out[0] = c->h_out[0];
   out[1] = c->h_out[1];
   out[2] = c->h_out[2];
