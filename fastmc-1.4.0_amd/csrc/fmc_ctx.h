@@ -32,12 +32,12 @@ struct fmc_ctx {
   double2* d_rpart = nullptr;
   int recip_chunk = 1024, recip_nchunk_cap = 0;
   double* d_epart = nullptr;
-  double* d_partial = nullptr;
-  unsigned* d_counter = nullptr;
   double* d_out = nullptr;
-  // pinned, coherent: [0..2] a trial's sums, [3] (as an integer) the
-  // sequence number its kernel publishes once they are written
   double* h_out = nullptr;
+  // pinned, coherent host memory a trial's work-groups write: three
+  // partial sums each, then a sequence word once they are written
+  double* h_part = nullptr;               // [block][3]
+  unsigned long long* h_word = nullptr;   // [block]
   unsigned long long trial_seq = 0;
   hipStream_t stream = nullptr;
   int nb_recip_max = 64, nb_real_max = 64;

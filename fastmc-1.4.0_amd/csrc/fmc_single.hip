@@ -61,10 +61,9 @@ struct TrialParams {
   double rvolm2;   // 2*rvolm
   double engsic;   // self term of the moving molecule
   int nb_recip, nb_real;
-  double* partial;  // [3][nb_recip+nb_real]
-  unsigned* counter;
-  double* out;            // pinned host memory: three sums, then the sequence word
-  unsigned long long seq;  // published in out[3] after the sums
+  double* part;              // pinned host memory: [block][3] partial sums
+  unsigned long long* word;  // pinned host memory: [block] sequence words
+  unsigned long long seq;    // this trial's number, published after the sums
 };
 
 __device__ __forceinline__ int unpack_l(int v) { return static_cast<signed char>(v & 0xff); }
@@ -95,43 +94,10 @@ __device__ __forceinline__ bool load_atom(const DevSys& S, const GuestSeg* g,
   return false;
 }
 
-// Last-block-done reduction of [3][nb] partials into out (fixed order).
-__device__ void finish_trial(const TrialParams& P, double* red) {
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  const int nb = P.nb_recip + P.nb_real;
-  if (threadIdx.x == 0) last = (atomicAdd(P.counter, 1u) == static_cast<unsigned>(nb - 1));
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double v[3];
-  for (int c = 0; c < 3; ++c) {
-    double a = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-      a += __builtin_nontemporal_load(&P.partial[c * nb + b]);
-    v[c] = block_sum(a, red);
-  }
-  if (threadIdx.x == 0) {
-    double e1 = P.rvolm2 * kR4pie0 * v[0] / P.S.epsq;
-    if (P.kind == FMC_INSERT) e1 = e1 + P.engsic;
-    if (P.kind == FMC_DELETE) e1 = e1 - P.engsic;
-    P.out[0] = e1;
-    P.out[1] = v[1];
-    P.out[2] = v[2];
-    *P.counter = 0u;
-    // the sums reach host memory before the sequence word (release at
-    // system scope), so the host may read them before the kernel retires
-    __hip_atomic_store(reinterpret_cast<unsigned long long*>(P.out + 3), P.seq,
-                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-}
-
 __global__ void __launch_bounds__(kBS) trial_kernel(TrialParams P) {
   __shared__ double2 tab[2][kMaxSites][3][kTabMax];
   __shared__ double red[kBS / kWave];
   const int tid = threadIdx.x;
-  const int nb = P.nb_recip + P.nb_real;
   const bool use_old = (P.kind != FMC_INSERT);
   const bool use_new = (P.kind != FMC_DELETE);
   double e1 = 0.0, e2 = 0.0, ev = 0.0;
@@ -220,11 +186,15 @@ __global__ void __launch_bounds__(kBS) trial_kernel(TrialParams P) {
   e2 = block_sum(e2, red);
   ev = block_sum(ev, red);
   if (tid == 0) {
-    P.partial[0 * nb + blockIdx.x] = e1;
-    P.partial[1 * nb + blockIdx.x] = e2;
-    P.partial[2 * nb + blockIdx.x] = ev;
+    // the block's sums reach host memory before its sequence word (release
+    // at system scope); the host adds the blocks' sums in block order
+    double* o = P.part + 3 * blockIdx.x;
+    o[0] = e1;
+    o[1] = e2;
+    o[2] = ev;
+    __hip_atomic_store(P.word + blockIdx.x, P.seq, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  finish_trial(P, red);
 }
 
 // ---- full structure factor (ewald1, ewald.f:368-630) ----
@@ -724,13 +694,14 @@ int fmc_ctx::upload_state() {
   recip_nchunk_cap = static_cast<int>((nfw + cap_atoms) / recip_chunk) + 2;
   if ((r = dalloc(&d_rpart, static_cast<size_t>(recip_nchunk_cap) * nk))) return r;
   if ((r = dalloc(&d_epart, static_cast<size_t>(nk / 64 + 64)))) return r;
-  if ((r = dalloc(&d_partial, 3 * kMaxBlocks))) return r;
-  if ((r = dalloc(&d_counter, 1))) return r;
-  HIPCHECK(hipMemset(d_counter, 0, sizeof(unsigned)));
   if ((r = dalloc(&d_out, 8))) return r;
   HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_out), 8 * sizeof(double),
                          hipHostMallocCoherent));
-  std::memset(h_out, 0, 8 * sizeof(double));
+  HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_part), 3 * kMaxBlocks * sizeof(double),
+                         hipHostMallocCoherent));
+  HIPCHECK(hipHostMalloc(reinterpret_cast<void**>(&h_word),
+                         kMaxBlocks * sizeof(unsigned long long), hipHostMallocCoherent));
+  std::memset(h_word, 0, kMaxBlocks * sizeof(unsigned long long));
   HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   return FMC_OK;
 }
@@ -743,13 +714,17 @@ void fmc_ctx::release() {
   f(d_ftype); f(d_kv); f(d_ak);
   for (int b = 0; b < 2; ++b) { f(d_sre[b]); f(d_sim[b]); }
   for (int t = 0; t < kMaxGuestTypes; ++t) { f(d_gx[t]); f(d_gy[t]); f(d_gz[t]); }
-  f(d_rpart); f(d_epart); f(d_partial); f(d_counter); f(d_out);
+  f(d_rpart); f(d_epart); f(d_out);
   if (stream) (void)hipStreamSynchronize(stream);
   if (h_out) (void)hipHostFree(h_out);
+  if (h_part) (void)hipHostFree(h_part);
+  if (h_word) (void)hipHostFree(h_word);
   if (stream) (void)hipStreamDestroy(stream);
   d_erc = d_vvv = d_fx = d_fy = d_fz = d_fq = d_ak = nullptr;
   d_pairtab = d_ftype = d_kv = nullptr;
   h_out = nullptr;
+  h_part = nullptr;
+  h_word = nullptr;
   stream = nullptr;
 }
 
@@ -857,25 +832,33 @@ extern "C" int fmc_recip_full(fmc_ctx* c, int all_atoms, double* energy) {
   return recip_full_impl(c, all_atoms != 0, all_atoms != 0, energy);
 }
 
-// Wait for a trial's sums: spin on the sequence word the last work-group
-// publishes, which lands before the kernel retires. The stream is queried
-// every few thousand reads, so a kernel that ends (or fails) without
-// publishing is reported instead of waited on.
-static int wait_trial(fmc_ctx* c, unsigned long long seq) {
-  auto* word = reinterpret_cast<unsigned long long*>(c->h_out + 3);
-  for (unsigned spin = 1;; ++spin) {
-    if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) return FMC_OK;
-    __builtin_ia32_pause();
-    if ((spin & 4095) == 0) {
-      const hipError_t q = hipStreamQuery(c->stream);
-      if (q == hipSuccess)
-        return __atomic_load_n(word, __ATOMIC_ACQUIRE) == seq ? FMC_OK : FMC_ERR_HIP;
-      if (q != hipErrorNotReady) {
-        std::fprintf(stderr, "fastmc_amd: HIP error %s in a trial\n", hipGetErrorString(q));
-        return FMC_ERR_HIP;
+// Wait for a trial's sums: spin on the sequence word each work-group
+// publishes after its partial sums, which land before the kernel retires,
+// then add the partials in block order. The stream is queried every few
+// thousand reads, so a kernel that ends (or fails) without publishing is
+// reported instead of waited on.
+static int wait_trial(fmc_ctx* c, unsigned long long seq, int nb, double v[3]) {
+  for (int b = 0; b < nb; ++b) {
+    unsigned long long* word = c->h_word + b;
+    for (unsigned spin = 1; __atomic_load_n(word, __ATOMIC_ACQUIRE) != seq; ++spin) {
+      __builtin_ia32_pause();
+      if ((spin & 4095) == 0) {
+        const hipError_t q = hipStreamQuery(c->stream);
+        if (q == hipSuccess) {
+          if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == seq) break;
+          return FMC_ERR_HIP;
+        }
+        if (q != hipErrorNotReady) {
+          std::fprintf(stderr, "fastmc_amd: HIP error %s in a trial\n", hipGetErrorString(q));
+          return FMC_ERR_HIP;
+        }
       }
     }
   }
+  v[0] = v[1] = v[2] = 0.0;
+  for (int b = 0; b < nb; ++b)
+    for (int k = 0; k < 3; ++k) v[k] += c->h_part[3 * b + k];
+  return FMC_OK;
 }
 
 extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
@@ -932,22 +915,26 @@ extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
   int ntot = P.S.nfw;
   for (int t = 0; t < P.ngt; ++t) ntot += P.g[t].natoms;
   P.nb_real = std::max(1, std::min(c->nb_real_max, (ntot + kBS - 1) / kBS));
-  P.partial = c->d_partial;
-  P.counter = c->d_counter;
-  // the last work-group writes the three sums straight into the pinned
-  // host buffer, then the trial's sequence number
-  P.out = c->h_out;
+  if (P.nb_recip + P.nb_real > kMaxBlocks) return FMC_ERR_ARG;  // partial-sum slots
+  // every work-group writes its sums straight into pinned host memory,
+  // then the trial's sequence number
+  P.part = c->h_part;
+  P.word = c->h_word;
   P.seq = ++c->trial_seq;
   hipLaunchKernelGGL(trial_kernel, dim3(P.nb_recip + P.nb_real), dim3(kBS), 0,
                      c->stream, P);
   HIPCHECK(hipGetLastError());
+  double v[3];
   {
-    const int rw = wait_trial(c, P.seq);
+    const int rw = wait_trial(c, P.seq, P.nb_recip + P.nb_real, v);
     if (rw) return rw;
   }
-  out[0] = c->h_out[0];
-  out[1] = c->h_out[1];
-  out[2] = c->h_out[2];
+  double e1 = P.rvolm2 * kR4pie0 * v[0] / P.S.epsq;
+  if (kind == FMC_INSERT) e1 = e1 + P.engsic;
+  if (kind == FMC_DELETE) e1 = e1 - P.engsic;
+  out[0] = e1;
+  out[1] = v[1];
+  out[2] = v[2];
   c->pending = kind;
   c->pend_gt = gt;
   c->pend_mol = imol;
