@@ -28,6 +28,7 @@
 namespace fmc {
 
 constexpr int kBS = 256;       // threads per block, 4 waves
+constexpr int kTrialBS = 256;  // threads per trial work-group
 constexpr int kTabMax = 33;    // phase table entries per dimension (kmax<=32)
 constexpr int kMaxBlocks = 512;
 
@@ -94,9 +95,9 @@ __device__ __forceinline__ bool load_atom(const DevSys& S, const GuestSeg* g,
   return false;
 }
 
-__global__ void __launch_bounds__(kBS) trial_kernel(TrialParams P) {
+__global__ void __launch_bounds__(kTrialBS) trial_kernel(TrialParams P) {
   __shared__ double2 tab[2][kMaxSites][3][kTabMax];
-  __shared__ double red[kBS / kWave];
+  __shared__ double2 red[kTrialBS / kWave];
   const int tid = threadIdx.x;
   const bool use_old = (P.kind != FMC_INSERT);
   const bool use_new = (P.kind != FMC_DELETE);
@@ -115,7 +116,7 @@ __global__ void __launch_bounds__(kBS) trial_kernel(TrialParams P) {
       phase_table(f, P.kmax[d], &tab[set][s][d][0]);
     }
     __syncthreads();
-    for (int k = blockIdx.x * kBS + tid; k < P.nk; k += P.nb_recip * kBS) {
+    for (int k = blockIdx.x * kTrialBS + tid; k < P.nk; k += P.nb_recip * kTrialBS) {
       const int v = P.kv[k];
       const int l = unpack_l(v), m = unpack_m(v), n = unpack_n(v);
       const int am = m < 0 ? -m : m, an = n < 0 ? -n : n;
@@ -156,7 +157,7 @@ __global__ void __launch_bounds__(kBS) trial_kernel(TrialParams P) {
     const int rb = blockIdx.x - P.nb_recip;
     int ntot = P.S.nfw;
     for (int k = 0; k < P.ngt; ++k) ntot += P.g[k].natoms;
-    for (int a = rb * kBS + tid; a < ntot; a += P.nb_real * kBS) {
+    for (int a = rb * kTrialBS + tid; a < ntot; a += P.nb_real * kTrialBS) {
       double x, y, z, q;
       int t, gt, gi;
       load_atom(P.S, P.g, P.ngt, a, x, y, z, q, t, gt, gi);
@@ -182,10 +183,22 @@ __global__ void __launch_bounds__(kBS) trial_kernel(TrialParams P) {
       }
     }
   }
-  e1 = block_sum(e1, red);
-  e2 = block_sum(e2, red);
-  ev = block_sum(ev, red);
+  // One reduction pass: a reciprocal block carries e1 only, a real-space
+  // block e2 and ev (the other sums are zero). Waves are added in order.
+  const bool rec = static_cast<int>(blockIdx.x) < P.nb_recip;
+  const double ra = wave_sum(rec ? e1 : e2);
+  const double rb = rec ? 0.0 : wave_sum(ev);
+  if (tid % kWave == 0) red[tid / kWave] = make_double2(ra, rb);
+  __syncthreads();
   if (tid == 0) {
+    double a = 0.0, b = 0.0;
+    for (int w = 0; w < kTrialBS / kWave; ++w) {
+      a += red[w].x;
+      b += red[w].y;
+    }
+    e1 = rec ? a : 0.0;
+    e2 = rec ? 0.0 : a;
+    ev = rec ? 0.0 : b;
     // the block's sums reach host memory before its sequence word (release
     // at system scope); the host adds the blocks' sums in block order
     double* o = P.part + 3 * blockIdx.x;
@@ -911,17 +924,17 @@ extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
   }
   P.rvolm2 = 2.0 * (2.0 * kPi / c->sys.volm);
   P.engsic = tp.engsic;
-  P.nb_recip = std::max(1, std::min(c->nb_recip_max, (c->nk + kBS - 1) / kBS));
+  P.nb_recip = std::max(1, std::min(c->nb_recip_max, (c->nk + kTrialBS - 1) / kTrialBS));
   int ntot = P.S.nfw;
   for (int t = 0; t < P.ngt; ++t) ntot += P.g[t].natoms;
-  P.nb_real = std::max(1, std::min(c->nb_real_max, (ntot + kBS - 1) / kBS));
+  P.nb_real = std::max(1, std::min(c->nb_real_max, (ntot + kTrialBS - 1) / kTrialBS));
   if (P.nb_recip + P.nb_real > kMaxBlocks) return FMC_ERR_ARG;  // partial-sum slots
   // every work-group writes its sums straight into pinned host memory,
   // then the trial's sequence number
   P.part = c->h_part;
   P.word = c->h_word;
   P.seq = ++c->trial_seq;
-  hipLaunchKernelGGL(trial_kernel, dim3(P.nb_recip + P.nb_real), dim3(kBS), 0,
+  hipLaunchKernelGGL(trial_kernel, dim3(P.nb_recip + P.nb_real), dim3(kTrialBS), 0,
                      c->stream, P);
   HIPCHECK(hipGetLastError());
   double v[3];
