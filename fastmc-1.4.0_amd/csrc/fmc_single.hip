@@ -95,6 +95,107 @@ __device__ __forceinline__ bool load_atom(const DevSys& S, const GuestSeg* g,
   return false;
 }
 
+// One atom against the trial's NS old and/or new sites, in the order of the
+// site loop (per site: old, then new). Every table load is issued whatever
+// the distance: a lane beyond the cutoff reads a clamped, valid entry (and a
+// pair without a vdW table reads the erc table) and its terms are dropped,
+// so the loads of all positions overlap instead of forming one dependent
+// chain per position. Sums and their order are those of ewald.f:670-684 and
+// vdw_terms.f:476-489 per pair.
+template <int NS, bool OLD, bool NEW>
+__device__ __forceinline__ void trial_atom(const TrialParams& P, int s0, double x, double y,
+                                           double z, double q, int t, double& e2, double& ev) {
+  const DevSys& S = P.S;
+  const int imax = S.mxgrid - 2;
+  const bool disp = P.kind == FMC_DISPLACE;
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    const int s = s0 + j;
+    const double ce = P.q[s] / S.epsq * kR4pie0;
+    const double chgprd = ce * q;
+    const bool qon = fabs(chgprd) > 1e-10;
+    const int k = S.pairtab[P.t[s] * S.ntpatm + t];
+    const double* vrow = k >= 0 ? S.vvv + static_cast<size_t>(k) * (S.mxgrid + 1) : S.erc;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if ((h == 0 && !OLD) || (h == 1 && !NEW)) continue;
+      double dx = (h ? P.nx[s] : P.ox[s]) - x;
+      double dy = (h ? P.ny[s] : P.oy[s]) - y;
+      double dz = (h ? P.nz[s] : P.oz[s]) - z;
+      min_image(S, dx, dy, dz);
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      const bool in = S.rcut2 > rsq;
+      const double r = sqrt(rsq);
+      const double vc = interp3(S.erc, r, S.rdrewd, imax);
+      const double vv = interp3(vrow, r, S.rdlrpot, imax);
+      const double a2 = (in && qon) ? vc * chgprd : 0.0;
+      const double av = (in && k >= 0) ? vv : 0.0;
+      if (h == 0 && disp) {
+        e2 -= a2;
+        ev -= av;
+      } else {
+        e2 += a2;
+        ev += av;
+      }
+    }
+  }
+}
+
+// Real space over every atom (ewald2 + srfrce, ewald.f:632-694,
+// vdw_terms.f:398-498) for a molecule of NS sites; pass 1 of a displacement
+// with the minus sign.
+// Real space for molecules of more than four sites: one position at a time
+// (unrolling them all would not fit the registers).
+__device__ __forceinline__ void trial_real_any(const TrialParams& P, int rb, bool use_old,
+                                               bool use_new, double& e2, double& ev) {
+  int ntot = P.S.nfw;
+  for (int k = 0; k < P.ngt; ++k) ntot += P.g[k].natoms;
+  for (int a = rb * kTrialBS + static_cast<int>(threadIdx.x); a < ntot;
+       a += P.nb_real * kTrialBS) {
+    double x, y, z, q;
+    int t, gt, gi;
+    load_atom(P.S, P.g, P.ngt, a, x, y, z, q, t, gt, gi);
+    if (gt == P.own_t && gi >= P.own_begin && gi < P.own_begin + P.ns &&
+        P.kind != FMC_INSERT)
+      continue;
+    for (int s = 0; s < P.ns; ++s) {
+      const double ce = P.q[s] / P.S.epsq * kR4pie0;
+      if (use_old) {
+        double a2 = 0.0, av = 0.0;
+        pair_fp64(P.S, ce, P.t[s], P.ox[s] - x, P.oy[s] - y, P.oz[s] - z, q, t, a2, av);
+        if (P.kind == FMC_DISPLACE) { e2 -= a2; ev -= av; }
+        else { e2 += a2; ev += av; }
+      }
+      if (use_new) {
+        double a2 = 0.0, av = 0.0;
+        pair_fp64(P.S, ce, P.t[s], P.nx[s] - x, P.ny[s] - y, P.nz[s] - z, q, t, a2, av);
+        e2 += a2;
+        ev += av;
+      }
+    }
+  }
+}
+
+template <int NS, bool OLD, bool NEW>
+__device__ __forceinline__ void trial_real(const TrialParams& P, int rb, double& e2,
+                                           double& ev) {
+  int ntot = P.S.nfw;
+  for (int k = 0; k < P.ngt; ++k) ntot += P.g[k].natoms;
+  for (int a = rb * kTrialBS + static_cast<int>(threadIdx.x); a < ntot;
+       a += P.nb_real * kTrialBS) {
+    double x, y, z, q;
+    int t, gt, gi;
+    load_atom(P.S, P.g, P.ngt, a, x, y, z, q, t, gt, gi);
+    if (gt == P.own_t && gi >= P.own_begin && gi < P.own_begin + NS && P.kind != FMC_INSERT)
+      continue;
+    trial_atom<NS, OLD, NEW>(P, 0, x, y, z, q, t, e2, ev);
+  }
+}
+
+// NS 1-4: the molecule's size, with the unrolled real-space form; NS 0:
+// any size, one position at a time. One kernel per form keeps each one's
+// registers to its own needs.
+template <int NS>
 __global__ void __launch_bounds__(kTrialBS) trial_kernel(TrialParams P) {
   __shared__ double2 tab[2][kMaxSites][3][kTabMax];
   __shared__ double2 red[kTrialBS / kWave];
@@ -152,35 +253,17 @@ __global__ void __launch_bounds__(kTrialBS) trial_kernel(TrialParams P) {
       e1 += P.ak[k] * (c * c + s * s - cold * cold - sold * sold);
     }
   } else {
-    // ---- real space over every atom (ewald2 + srfrce, ewald.f:632-694,
-    //      vdw_terms.f:398-498), pass 1 with minus sign for displace ----
+    // ---- real space: the move's halves (old sites for a delete, new for
+    //      an insert, both for a displacement) select the unrolled form ----
     const int rb = blockIdx.x - P.nb_recip;
-    int ntot = P.S.nfw;
-    for (int k = 0; k < P.ngt; ++k) ntot += P.g[k].natoms;
-    for (int a = rb * kTrialBS + tid; a < ntot; a += P.nb_real * kTrialBS) {
-      double x, y, z, q;
-      int t, gt, gi;
-      load_atom(P.S, P.g, P.ngt, a, x, y, z, q, t, gt, gi);
-      if (gt == P.own_t && gi >= P.own_begin && gi < P.own_begin + P.ns &&
-          P.kind != FMC_INSERT)
-        continue;
-      for (int s = 0; s < P.ns; ++s) {
-        const double ce = P.q[s] / P.S.epsq * kR4pie0;
-        if (use_old) {
-          double a2 = 0.0, av = 0.0;
-          pair_fp64(P.S, ce, P.t[s], P.ox[s] - x, P.oy[s] - y, P.oz[s] - z, q,
-                    t, a2, av);
-          if (P.kind == FMC_DISPLACE) { e2 -= a2; ev -= av; }
-          else { e2 += a2; ev += av; }
-        }
-        if (use_new) {
-          double a2 = 0.0, av = 0.0;
-          pair_fp64(P.S, ce, P.t[s], P.nx[s] - x, P.ny[s] - y, P.nz[s] - z, q,
-                    t, a2, av);
-          e2 += a2;
-          ev += av;
-        }
-      }
+    if constexpr (NS == 0) {
+      trial_real_any(P, rb, use_old, use_new, e2, ev);
+    } else if (use_old && use_new) {
+      trial_real<NS, true, true>(P, rb, e2, ev);
+    } else if (use_old) {
+      trial_real<NS, true, false>(P, rb, e2, ev);
+    } else {
+      trial_real<NS, false, true>(P, rb, e2, ev);
     }
   }
   // One reduction pass: a reciprocal block carries e1 only, a real-space
@@ -934,7 +1017,10 @@ extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
   P.part = c->h_part;
   P.word = c->h_word;
   P.seq = ++c->trial_seq;
-  hipLaunchKernelGGL(trial_kernel, dim3(P.nb_recip + P.nb_real), dim3(kTrialBS), 0,
+  static void (*const kTrialForms[5])(TrialParams) = {trial_kernel<0>, trial_kernel<1>,
+                                                      trial_kernel<2>, trial_kernel<3>,
+                                                      trial_kernel<4>};
+  hipLaunchKernelGGL(kTrialForms[ns <= 4 ? ns : 0], dim3(P.nb_recip + P.nb_real), dim3(kTrialBS), 0,
                      c->stream, P);
   HIPCHECK(hipGetLastError());
   double v[3];

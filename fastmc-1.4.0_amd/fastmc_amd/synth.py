@@ -45,6 +45,8 @@ GUEST_SITES = {
     "Ar": (39.948, 0.2381, 3.405),   # argon, one uncharged LJ site
     "Ce": (12.0107, 0.0660, 3.50),   # ethane carbon
     "He": (1.00794, 0.0300, 2.50),   # ethane hydrogen
+    "Co": (12.0107, 0.0526, 3.385),  # carbon monoxide carbon
+    "Oo": (15.9994, 0.1610, 2.955),  # carbon monoxide oxygen
 }
 
 
@@ -77,6 +79,10 @@ def guest_template(name: str):
                 ("Mw", 0.0, -1.04, 0.0, 0.0, 0.15)]
     if name == "ar":
         return [("Ar", 39.948, 0.0, 0.0, 0.0, 0.0)]
+    if name == "co":
+        # two-site carbon monoxide, C-O 1.128 A, point charges -/+0.2 e
+        d = 1.128 / 2
+        return [("Co", 12.0107, -0.2, 0.0, 0.0, d), ("Oo", 15.9994, 0.2, 0.0, 0.0, -d)]
     if name == "c2h6":
         # rigid staggered ethane, all 8 atoms: C-C 1.54 A, C-H 1.09 A,
         # tetrahedral angles, q_C = -0.18 e, q_H = +0.06 e
@@ -96,7 +102,8 @@ def guest_template(name: str):
 
 GUEST_TITLE = {"co2": "CO2 rigid 3-site", "n2": "N2 3-site with centre charge",
                "ch4": "CH4 rigid 5-site", "h2o": "H2O rigid 4-site",
-               "ar": "Ar single LJ site", "c2h6": "C2H6 rigid 8-site"}
+               "ar": "Ar single LJ site", "c2h6": "C2H6 rigid 8-site",
+               "co": "CO rigid 2-site"}
 
 
 @dataclasses.dataclass

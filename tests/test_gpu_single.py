@@ -111,6 +111,23 @@ def test_trial_parity(fmc, oracle, name, gtypes, n):
     assert abs(e_all - orc.recip_full(1)) < 1e-10 * abs(e_all)
 
 
+@pytest.mark.parametrize("guests", [("ar", "co", "h2o"), ("ch4", "c2h6", "co2")])
+def test_trial_parity_molecule_sizes(fmc, oracle, guests):
+    """Molecules of 1, 2, 4 sites (and 3) take the trial kernel's unrolled
+    forms; 5 and 8 sites the form that goes one position at a time. Random
+    inserts, deletes and displacements against the oracle, with commits."""
+    from fastmc_amd import synth
+    rng = np.random.default_rng(17)
+    fw = synth.make_framework(rng, (25.8, 25.8, 25.8), 424)
+    w = synth.Workload("sizes", fw, list(guests), [1.0] * len(guests), 298.0, (8, 8, 8))
+    A = w.system_arrays()
+    ctx, orc, worst = _random_moves(fmc, oracle, A, 150, seed=11, gtypes=len(guests))
+    print(f"{guests}: worst component rel diff {worst:.3e}")
+    for g in range(len(guests)):
+        assert orc.nmol(g) > 0   # every size was inserted and moved
+        assert ctx.nmol(g) == orc.nmol(g)
+
+
 def test_triclinic_trial_parity(fmc, oracle, restart_example):
     """imcon 3 (the reference's calf-20 example) with 90 CO2 loaded."""
     ex = restart_example
