@@ -803,6 +803,9 @@ int fmc_ctx::upload_state() {
 }
 
 void fmc_ctx::release() {
+  // a trial returns before its kernel retires and a commit does not wait:
+  // drain the stream before any buffer it uses is freed
+  if (stream) (void)hipStreamSynchronize(stream);
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
@@ -811,7 +814,6 @@ void fmc_ctx::release() {
   for (int b = 0; b < 2; ++b) { f(d_sre[b]); f(d_sim[b]); }
   for (int t = 0; t < kMaxGuestTypes; ++t) { f(d_gx[t]); f(d_gy[t]); f(d_gz[t]); }
   f(d_rpart); f(d_epart); f(d_out);
-  if (stream) (void)hipStreamSynchronize(stream);
   if (h_out) (void)hipHostFree(h_out);
   if (h_part) (void)hipHostFree(h_part);
   if (h_word) (void)hipHostFree(h_word);
