@@ -115,7 +115,10 @@ int fmc_recip_full(fmc_ctx *ctx, int all_atoms, double *energy);
  * out[0] = ewld1eng (reciprocal delta incl. the +/- self term of
  *          ewald.f:335-362), out[1] = ewld2sum, out[2] = vdwsum, with the
  * reference's signs, so the driver forms estep exactly as gcmc.f:946,1112,
- * 1328.  Blocking; the move is pending until fmc_commit or the next trial.
+ * 1328.  Blocking until out is filled; the move is pending until fmc_commit
+ * or the next trial.  It returns once the sums have reached host memory,
+ * which can be before its kernel has retired: later calls on the context
+ * are ordered after it, and fmc_get_sk and fmc_ctx_destroy drain it.
  */
 int fmc_trial(fmc_ctx *ctx, int kind, int gtype, int imol, const double *x,
               const double *y, const double *z, double *out);
