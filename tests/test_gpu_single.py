@@ -128,6 +128,39 @@ def test_trial_parity_molecule_sizes(fmc, oracle, guests):
         assert ctx.nmol(g) == orc.nmol(g)
 
 
+def test_trial_sums_are_each_calls_own(fmc, oracle):
+    """A trial returns once its work-groups' sums have reached host memory,
+    not when its stream drains. Two different trials alternated without a
+    commit must each read their own sums every time, bit for bit, never a
+    previous trial's."""
+    from fastmc_amd import synth
+    A = synth.c2().system_arrays()
+    ctx = fmc.Context.from_arrays(A)
+    orc = oracle.OracleSystem(A["imcon"], A["cell"], A["alpha"], A["kmax"],
+                              A["rcut"], A["delr"], A["ntpatm"], A["vdw"],
+                              A["fw"], A["guests"])
+    tpl = A["guests"][0]
+    p = [np.asarray(tpl[k]) + 7.0 for k in "xyz"]
+    r = [np.asarray(tpl[k]) + 19.0 for k in "xyz"]
+    a = ctx.trial(1, 0, -1, *p)
+    b = ctx.trial(1, 0, -1, *r)
+    assert _close(a, orc.trial(1, 0, -1, *p))
+    assert _close(b, orc.trial(1, 0, -1, *r))
+    assert not np.array_equal(a, b)
+    for _ in range(50):
+        assert np.array_equal(ctx.trial(1, 0, -1, *p), a)
+        assert np.array_equal(ctx.trial(1, 0, -1, *r), b)
+    # the last trial (b) commits; S(k) read afterwards includes it
+    ctx.commit()
+    orc.commit()
+    re, im = ctx.sk()
+    ctx.recip_full(1)
+    re2, im2 = ctx.sk()
+    scale = max(np.abs(re2).max(), 1.0)
+    assert np.abs(re - re2).max() < 1e-9 * scale
+    assert np.abs(im - im2).max() < 1e-9 * scale
+
+
 def test_triclinic_trial_parity(fmc, oracle, restart_example):
     """imcon 3 (the reference's calf-20 example) with 90 CO2 loaded."""
     ex = restart_example
