@@ -19,9 +19,11 @@
 //   * the serial part runs wave-wide (RNG draws and proposal: every lane
 //     the same scalar steps, lane i draw i, lane s site s) or on one lane
 //     (acceptance and bookkeeping).
-// Precision FP64 reproduces the oracle's accept/reject stream; MIXED does
-// the per-site phase products and the pair interpolation in FP32 with FP64
-// coordinates, S(k), energies and acceptance.
+// Precision FP64 reproduces the oracle's accept/reject stream; MIXED keeps
+// the per-position phase-factor products q X[l] Y[m], Z[n] and the trial's
+// dS(k) in FP32 (a displacement's as the change of each site's term, formed
+// in FP64 before rounding), with FP64 coordinates, S(k), real-space pair
+// terms, energies and acceptance.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -94,8 +96,6 @@ struct BSys {
   const Rec* grec;
   const int* grec_t;
   int gcap;
-  const float* erc32;
-  const float* vvv32;
   SysQ q;
 };
 
@@ -331,42 +331,6 @@ __device__ __forceinline__ int cell_of(const SysQ& Z, double x, double y,
     idx[d] = i >= Z.gdim[d] ? Z.gdim[d] - 1 : (i < 0 ? 0 : i);
   }
   return (idx[0] * Z.gdim[1] + idx[1]) * Z.gdim[2] + idx[2];
-}
-
-// Mixed-precision pair: FP64 separation and minimum image, FP32 distance
-// and table interpolation.
-// vk = the pair's vdW table (pairtab), -1 for none.
-__device__ __forceinline__ void pair_mixed(const BSys& Y, float chgea, int vk,
-                                           double dx, double dy, double dz,
-                                           float qj, float& e2, float& ev) {
-  min_image(Y.S, dx, dy, dz);
-  const float fx = static_cast<float>(dx), fy = static_cast<float>(dy),
-              fz = static_cast<float>(dz);
-  const float rsq = fx * fx + fy * fy + fz * fz;
-  if (!(static_cast<float>(Y.S.rcut2) > rsq)) return;
-  const float r = sqrtf(rsq);
-  const int imax = Y.S.mxgrid - 2;
-  const float chgprd = chgea * qj;
-  {
-    const float u = r * static_cast<float>(Y.S.rdrewd);
-    int l = static_cast<int>(u);
-    l = l > imax ? imax : l;
-    const float p = u - static_cast<float>(l);
-    const float v0 = gload(Y.erc32 + l), v1 = gload(Y.erc32 + l + 1),
-                v2 = gload(Y.erc32 + l + 2);
-    const float t1 = v0 + (v1 - v0) * p, t2 = v1 + (v2 - v1) * (p - 1.0f);
-    if (fabsf(chgprd) > 1e-10f) e2 += (t1 + (t2 - t1) * p * 0.5f) * chgprd;
-  }
-  if (vk >= 0) {
-    const float* tab = Y.vvv32 + static_cast<size_t>(vk) * (Y.S.mxgrid + 1);
-    const float u = r * static_cast<float>(Y.S.rdlrpot);
-    int l = static_cast<int>(u);
-    l = l > imax ? imax : l;
-    const float p = u - static_cast<float>(l);
-    const float v0 = gload(tab + l), v1 = gload(tab + l + 1), v2 = gload(tab + l + 2);
-    const float t1 = v0 + (v1 - v0) * p, t2 = v1 + (v2 - v1) * (p - 1.0f);
-    ev += t1 + (t2 - t1) * p * 0.5f;
-  }
 }
 
 template <typename T>
@@ -946,34 +910,69 @@ __device__ __forceinline__ void recompute_finish(const SysQ& Z, BChain& C, doubl
 // and candidate-list start for real space.  Positions are independent, so
 // no work-group barrier is needed inside; the caller's next barrier
 // publishes the tables.
-template <typename T>
+//
+// A displacement in DELTA form (mixed precision) sums, per site, the change
+// of q e^{ik.r} rather than the new term minus the old one:
+//   q (P_n Z_n - P_o Z_o) = q P_o (Z_n - Z_o) + q (P_n - P_o) Z_n,
+// with P = X[l] Y[m].  The old position's rows are (q P_o, Z_n - Z_o), the
+// new position's (q (P_n - P_o), Z_n).  The differences are formed from
+// FP64 phase factors in FP64 and only then rounded to T, so their rounding
+// error is relative to the change of the term, not to the term: the FP32
+// difference of two nearly equal phase factors would lose the digits that
+// the framework's large |S(k)| amplifies in 2 a_k S.dS.  Each position's
+// wave computes the partner position's phase factors it needs into tabp.
+template <typename T, bool DELTA>
 __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveDesc& D,
                                              int wave, int nw, int lane,
-                                             Cplx<double> (*tab)[3][kBTab], Cplx<T>* xy_l,
+                                             Cplx<double> (*tab)[3][kBTab],
+                                             Cplx<double> (*tabp)[3][kBTab], Cplx<T>* xy_l,
                                              Cplx<T>* zt, int nlm) {
   const int np = D.np, ig = D.ig, kind = D.kind;
   const int ntp = Y.S.ntpatm;
   const int w2 = 2 * Z.kmax[1] + 1, k3 = Z.kmax[2], nz = 2 * k3 + 1;
   const float rw2 = 1.0f / w2;
+  const bool delta = DELTA && kind == FMC_DISPLACE;
   for (int p = wave; p < np; p += nw) {
     const int set = D.pset[p], s = D.psite[p];
     const double x = set ? D.nx[s] : D.ox[s];
     const double y = set ? D.ny[s] : D.oy[s];
     const double z = set ? D.nz[s] : D.oz[s];
     const double q = Z.gq[ig][s];
-    const double pq = set ? q : -q;
+    const double pq = (set || delta) ? q : -q;
     const int ts = Z.gtype[ig][s];
-    if (lane < 3 * kBTab) {
-      const int d = lane / kBTab, l = lane - d * kBTab;
-      if (l <= Z.kmax[d]) {
-        double f = Z.rcell[d] * x + Z.rcell[3 + d] * y + Z.rcell[6 + d] * z;
-        f = f - floor(f);
-        double ph = l * f;
-        ph = ph - rint(ph);
-        double sv, cv;
-        sincospi(2.0 * ph, &sv, &cv);
-        tab[p][d][l] = {cv, sv};
+    // one sincospi per lane: first the position's own phase factors (kmax+1
+    // per dimension, packed), then, in delta form, the partner position's
+    // that the rows below need (the new Z for the old position, the old X
+    // and Y for the new one); both fit one pass of the wave while the
+    // entries number at most 64
+    const int k0 = Z.kmax[0] + 1, k1 = Z.kmax[1] + 1, k2 = Z.kmax[2] + 1;
+    const int nown = k0 + k1 + k2;
+    const int d0 = set ? 0 : 2;   // the partner's first dimension
+    const int npart = delta ? (set ? k0 + k1 : k2) : 0;
+    for (int j = lane; j < nown + npart; j += kWave) {
+      const bool own = j < nown;
+      int d, l;
+      if (own) {
+        d = j < k0 ? 0 : (j < k0 + k1 ? 1 : 2);
+        l = j - (d == 0 ? 0 : (d == 1 ? k0 : k0 + k1));
+      } else {
+        const int jj = j - nown;
+        d = d0 + (d0 == 0 && jj >= k0 ? 1 : 0);
+        l = jj - (d == 1 ? k0 : 0);
       }
+      const double px = own ? x : (set ? D.ox[s] : D.nx[s]);
+      const double py = own ? y : (set ? D.oy[s] : D.ny[s]);
+      const double pz = own ? z : (set ? D.oz[s] : D.nz[s]);
+      double f = Z.rcell[d] * px + Z.rcell[3 + d] * py + Z.rcell[6 + d] * pz;
+      f = f - floor(f);
+      double ph = l * f;
+      ph = ph - rint(ph);
+      double sv, cv;
+      sincospi(2.0 * ph, &sv, &cv);
+      if (!DELTA || own)
+        tab[p][d][l] = {cv, sv};
+      else
+        tabp[p][d][l] = {cv, sv};
     }
     if (lane == 0) {
       D.plo[p] = cell_of(Z, x, y, z) * Y.gcap;
@@ -992,12 +991,15 @@ __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveD
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // X[l] * Y[m] for every (l, m) (m < 0 conjugates Y)
+    // X[l] * Y[m] for every (l, m) (m < 0 conjugates Y); in delta form the
+    // new position's row is X_n Y_n - X_o Y_o
     Cplx<T>* xr = xy_l + p * nlm;
+    const bool dxy = delta && set;
     for (int r = lane; r < nlm; r += kWave) {
       const int l = div_rc(r, w2, rw2), m = r - l * w2 - Z.kmax[1];
+      const int am = m < 0 ? -m : m;
       const Cplx<double> a = tab[p][0][l];
-      const Cplx<double> b = tab[p][1][m < 0 ? -m : m];
+      const Cplx<double> b = tab[p][1][am];
       double ox, oy;
       if (m >= 0) {
         ox = a.x * b.x - a.y * b.y;
@@ -1006,11 +1008,29 @@ __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveD
         ox = a.x * b.x + a.y * b.y;
         oy = a.y * b.x - b.y * a.x;
       }
+      if (dxy) {
+        const Cplx<double> c = tabp[p][0][l];
+        const Cplx<double> e = tabp[p][1][am];
+        if (m >= 0) {
+          ox -= c.x * e.x - c.y * e.y;
+          oy -= c.y * e.x + e.y * c.x;
+        } else {
+          ox -= c.x * e.x + c.y * e.y;
+          oy -= c.y * e.x - e.y * c.x;
+        }
+      }
       xr[r] = {static_cast<T>(pq * ox), static_cast<T>(pq * oy)};
     }
+    // signed Z[n]; in delta form the old position's row is Z_n - Z_o
+    const bool dz = delta && !set;
     for (int i = lane; i < nz; i += kWave) {
       const int n = i - k3;
-      const Cplx<double> c = tab[p][2][n < 0 ? -n : n];
+      const int an = n < 0 ? -n : n;
+      Cplx<double> c = tab[p][2][an];
+      if (dz) {
+        const Cplx<double> e = tabp[p][2][an];
+        c = {e.x - c.x, e.y - c.y};
+      }
       zt[p * kZRow + i] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
     }
   }
@@ -1034,6 +1054,8 @@ __global__ void __launch_bounds__(BT)
                  long long budget_ticks) {
   using T = typename std::conditional<MIXED, float, double>::type;
   __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
+  // a displacement's partner phase factors (mixed precision: delta form)
+  __shared__ Cplx<double> tabp[MIXED ? 2 * kMaxSites : 1][3][kBTab];
   __shared__ Cplx<T> zt[2 * kMaxSites * kZRow];           // Z_p[n], signed n
   // Move descriptors and duni tables, two of each: while the waves work on
   // one move, the last wave proposes the next one on copies of the chain
@@ -1167,7 +1189,8 @@ __global__ void __launch_bounds__(BT)
       // the positions' tables: built by every wave for a fresh proposal;
       // a prepared one had them built while the previous move was decided
       if (fresh) {
-        build_tables<T>(Z, Y, D, tid / kWave, BT / kWave, tid % kWave, tab, xy_l, zt, nlm);
+        build_tables<T, MIXED>(Z, Y, D, tid / kWave, BT / kWave, tid % kWave, tab, tabp, xy_l,
+                               zt, nlm);
         __syncthreads();
       }
       const int ntp = Y.S.ntpatm;
@@ -1336,23 +1359,22 @@ __global__ void __launch_bounds__(BT)
           for (int u = 0; u < kU; ++u) {
             if (pp[u] < 0) continue;
             const int p = pp[u];
-            T b2 = 0, bv = 0;
+            // FP64 in both precisions: the nearest pairs' terms reach tens
+            // of kcal/mol and cancel against each other and against the
+            // reciprocal part, so an FP32 rounding of each would dominate
+            // the error of the move's energy change, and on this path FP64
+            // pair arithmetic costs about what FP32 does
+            double b2 = 0, bv = 0;
             const int vk =
                 ldsrow ? D.ptab[p][at[u]] : gload(Y.S.pairtab + D.pts[p] * ntp + at[u]);
-            if constexpr (MIXED) {
-              pair_mixed(Y, static_cast<float>(D.pce[p]), vk, D.ppx[p] - ax[u],
-                         D.ppy[p] - ay[u], D.ppz[p] - az[u], static_cast<float>(aq[u]), b2,
-                         bv);
-            } else {
-              pair_fp64_vk(Y.S, D.pce[p], vk, D.ppx[p] - ax[u], D.ppy[p] - ay[u],
-                           D.ppz[p] - az[u], aq[u], b2, bv);
-            }
+            pair_fp64_vk(Y.S, D.pce[p], vk, D.ppx[p] - ax[u], D.ppy[p] - ay[u],
+                         D.ppz[p] - az[u], aq[u], b2, bv);
             if (D.psg[p] < 0) {
-              a2 -= static_cast<double>(b2);
-              av -= static_cast<double>(bv);
+              a2 -= b2;
+              av -= bv;
             } else {
-              a2 += static_cast<double>(b2);
-              av += static_cast<double>(bv);
+              a2 += b2;
+              av += bv;
             }
           }
         }
@@ -1377,8 +1399,8 @@ __global__ void __launch_bounds__(BT)
     if (kind != 0 && tid >= kWave) {
       MoveDesc& Dn = Dbuf[dcur ^ 1];
       if (Dn.kind != 0)
-        build_tables<T>(Z, Y, Dn, tid / kWave - 1, BT / kWave - 1, tid % kWave, tab, xy_l, zt,
-                        nlm);
+        build_tables<T, MIXED>(Z, Y, Dn, tid / kWave - 1, BT / kWave - 1, tid % kWave, tab,
+                               tabp, xy_l, zt, nlm);
     }
     if (tid == 0) {
       double a = 0.0, b = 0.0, c = 0.0;
@@ -2024,14 +2046,8 @@ int build_system(fmc_batch* b, const fmc_ctx* ctx, uint64_t key,
       (r = dput(K, &vvv, s.vvv.data(), s.vvv.size())) ||
       (r = dput(K, &pairtab, s.pairtab.data(), s.pairtab.size())))
     return r;
-  std::vector<float> erc32(s.erc.begin(), s.erc.end()), vvv32(s.vvv.begin(), s.vvv.end());
-  float *e32, *v32;
-  if ((r = dput(K, &e32, erc32.data(), erc32.size())) ||
-      (r = dput(K, &v32, vvv32.data(), vvv32.size())))
-    return r;
   S.fx = fx; S.fy = fy; S.fz = fz; S.fq = fq; S.ftype = ftype;
   S.erc = erc; S.vvv = vvv; S.pairtab = pairtab;
-  Y.erc32 = e32; Y.vvv32 = v32;
   const int nk = static_cast<int>(s.kvecs.size());
   std::vector<int> kvh(nk);
   std::vector<double> akh(nk);
@@ -2972,6 +2988,9 @@ extern "C" int fmc_batch_run(fmc_batch* b, long long n) {
 
 extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
   if (!b || !st || id < 0 || id >= b->max_chains) return FMC_ERR_ARG;
+  // a launch in flight holds the chain struct in LDS and writes all of it
+  // back at its end: read it only after the stream has drained
+  BCHECK(hipStreamSynchronize(b->stream));
   std::lock_guard<std::mutex> g(b->mu);
   if (!b->chains[id]) return FMC_ERR_ARG;
   BChain C;
@@ -3027,6 +3046,8 @@ extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
 
 extern "C" int fmc_batch_enable_log(fmc_batch* b, int id, long long cap) {
   if (!b || id < 0 || id >= b->max_chains || cap < 0) return FMC_ERR_ARG;
+  // the struct is rewritten here; a launch in flight would overwrite it
+  BCHECK(hipStreamSynchronize(b->stream));
   std::lock_guard<std::mutex> g(b->mu);
   if (!b->chains[id]) return FMC_ERR_ARG;
   BChain C;
@@ -3045,6 +3066,7 @@ extern "C" int fmc_batch_enable_log(fmc_batch* b, int id, long long cap) {
 
 extern "C" int fmc_batch_get_log(fmc_batch* b, int id, long long n, double* rec) {
   if (!b || !rec || id < 0 || id >= b->max_chains) return FMC_ERR_ARG;
+  BCHECK(hipStreamSynchronize(b->stream));
   std::lock_guard<std::mutex> g(b->mu);
   if (!b->chains[id]) return FMC_ERR_ARG;
   BChain C;

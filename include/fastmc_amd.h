@@ -144,7 +144,8 @@ int fmc_energy_full(fmc_ctx *ctx, double *out);
  * (gcmc.f:366-368).
  * ------------------------------------------------------------------- */
 #define FMC_PREC_FP64 0  /* everything FP64: same accept/reject stream */
-#define FMC_PREC_MIXED 1 /* FP32 pair terms and phases, FP64 S(k)/totals */
+#define FMC_PREC_MIXED 1 /* FP32 phase factors e^{ik.r} and dS(k), FP64
+                          * S(k), real-space pair terms and totals */
 
 typedef struct fmc_batch fmc_batch;
 
@@ -248,11 +249,12 @@ int fmc_batch_launch_budget(fmc_batch *b, long long max_moves, double budget_us)
 /* *done = 1 when nothing enqueued on the batch is still running (does not
  * wait) */
 int fmc_batch_query(fmc_batch *b, int *done);
+/* a chain's counters and statistics; waits for the launch in flight */
 int fmc_batch_stats(fmc_batch *b, int chain_id, fmc_chain_stats *st);
 /* per-move log for one chain (parity checks): rec[8*i..8*i+7] = {kind,
  * iguest, accepted, nmol_after, estep (kcal/mol), ewld1eng incl. self term,
  * ewld2sum, vdwsum (internal units)}; enable before running, with capacity
- * moves */
+ * moves.  Both wait for the launch in flight. */
 int fmc_batch_enable_log(fmc_batch *b, int chain_id, long long capacity);
 int fmc_batch_get_log(fmc_batch *b, int chain_id, long long n, double *rec);
 /* FP64 full recompute of every active chain (periodic resync; no

@@ -1,7 +1,9 @@
 """Batched device-side chains against the oracle's restatement of the move
 loop.  FP64: the same duni seed gives the same trajectory -- identical move
 kinds, accept/reject decisions and loadings, estep within 1e-10 relative.
-Mixed precision: estep within 1e-6 relative of FP64 on the same states."""
+Mixed precision: the same decisions while the trajectories coincide, with
+the estep error bars each test states (tests/test_gpu_mixed_c3.py holds the
+bars on the batched workload itself)."""
 import numpy as np
 import pytest
 
@@ -313,9 +315,10 @@ def test_serving_requests_fp64(fmc, oracle):
 
 
 def test_batch_mixed_precision_error(fmc, oracle):
-    """Mixed precision on the same systems: per-move estep within 1e-6
-    relative of FP64 while the trajectories agree; loadings evolve
-    sensibly."""
+    """Mixed precision on the same system as the FP64 oracle chain: the
+    same decisions for the first 50 moves at least, and while the
+    trajectories agree, on the moves with |estep| < 50 kT, an estep error
+    below 1e-4 kT and below 1e-4 relative to max(|estep|, 1e-3 kT)."""
     from fastmc_amd.batch import Batch
     from fastmc_amd.capi import PREC_MIXED
     A = _porous(seed=4, guests=("co2",), press=(20.0,)).system_arrays()
@@ -343,15 +346,16 @@ def test_batch_mixed_precision_error(fmc, oracle):
           f"max err {err.max():.2e} kcal/mol = {err.max() / kT:.2e} kT, median "
           f"{np.median(err):.2e}; raw rel max {raw.max():.2e} (within 1e-6: "
           f"{np.mean(raw < 1e-6):.1%}); floored(1e-3 kT) rel max {floored.max():.2e}")
-    assert err.max() < 1e-3 * kT
-    assert floored.max() < 1e-2
+    assert err.max() < 1e-4 * kT
+    assert floored.max() < 1e-4
 
 
 @pytest.mark.parametrize("kmax", [12, 14])
 def test_batch_mixed_large_nk_register_tiles(fmc, oracle, kmax):
     """Mixed precision on the 512x12 and 512x14 register tiles: the same
     decisions as the FP64 oracle while the trajectories agree, and the
-    estep error bars of the small-system test."""
+    estep error bars of the small-system test (below 1e-4 kT, and below
+    1e-4 relative to max(|estep|, 1e-3 kT))."""
     from fastmc_amd.batch import Batch
     from fastmc_amd.capi import PREC_MIXED
     from fastmc_amd import synth
@@ -379,8 +383,8 @@ def test_batch_mixed_large_nk_register_tiles(fmc, oracle, kmax):
     floored = err / np.maximum(np.abs(a[keep]), 1e-3 * kT)
     print(f"kmax {kmax}: identical for {first_diff} moves; max err {err.max() / kT:.2e} kT, "
           f"median {np.median(err) / kT:.2e} kT; floored rel max {floored.max():.2e}")
-    assert err.max() < 1e-3 * kT
-    assert floored.max() < 1e-2
+    assert err.max() < 1e-4 * kT
+    assert floored.max() < 1e-4
 
 
 def test_batch_recompute_matches_oracle_energy(fmc, oracle):
@@ -530,8 +534,8 @@ def test_batch_binary_example(fmc, oracle, restart_example):
     """The reference's binary example (CO2 + TIP4P-Ew with its massless
     charge site, started from the unary REVCON): FP64 gives the oracle
     chain's trajectory; mixed precision makes the same decisions, with
-    estep within 1e-4 kcal/mol of FP64 where |estep| < 100 kcal/mol and
-    within 1e-4 relative on overlapping trials (|estep| up to ~1e9, all
+    estep within 1e-5 kcal/mol of FP64 where |estep| < 100 kcal/mol and
+    within 1e-6 relative on overlapping trials (|estep| up to ~1e9, all
     rejected)."""
     from fastmc_amd.batch import Batch
     from test_restart_files import binary_example
@@ -566,7 +570,7 @@ def test_batch_binary_example(fmc, oracle, restart_example):
           f"mixed identical for {k} moves, max |dE| {dmix:.2e} kcal/mol "
           f"(|estep| < 100), max rel {rbig:.2e} on {int((~small).sum())} overlaps")
     assert (ref[:, 1] == 1).sum() > 100
-    assert k > 100 and dmix < 1e-4 and rbig < 1e-4
+    assert k > 100 and dmix < 1e-5 and rbig < 1e-6
 
 
 def test_capacity_fault_reported(fmc):
@@ -656,3 +660,32 @@ def test_budget_launches_do_not_change_trajectories(fmc):
         assert np.array_equal(a.get_log(cid, steps[cid]), bb.get_log(cid, steps[cid]))
         assert a.stats(cid)["recomputes"] == bb.stats(cid)["recomputes"]
         bb.close()
+
+
+def test_stats_and_log_calls_wait_for_the_launch_in_flight(fmc):
+    """fmc_batch_stats, fmc_batch_enable_log and fmc_batch_get_log read or
+    rewrite the chain struct that a running launch holds in LDS and writes
+    back at its end: issued while a launch is in flight, they wait for it.
+    The log enabled that way stays enabled and records the next launch; the
+    stats read meanwhile equal a read after an explicit sync."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd import synth
+    A = synth.c1().system_arrays()
+    b = Batch(2)
+    ctx = fmc.Context.from_arrays(A)
+    c0 = b.add_chain(ctx, 298.0, [0.15e5], (12, 34, 56, 78), delr=A["delr"])
+    c1 = b.add_chain(ctx, 298.0, [0.15e5], (21, 43, 65, 87), delr=A["delr"])
+    b.launch(400)
+    st_early = b.stats(c0)          # no sync() before it
+    b.sync()
+    st_synced = b.stats(c0)
+    assert st_early["steps"] == st_synced["steps"] == 400
+    assert st_early == st_synced
+    b.launch(300)
+    b.enable_log(c1, 200)           # while the second launch runs
+    b.sync()
+    b.launch(200)
+    log = b.get_log(c1, 200)        # while the third launch runs
+    assert b.stats(c1)["steps"] == 900
+    assert np.all(log[:, 0] >= 0) and np.any(log[:, 0] != 0)
+    b.close()
