@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -533,7 +534,31 @@ void fill_segs(fmc_ctx* c, GuestSeg* g, int* ngt) {
   }
 }
 
-int recip_full_impl(fmc_ctx* c, bool all_atoms, bool set_running,
+// Device buffers one full-energy evaluation works in: the recip partial
+// sums [nchunk][nk], the energy partials [nk/256], two output slots and the
+// real-space partial sums [2][blocks]; the context's own buffers, or the
+// library's static scratch array (fmc_energy_full_noalloc).
+struct FullWork {
+  double2* rpart;
+  size_t rpart_cap;   // double2 entries
+  double* epart;
+  double* out;
+  double* pair;       // nullptr: allocated per call (fmc_energy_full)
+  size_t pair_cap;    // doubles
+};
+
+FullWork ctx_work(fmc_ctx* c) {
+  FullWork w;
+  w.rpart = c->d_rpart;
+  w.rpart_cap = static_cast<size_t>(c->recip_nchunk_cap) * c->nk;
+  w.epart = c->d_epart;
+  w.out = c->d_out;
+  w.pair = nullptr;
+  w.pair_cap = 0;
+  return w;
+}
+
+int recip_full_impl(fmc_ctx* c, const FullWork& w, bool all_atoms, bool set_running,
                     double* energy) {
   RecipFullParams P;
   fill_devsys(c, &P.S);
@@ -546,8 +571,8 @@ int recip_full_impl(fmc_ctx* c, bool all_atoms, bool set_running,
   P.ak = c->d_ak;
   P.chunk = c->recip_chunk;
   const int nchunk = std::max(1, (P.natoms + P.chunk - 1) / P.chunk);
-  if (nchunk > c->recip_nchunk_cap) return FMC_ERR_CAPACITY;
-  P.part = c->d_rpart;
+  if (static_cast<size_t>(nchunk) * c->nk > w.rpart_cap) return FMC_ERR_CAPACITY;
+  P.part = w.rpart;
   const int kb = (c->nk + kBS - 1) / kBS;
   hipLaunchKernelGGL(recip_partial_kernel, dim3(kb, nchunk), dim3(kBS), 0,
                      c->stream, P);
@@ -555,12 +580,12 @@ int recip_full_impl(fmc_ctx* c, bool all_atoms, bool set_running,
   double* sre = set_running ? c->d_sre[c->cur] : nullptr;
   double* sim = set_running ? c->d_sim[c->cur] : nullptr;
   hipLaunchKernelGGL(recip_reduce_kernel, dim3(kb), dim3(kBS), 0, c->stream,
-                     c->d_rpart, nchunk, c->nk, c->d_ak, sre, sim, c->d_epart);
+                     w.rpart, nchunk, c->nk, c->d_ak, sre, sim, w.epart);
   HIPCHECK(hipGetLastError());
-  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, c->d_epart,
-                     kb, c->d_out);
+  hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, w.epart,
+                     kb, w.out);
   HIPCHECK(hipGetLastError());
-  HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, sizeof(double),
+  HIPCHECK(hipMemcpyAsync(c->h_out, w.out, sizeof(double),
                           hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
   double sq = 0.0;
@@ -573,6 +598,64 @@ int recip_full_impl(fmc_ctx* c, bool all_atoms, bool set_running,
   const double engsic = -kR4pie0 / s.epsq * s.alpha * sq / kSqrpi;
   const double rvolm = 2.0 * kPi / s.volm;
   *energy = 2.0 * rvolm * kR4pie0 * c->h_out[0] / s.epsq + engsic;
+  return FMC_OK;
+}
+
+// single_point (gcmc.f:2518-2710) on the work buffers w: framework and
+// all-atom reciprocal sums (the latter resets the running S(k), as the
+// ewald1 call at gcmc.f:522 does), every guest atom's real-space pairs,
+// the ewald3 exclusions and the tail.
+int energy_full_impl(fmc_ctx* c, const FullWork& w, double* out) {
+  int r;
+  double efw = 0.0, eall = 0.0;
+  if ((r = recip_full_impl(c, w, false, false, &efw))) return r;
+  if ((r = recip_full_impl(c, w, true, true, &eall))) return r;
+  PairAllParams P;
+  fill_devsys(c, &P.S);
+  fill_segs(c, P.g, &P.ngt);
+  P.nguest = 0;
+  for (int t = 0; t < P.ngt; ++t) P.nguest += P.g[t].natoms;
+  double e2 = 0.0, ev = 0.0, e3 = 0.0;
+  if (P.nguest > 0) {
+    const int ntot = P.S.nfw + P.nguest;
+    const int gx = std::min(8, (ntot + kBS - 1) / kBS);
+    const size_t nbt = static_cast<size_t>(gx) * P.nguest;
+    double* part = w.pair;
+    if (part) {
+      if (2 * nbt > w.pair_cap) return FMC_ERR_CAPACITY;
+    } else {
+      HIPCHECK(hipMalloc(&part, 2 * nbt * sizeof(double)));
+    }
+    P.part = part;
+    hipLaunchKernelGGL(pair_all_kernel, dim3(gx, P.nguest), dim3(kBS), 0,
+                       c->stream, P);
+    HIPCHECK(hipGetLastError());
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, part,
+                       static_cast<int>(nbt), w.out);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, part + nbt,
+                       static_cast<int>(nbt), w.out + 1);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(c->h_out, w.out, 2 * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (!w.pair) HIPCHECK(hipFree(part));
+    e2 = c->h_out[0];
+    ev = c->h_out[1];
+    for (size_t t = 0; t < c->sys.guests.size(); ++t) {
+      const int ns = c->sys.guests[t].ns;
+      for (int m = 0; m < c->nmol[t]; ++m)
+        e3 += ewald3_molecule(c->sys, c->sys.guests[t], &c->hx[t][m * ns],
+                              &c->hy[t][m * ns], &c->hz[t][m * ns]);
+    }
+  }
+  std::vector<int> nt, nf;
+  c->sys.counts(c->nmol, &nt, &nf);
+  out[0] = efw;
+  out[1] = eall - efw;
+  out[2] = e2;
+  out[3] = ev;
+  out[4] = e3;
+  out[5] = c->sys.tail(nt, nf);
   return FMC_OK;
 }
 
@@ -847,7 +930,7 @@ static int ensure_device(fmc_ctx* c) {
   int r;
   if ((r = c->upload_state())) return r;
   double e;
-  if ((r = recip_full_impl(c, true, true, &e))) return r;
+  if ((r = recip_full_impl(c, ctx_work(c), true, true, &e))) return r;
   c->dev_ready = true;
   return FMC_OK;
 }
@@ -927,7 +1010,7 @@ extern "C" int fmc_recip_full(fmc_ctx* c, int all_atoms, double* energy) {
   int r;
   if ((r = ensure_device(c))) return r;
   c->pending = 0;
-  return recip_full_impl(c, all_atoms != 0, all_atoms != 0, energy);
+  return recip_full_impl(c, ctx_work(c), all_atoms != 0, all_atoms != 0, energy);
 }
 
 // Wait for a trial's sums: spin on the sequence word each work-group
@@ -1127,51 +1210,92 @@ extern "C" int fmc_energy_full(fmc_ctx* c, double* out) {
   int r;
   if ((r = ensure_device(c))) return r;
   c->pending = 0;
-  double efw = 0.0, eall = 0.0;
-  if ((r = recip_full_impl(c, false, false, &efw))) return r;
-  if ((r = recip_full_impl(c, true, true, &eall))) return r;
-  PairAllParams P;
-  fill_devsys(c, &P.S);
-  fill_segs(c, P.g, &P.ngt);
-  P.nguest = 0;
-  for (int t = 0; t < P.ngt; ++t) P.nguest += P.g[t].natoms;
-  double e2 = 0.0, ev = 0.0, e3 = 0.0;
-  if (P.nguest > 0) {
-    const int ntot = P.S.nfw + P.nguest;
-    const int gx = std::min(8, (ntot + kBS - 1) / kBS);
-    const size_t nbt = static_cast<size_t>(gx) * P.nguest;
-    double* part = nullptr;
-    HIPCHECK(hipMalloc(&part, 2 * nbt * sizeof(double)));
-    P.part = part;
-    hipLaunchKernelGGL(pair_all_kernel, dim3(gx, P.nguest), dim3(kBS), 0,
-                       c->stream, P);
-    HIPCHECK(hipGetLastError());
-    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, part,
-                       static_cast<int>(nbt), c->d_out);
-    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kBS), 0, c->stream, part + nbt,
-                       static_cast<int>(nbt), c->d_out + 1);
-    HIPCHECK(hipMemcpyAsync(c->h_out, c->d_out, 2 * sizeof(double),
-                            hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
-    HIPCHECK(hipFree(part));
-    e2 = c->h_out[0];
-    ev = c->h_out[1];
-    for (size_t t = 0; t < c->sys.guests.size(); ++t) {
-      const int ns = c->sys.guests[t].ns;
-      for (int m = 0; m < c->nmol[t]; ++m)
-        e3 += ewald3_molecule(c->sys, c->sys.guests[t], &c->hx[t][m * ns],
-                              &c->hy[t][m * ns], &c->hz[t][m * ns]);
+  return energy_full_impl(c, ctx_work(c), out);
+}
+
+// ---------------------------------------------------------------------------
+// single_point without allocation.  Its scratch -- every partial sum and
+// output slot of fmc_energy_full -- is one file-scope device array of
+// kScratchBytes, held by each GPU the library runs on until the process
+// ends.  Host code never passes the array's name to a kernel or a copy (on
+// the host it names a stand-in, not device memory): it uses the device
+// address hipGetSymbolAddress returns, looked up once per device with that
+// device current.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t kScratchBytes = size_t(4) << 20;
+constexpr int kScratchDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kScratchDevices] = {nullptr};
+
+// the current device's address of scratch
+int scratch_address(void** out) {
+  int dev = 0;
+  HIPCHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return FMC_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      std::fprintf(stderr, "fastmc_amd: hipGetSymbolAddress(scratch) failed: %s\n",
+                   hipGetErrorString(e));
+      return FMC_ERR_HIP;
     }
+    g_scratch_ptr[dev] = p;
   }
-  std::vector<int> nt, nf;
-  c->sys.counts(c->nmol, &nt, &nf);
-  out[0] = efw;
-  out[1] = eall - efw;
-  out[2] = e2;
-  out[3] = ev;
-  out[4] = e3;
-  out[5] = c->sys.tail(nt, nf);
+  *out = g_scratch_ptr[dev];
   return FMC_OK;
+}
+
+}  // namespace
+
+extern "C" int fmc_energy_full_noalloc(fmc_ctx* c, double* out) {
+  if (!c || !c->sys.finalized || !out) return FMC_ERR_STATE;
+  // building the device state allocates: it must exist already
+  if (!c->dev_ready) return FMC_ERR_STATE;
+  void* base = nullptr;
+  int r;
+  if ((r = scratch_address(&base))) return r;
+  // layout: out[8] | epart[kb] | pair[2 * 8 * nguest] | rpart[nchunk * nk],
+  // each from a 256-byte boundary
+  auto up = [](size_t n) { return (n + 255) & ~size_t(255); };
+  int nguest = 0;
+  for (size_t t = 0; t < c->sys.guests.size(); ++t) nguest += c->nmol[t] * c->sys.guests[t].ns;
+  const int natoms = static_cast<int>(c->sys.fx.size()) + nguest;
+  const size_t nchunk = static_cast<size_t>(std::max(1, (natoms + c->recip_chunk - 1) /
+                                                            c->recip_chunk));
+  const size_t kb = static_cast<size_t>((c->nk + kBS - 1) / kBS);
+  const size_t n_out = up(8 * sizeof(double));
+  const size_t n_ep = up(kb * sizeof(double));
+  const size_t n_pair = up(2 * 8 * static_cast<size_t>(nguest) * sizeof(double));
+  const size_t n_rp = nchunk * c->nk * sizeof(double2);
+  if (n_out + n_ep + n_pair + n_rp > kScratchBytes) {
+    std::fprintf(stderr,
+                 "fastmc_amd: fmc_energy_full_noalloc needs %zu bytes of scratch (%zu atoms, "
+                 "%d k-vectors, %d guest atoms); the library holds %zu\n",
+                 n_out + n_ep + n_pair + n_rp, static_cast<size_t>(natoms), c->nk, nguest,
+                 kScratchBytes);
+    return FMC_ERR_CAPACITY;
+  }
+  char* p = static_cast<char*>(base);
+  FullWork w;
+  w.out = reinterpret_cast<double*>(p);
+  w.epart = reinterpret_cast<double*>(p + n_out);
+  w.pair = reinterpret_cast<double*>(p + n_out + n_ep);
+  w.pair_cap = n_pair / sizeof(double);
+  w.rpart = reinterpret_cast<double2*>(p + n_out + n_ep + n_pair);
+  w.rpart_cap = (kScratchBytes - n_out - n_ep - n_pair) / sizeof(double2);
+  c->pending = 0;
+  return energy_full_impl(c, w, out);
 }
 
 extern "C" int fmc_version(int* major, int* minor) {

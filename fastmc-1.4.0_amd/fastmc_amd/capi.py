@@ -80,6 +80,7 @@ _SIGS = {
     "fmc_commit": [C.c_void_p],
     "fmc_tail_delta": [C.c_void_p, C.c_int, C.c_int, C.c_double, DP],
     "fmc_energy_full": [C.c_void_p, DP],
+    "fmc_energy_full_noalloc": [C.c_void_p, DP],
     "fmc_batch_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int],
     "fmc_batch_destroy": [C.c_void_p],
     "fmc_batch_add_chain": [C.c_void_p, C.c_void_p, C.POINTER(ChainParams), IP],
@@ -272,6 +273,14 @@ class Context:
     def energy_full(self):
         out = np.zeros(6)
         check(self.L.fmc_energy_full(self.h, out.ctypes.data_as(DP)), "fmc_energy_full")
+        return out
+
+    def energy_full_noalloc(self):
+        """energy_full on the library's static device scratch (no device
+        allocation); the device state must exist already."""
+        out = np.zeros(6)
+        check(self.L.fmc_energy_full_noalloc(self.h, out.ctypes.data_as(DP)),
+              "fmc_energy_full_noalloc")
         return out
 
 

@@ -134,6 +134,18 @@ int fmc_tail_delta(fmc_ctx *ctx, int gtype, int sign, double elrc,
  * recip(all)-recip(framework), real-space coulomb, vdW, ewald3 exclusions,
  * elrc}; internal units.  Also resets the running S(k) like gcmc.f:522. */
 int fmc_energy_full(fmc_ctx *ctx, double *out);
+/* fmc_energy_full without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Every
+ * partial sum and output slot lives in one static device array of 4 MiB
+ * that each GPU the library runs on holds until the process ends: a system
+ * fits while 16 * ceil(atoms / 1024) * Nk + 128 * guest atoms + 8 KiB
+ * stays within it, else FMC_ERR_CAPACITY (with a message on stderr).  The
+ * context's device state must exist already (any earlier fmc_trial,
+ * fmc_recip_full or fmc_energy_full builds it), else FMC_ERR_STATE.  Calls
+ * share the array of their GPU and keep its last contents; calls that
+ * overlap on one GPU race. */
+int fmc_energy_full_noalloc(fmc_ctx *ctx, double *out);
 
 /* ---------------------------------------------------------------------
  * Batched device-resident chains.  No reference counterpart call site: it
