@@ -55,6 +55,8 @@ constexpr int kWinDoubles = kWinSnap * kWinQ * kMaxGuestTypes;
 constexpr int kWinMax = 500000;  // samples beyond which the reference keeps
                                  // rolling averages instead (gcmc.f:1971)
 constexpr int kLdsTypes = 32;   // atom types whose pair rows sit in LDS
+constexpr int kSelPairs = kMaxGuestTypes * (kMaxGuestTypes - 1) / 2;
+constexpr long long kSelWindow = 100000;   // readinputs.f:888-889
 
 // The small per-system constants the serial lane and the per-move set-up
 // read; the batch kernel keeps a copy in LDS for the whole launch.
@@ -150,6 +152,11 @@ struct BChain {
   // recompute (kcal/mol); drift = e_acc - e_full at that recompute
   double e_acc, e_full, drift, sk_dev;
   long long n_recompute;
+  // selectivity (gcmc.f:1887-1916): windows of sel_nwind production
+  // samples; pair p = i (i - 1) / 2 + j of guests j < i
+  long long sel_nwind, sel_count;
+  double sel_base[kMaxGuestTypes];   // sum_n when the last window closed
+  double sel_mean[kSelPairs], sel_sec[kSelPairs], sel_var[kSelPairs];
 };
 
 // ---- RNG on one lane (gcmc.f:2786-2799) ----
@@ -659,6 +666,31 @@ __device__ void window_snapshot(BChain& C, double* snaps) {
   }
 }
 
+// A selectivity window closes (gcmc.f:1887-1916 with weight 1): its <N_g>
+// are the production sums since the last close over the window's samples;
+// every closed window counts, a pair whose <N_i> is zero is not updated.
+__device__ __forceinline__ void selectivity_window(BChain& C) {
+#pragma clang fp contract(off)
+  C.sel_count++;
+  const double av = static_cast<double>(C.sel_count);
+  const double nw = static_cast<double>(C.sel_nwind);
+  const double weight = 1.0;
+  for (int i = 1; i < C.ntpguest; ++i) {
+    const double aN = (C.sum_n[i] - C.sel_base[i]) / nw;
+    if (aN == 0.0) continue;
+    for (int j = 0; j < i; ++j) {
+      const int p = i * (i - 1) / 2 + j;
+      const double sij = ((C.sum_n[j] - C.sel_base[j]) / nw) / aN;
+      const double delta = weight * (sij - C.sel_mean[p]);
+      C.sel_mean[p] = C.sel_mean[p] + delta / av;
+      C.sel_sec[p] = C.sel_sec[p] + weight * (sij * sij - C.sel_sec[p]) / (av - 1.0 + weight);
+      const double w = av - 1.0 + weight;
+      C.sel_var[p] = (w * C.sel_sec[p] - (w * C.sel_mean[p] * C.sel_mean[p])) / w;
+    }
+  }
+  for (int g = 0; g < C.ntpguest; ++g) C.sel_base[g] = C.sum_n[g];
+}
+
 // Production statistics of one move (gcmc.f:1814-1848), after C.steps has
 // counted it; true when it is a production sample.
 __device__ __forceinline__ bool production_sample(BChain& C) {
@@ -673,6 +705,7 @@ __device__ __forceinline__ bool production_sample(BChain& C) {
     C.sum_en[g] += e * n;
     C.sum_e2[g] += e * e;
   }
+  if (C.ntpguest > 1 && C.prod_steps % C.sel_nwind == 0) selectivity_window(C);
   return true;
 }
 
@@ -2634,6 +2667,7 @@ extern "C" int fmc_batch_add_chain(fmc_batch* b, const fmc_ctx* ctx,
       }
     }
     C.recomp_every = p->recompute_every > 0 ? p->recompute_every : 0;
+    C.sel_nwind = p->window > 0 ? p->window : kSelWindow;
     duni_setup(p->seed[0], p->seed[1], p->seed[2], p->seed[3], &C);
     for (auto& kb : K) ch->bytes += kb->n;
     // the slot is outside any launch in flight: its struct and the
@@ -2689,7 +2723,7 @@ struct CkHeader {
   int chain_bytes;       // sizeof(BChain)
 };
 constexpr char kCkMagic[8] = {'F', 'M', 'C', 'C', 'H', 'K', 0, 0};
-constexpr int kCkVersion = 1;
+constexpr int kCkVersion = 2;
 
 // A duni state the generator can reach (gcmc.f:2760-2799): the table and c
 // multiples of 2^-24 in [0, 1) and [0, cm), cd and cm its constants, and the
@@ -2790,7 +2824,7 @@ extern "C" int fmc_batch_restore_chain(fmc_batch* b, const fmc_ctx* ctx, const v
       h.chain_bytes != static_cast<int>(sizeof(BChain)) || h.key != key ||
       h.nk != ctx->nk || h.ntpguest != static_cast<int>(ctx->sys.guests.size()) ||
       h.ntpguest != C.ntpguest || h.cap_atoms != C.cap_atoms || C.cap_atoms < 1 ||
-      !rng_state_ok(C))
+      !rng_state_ok(C) || C.sel_nwind < 1 || C.sel_count < 0)
     return FMC_ERR_ARG;
   for (int t = 0; t < h.ntpguest; ++t)
     if (h.ns[t] != ctx->sys.guests[t].ns || C.nmol[t] < 0 ||
@@ -3020,6 +3054,14 @@ extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
   st->drift = C.drift;
   st->sk_dev = C.sk_dev;
   st->recomputes = C.n_recompute;
+  st->sel_window = C.sel_nwind;
+  st->sel_count = C.sel_count;
+  for (int g = 0; g < kMaxGuestTypes; ++g) st->sel_base[g] = C.sel_base[g];
+  for (int q = 0; q < kSelPairs; ++q) {
+    st->sel_mean[q] = C.sel_mean[q];
+    st->sel_second[q] = C.sel_sec[q];
+    st->sel_var[q] = C.sel_var[q];
+  }
   st->win_len = C.win_len;
   st->win_closed = 0;
   if (C.win_len > 0) {

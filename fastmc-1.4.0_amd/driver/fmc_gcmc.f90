@@ -207,6 +207,10 @@ program fmc_gcmc
   integer :: nfw
   ! control
   integer :: mcsteps, eqsteps, nnumg, maxguest, seed(4), values(8)
+  ! selectivity windows (gcmc.f:1887-1916)
+  integer :: nwind, avcount
+  integer(8) :: rollcount
+  real(8) :: wsum(maxg), seltot(maxg,maxg), selwin(maxg,maxg), weight, ij_select, aN, delta
   ! move loop
   integer(8) :: gcmccount, prodcount
   integer :: randchoice, kind, natms, nmols, ndraw
@@ -232,6 +236,7 @@ program fmc_gcmc
   ! ------------------------------------------------------------------
   temp = 0.d0; rcut = 0.d0; delr_in = 1.d0; rvdw = 0.d0; have_sum = .false.
   have_prec = .false.; mcsteps = 1; eqsteps = 0; nnumg = 1000; maxguest = 3000
+  nwind = 100000
   do_trace = .false.; have_seed = .false.; rota_ratio = 0.5d0; lrestart = .false.
   gfreq = -1.d0; gdelr = -1.d0; gdisp = 0.5d0; gtran = 0.5d0
   press = 0.d0; freq = 0.d0; gfreq_set = .false.; gdelr_set = .false.
@@ -251,6 +256,9 @@ program fmc_gcmc
       end do
     else if (has(rec, 'trace')) then
       do_trace = .true.
+    else if (has(rec, 'averaging window')) then
+      nwind = intstr(rec)
+      if (nwind <= 0) nwind = 100000
     else if (has(rec, 'restart')) then
       lrestart = .true.
     else if (has(rec, 'temp')) then
@@ -660,6 +668,7 @@ program fmc_gcmc
   dele_last = 0.d0
   allocate(gasmol(maxsamp, ntpguest), gaseng(maxsamp, ntpguest))
   st_n = 0.d0; st_e = 0.d0; st_en = 0.d0; st_n2 = 0.d0; st_e2 = 0.d0
+  avcount = 0; rollcount = 0; wsum = 0.d0; seltot = 0.d0; selwin = 0.d0
   call system_clock(c0, crate)
   do while (lgchk)
     gcmccount = gcmccount + 1
@@ -869,6 +878,31 @@ program fmc_gcmc
         st_n2(i) = st_n2(i) + (ra * ra - st_n2(i)) / prodcount
         st_e2(i) = st_e2(i) + (energy(i) ** 2 - st_e2(i)) / prodcount
       end do
+      ! selectivity of guest j over guest i over windows of nwind samples,
+      ! the last one partial at the end of the run (gcmc.f:1887-1916)
+      rollcount = rollcount + 1
+      do i = 1, ntpguest
+        wsum(i) = wsum(i) + dble(nmol(i))
+      end do
+      if (mod(prodcount, int(nwind, 8)) == 0 .or. .not. lgchk) then
+        avcount = avcount + 1
+        weight = dble(rollcount) / dble(nwind)
+        do i = 2, ntpguest
+          aN = wsum(i) / dble(rollcount)
+          do j = 1, i - 1
+            if (aN == 0.d0) cycle
+            ij_select = (wsum(j) / dble(rollcount)) / aN
+            delta = weight * (ij_select - seltot(i, j))
+            seltot(i, j) = seltot(i, j) + delta / avcount
+            selwin(i, j) = selwin(i, j) + weight * (ij_select * ij_select - selwin(i, j)) / &
+              (avcount - 1 + weight)
+            seltot(j, i) = ((avcount - 1 + weight) * selwin(i, j) - &
+              ((avcount - 1 + weight) * seltot(i, j) * seltot(i, j))) / (avcount - 1 + weight)
+          end do
+        end do
+        wsum = 0.d0
+        rollcount = 0
+      end if
     end if
     if (mod(gcmccount, int(nnumg, 8)) == 0) then
       do i = 1, ntpguest
@@ -896,6 +930,15 @@ program fmc_gcmc
   do i = 1, ntpguest
     call block_stats(i)
   end do
+  ! gcmc.f:2315-2322: '+/-' is followed by seltot(j,i), a variance
+  if (ntpguest > 1) then
+    do i = 1, ntpguest
+      do j = 1, i - 1
+        write(6, "(/'selectivity stats for guest ',i2,'/'i2)") j, i
+        write(6, "(a36,f15.6,a5,f12.3)") '<S>:', seltot(i, j), '+/-', seltot(j, i)
+      end do
+    end do
+  end if
   close(6)
   close(401)
   if (do_trace) close(402)

@@ -10,6 +10,12 @@ from .capi import (ChainParams, ChainStats, DP, FmcError, PREC_FP64, check,
                    lib)
 
 
+class Report(list):
+    """A chain's end-of-run report: one stats.NodeStats per guest type, and
+    the selectivity of each guest pair ({(j, i): stats.Selectivity})."""
+    selectivity: dict = {}
+
+
 class Batch:
     """Many independent GCMC chains advanced together on the GPU, one chain
     per work-group of a persistent kernel."""
@@ -38,7 +44,7 @@ class Batch:
                   seed: Sequence[int], freq: Optional[Sequence[Sequence[float]]] = None,
                   delr: float = 1.0, eq_steps: int = 0, engunit: float = 418.4,
                   ratios=(0.5, 0.5, 0.5), max_moves: int = 0,
-                  recompute_every: int = 0) -> int:
+                  recompute_every: int = 0, window: int = 0) -> int:
         p = ChainParams()
         p.temp = temp
         p.ntpguest = len(press_pa)
@@ -57,6 +63,7 @@ class Batch:
         p.eq_steps = eq_steps
         p.max_moves = max_moves
         p.recompute_every = recompute_every
+        p.window = window
         cid = C.c_int()
         check(self.L.fmc_batch_add_chain(self.h, ctx.h, C.byref(p), C.byref(cid)),
               "fmc_batch_add_chain")
@@ -171,6 +178,9 @@ class Batch:
                     win_len=st.win_len, win_closed=st.win_closed,
                     win=[[[st.win[j][q][g] for g in range(ng)] for q in range(5)]
                          for j in range(5)],
+                    sel_window=st.sel_window, sel_count=st.sel_count,
+                    sel_base=list(st.sel_base)[:ng], sel_mean=list(st.sel_mean),
+                    sel_second=list(st.sel_second), sel_var=list(st.sel_var),
                     capacity_fault=(r == -4))
 
     def report(self, cid: int, temp: float):
@@ -186,11 +196,40 @@ class Batch:
                              "5 to 500000 production moves)")
         if st["win_closed"] < 4 or st["prod_steps"] < 5 * st["win_len"]:
             raise ValueError("chain has not finished its production moves")
-        out = []
+        out = Report()
         for g in range(len(st["nmol"])):
             sums = [[st["win"][j][q][g] for q in range(5)] for j in range(5)]
             out.append(S.node_stats(sums, st["win_len"], float(st["prod_steps"]), temp))
+        out.selectivity = self.selectivity(cid, st)
         return out
+
+    def selectivity(self, cid: int, st: dict = None) -> dict:
+        """The reference's selectivity of guest j over guest i, j < i
+        (gcmc.f:1887-1916), for a chain at the end of its run: the kernel's
+        closed windows, then the open window with weight samples / window,
+        as the reference takes it at its last move.  {(j, i):
+        stats.Selectivity}; empty for one guest type."""
+        from . import stats as S
+        st = st or self.stats(cid)
+        ng = len(st["nmol"])
+        if ng < 2:
+            return {}
+        state = S.selectivity_state(ng)
+        state[0] = st["sel_count"]
+        for i in range(1, ng):
+            for j in range(i):
+                p = i * (i - 1) // 2 + j
+                state[1][i][j] = st["sel_mean"][p]
+                state[2][i][j] = st["sel_second"][p]
+                state[3][i][j] = st["sel_var"][p]
+        nw = st["sel_window"]
+        rest = st["prod_steps"] - st["sel_count"] * nw
+        last_w = 1.0
+        if rest > 0:
+            means = [(st["sum_n"][g] - st["sel_base"][g]) / rest for g in range(ng)]
+            last_w = rest / nw
+            S.selectivity_update(state, means, last_w)
+        return S.selectivity_report(state, last_w)
 
     def enable_log(self, cid: int, cap: int):
         check(self.L.fmc_batch_enable_log(self.h, cid, cap), "fmc_batch_enable_log")

@@ -36,7 +36,7 @@ class ChainParams(C.Structure):
                 ("disp_ratio", C.c_double * 4), ("tran_ratio", C.c_double * 4),
                 ("rota_ratio", C.c_double), ("engunit", C.c_double),
                 ("eq_steps", C.c_longlong), ("max_moves", C.c_longlong),
-                ("recompute_every", C.c_longlong)]
+                ("recompute_every", C.c_longlong), ("window", C.c_longlong)]
 
 
 class ChainStats(C.Structure):
@@ -51,7 +51,10 @@ class ChainStats(C.Structure):
                 ("sk_dev", C.c_double), ("recomputes", C.c_longlong),
                 ("sum_en", C.c_double * 4), ("sum_e2", C.c_double * 4),
                 ("win_len", C.c_longlong), ("win_closed", C.c_int),
-                ("win", ((C.c_double * 4) * 5) * 5)]
+                ("win", ((C.c_double * 4) * 5) * 5),
+                ("sel_window", C.c_longlong), ("sel_count", C.c_longlong),
+                ("sel_base", C.c_double * 4), ("sel_mean", C.c_double * 6),
+                ("sel_second", C.c_double * 6), ("sel_var", C.c_double * 6)]
 
 
 _lib = None

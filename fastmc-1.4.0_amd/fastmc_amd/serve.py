@@ -55,6 +55,8 @@ class Result:
     footprint: int
     report: Optional[list] = None   # per guest type: stats.NodeStats, the
                                     # reference's end-of-run statistics
+    selectivity: Optional[dict] = None   # {(j, i): stats.Selectivity} of each
+                                         # guest pair j < i (gcmc.f:1887-1916)
 
 
     @property
@@ -104,9 +106,10 @@ class Server:
                 cv.wait_for(lambda: cid in self._live)
                 req, t_adm, need = self._live[cid]
             st = self.batch.stats(cid)
-            rep = None
+            rep, sel = None, None
             if st["win_len"] > 0 and not st["capacity_fault"]:
                 rep = self.batch.report(cid, req.temp)
+                sel = rep.selectivity
             # the library's slot and the _live entry go together, under the
             # lock: the admission thread counts slots from _live, so it never
             # adds a chain while this one still holds a library slot, and a
@@ -118,7 +121,7 @@ class Server:
                 self._used -= need
                 self.moves_done += st["steps"]
                 self.results[req.rid] = Result(req.rid, req.arrival, t_adm, now, st, need,
-                                               rep)
+                                               rep, sel)
                 cv.notify_all()
 
     def run(self, requests: Sequence[Request], realtime: bool = True,

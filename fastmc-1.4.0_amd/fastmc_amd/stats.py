@@ -157,3 +157,100 @@ def from_samples(n_samples, e_samples, temp: float) -> NodeStats:
         return node_stats(None, 0, float(n), temp, roll)
     sums, nwind = window_sums(n_samples, e_samples)
     return node_stats(sums, nwind, float(n), temp)
+
+
+# ---------------------------------------------------------------------------
+# selectivity (gcmc.f:1887-1916, printed at gcmc.f:2315-2322)
+# ---------------------------------------------------------------------------
+NWIND = 100000                    # readinputs.f:888-889 ("averaging window")
+
+
+@dataclass
+class Selectivity:
+    """Guest j over guest i (j < i, 0-based): the reference's running,
+    window-weighted mean of <N_j>_w / <N_i>_w over the averaging windows w
+    (seltot(i,j)), the quantity it prints after '+/-' (seltot(j,i): the
+    weighted second moment minus the squared mean, a variance), and the
+    weight sum avcount - 1 + weight of the last window."""
+    mean: float
+    var: float
+    weight: float
+    second: float = 0.0   # weighted mean of the squared ratio (selwin(i,j))
+
+
+def selectivity_update(state, means, weight):
+    """One window of the reference's update: avcount counts every window
+    (also those skipped for <N_i> = 0); means = the window's <N_g>;
+    weight = its samples / nwind (1 for a full window).  state =
+    [avcount, mean[i][j], second[i][j], var[i][j]] with i > j, updated in
+    place; var is recomputed, as the reference does, at each window that
+    updates the pair."""
+    state[0] += 1
+    av = state[0]
+    mean, sec, var = state[1], state[2], state[3]
+    ng = len(means)
+    for i in range(1, ng):
+        aN = means[i]
+        for j in range(i):
+            if aN == 0:
+                continue
+            s = _div(means[j], aN)
+            delta = weight * (s - mean[i][j])
+            mean[i][j] = mean[i][j] + delta / av
+            sec[i][j] = sec[i][j] + weight * (s * s - sec[i][j]) / (av - 1 + weight)
+            w = av - 1 + weight
+            var[i][j] = _div(w * sec[i][j] - (w * mean[i][j] * mean[i][j]), w)
+    return state
+
+
+def selectivity_state(ng: int):
+    z = lambda: [[0.0] * ng for _ in range(ng)]   # noqa: E731
+    return [0, z(), z(), z()]
+
+
+def selectivity_report(state, last_weight: float):
+    """{(j, i): Selectivity} from an update state; last_weight is the
+    weight of the last window taken (the partial window's samples / nwind,
+    or 1)."""
+    av, mean, sec, var = state
+    ng = len(mean)
+    out = {}
+    for i in range(1, ng):
+        for j in range(i):
+            out[(j, i)] = Selectivity(mean[i][j], var[i][j], av - 1 + last_weight, sec[i][j])
+    return out
+
+
+def selectivity_from_samples(n_samples: Sequence[Sequence[float]], nwind: int = NWIND):
+    """The reference's selectivity from one node's production samples
+    (n_samples[k][g] = N of guest g at sample k): windows of nwind samples,
+    the last one partial (weight = its samples / nwind, taken at the last
+    move as gcmc.f:1887 does when the run ends)."""
+    n = len(n_samples)
+    ng = len(n_samples[0]) if n else 0
+    st = selectivity_state(ng)
+    last_w = 1.0
+    for a in range(0, n, nwind):
+        blk = n_samples[a:a + nwind]
+        w = len(blk) / nwind
+        means = [sum(float(r[g]) for r in blk) / len(blk) for g in range(ng)]
+        selectivity_update(st, means, w)
+        last_w = w
+    return selectivity_report(st, last_w)
+
+
+def selectivity_union(nodes: Sequence[dict]) -> dict:
+    """Replicas of one state point combined: means and second moments
+    weighted by each replica's window weight, the variance from the pooled
+    second moments (the reference prints node 0's only; identical replicas
+    give the replica)."""
+    out = {}
+    for key in nodes[0]:
+        sw = sum(nd[key].weight for nd in nodes)
+        if not sw:
+            out[key] = Selectivity(0.0, 0.0, 0.0, 0.0)
+            continue
+        m = sum(nd[key].weight * nd[key].mean for nd in nodes) / sw
+        s2 = sum(nd[key].weight * nd[key].second for nd in nodes) / sw
+        out[key] = Selectivity(m, s2 - m * m, sw, s2)
+    return out

@@ -182,6 +182,9 @@ typedef struct fmc_chain_params {
                          * inside the launches, every this many moves of
                          * the chain, so its cadence does not depend on how
                          * the moves are split into launches */
+  long long window;     /* production samples per selectivity window
+                         * (CONTROL "averaging window", readinputs.f:740-742;
+                         * 0 = 100000 as readinputs.f:888-889) */
 } fmc_chain_params;
 
 typedef struct fmc_chain_stats {
@@ -212,6 +215,18 @@ typedef struct fmc_chain_stats {
   long long win_len;
   int win_closed;
   double win[5][5][4];
+  /* selectivity of guest j over guest i, j < i (gcmc.f:1887-1916): over
+   * the closed windows of sel_window production samples, the running
+   * window-weighted mean of <N_j>/<N_i> (sel_mean), of its square
+   * (sel_second) and the variance the reference prints after +/-
+   * (sel_var), pair p = i (i - 1) / 2 + j; sel_count windows are closed,
+   * and sel_base holds the production sums of N when the last one closed
+   * (the open window's samples are sum_n - sel_base).  The open window
+   * enters at the end of the run with weight samples / sel_window. */
+  long long sel_window;
+  long long sel_count;
+  double sel_base[4];
+  double sel_mean[6], sel_second[6], sel_var[6];
 } fmc_chain_stats;
 
 /* max_chains chains; precision FMC_PREC_*.  */

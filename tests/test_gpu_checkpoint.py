@@ -274,3 +274,52 @@ def test_window_statistics_follow_the_move_log(fmc, case):
             assert (np.isnan(a) and np.isnan(w)) or abs(a - w) <= 1e-7 * max(1.0, abs(w))
         print(f"guest {g}: <N> {rep[g].avg[0]:.4f} +/- {rep[g].std[0]:.4f}, "
               f"Q_st {rep[g].avg[5]:.3f} +/- {rep[g].std[5]:.3f}")
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_selectivity_follows_the_move_log(fmc, prec):
+    """The kernel's selectivity windows (gcmc.f:1887-1916; 120-sample
+    windows, closed in the kernel across launch boundaries and the prepared
+    rejections) and Batch.report's last, partial window equal the
+    restatement applied to the loadings rebuilt from the move log, to 1e-12
+    relative; a checkpoint carries the window state."""
+    from fastmc_amd import stats
+    A, P, seed, freq = _cases()[1]
+    ctx = fmc.Context.from_arrays(A)
+    M, Q, W = 1000, 150, 120
+    b = _batch(2, prec)
+    c = b.add_chain(ctx, 298.0, P, seed, freq=freq, delr=A["delr"], eq_steps=Q,
+                    max_moves=M, window=W)
+    st0 = b.stats(c)
+    b.enable_log(c, M)
+    for n in (300, 300, 400):
+        b.run(n)
+    st = b.stats(c)
+    log = b.get_log(c, M)
+    rep = b.report(c, 298.0)
+    assert st["sel_window"] == W and st["sel_count"] == (M - Q) // W == 7
+    nm = list(st0["nmol"])
+    samples = []
+    for i in range(M):
+        nm[int(log[i, 1])] = int(log[i, 3])
+        if i >= Q:
+            samples.append([float(v) for v in nm])
+    want = stats.selectivity_from_samples(samples, nwind=W)
+    got = rep.selectivity
+    assert sorted(got) == sorted(want) == [(0, 1)]
+    g, w = got[(0, 1)], want[(0, 1)]
+    print(f"\nprec {prec}: S(1/2) = {g.mean:.9f} +/- {g.var:.6f} (restatement "
+          f"{w.mean:.9f} +/- {w.var:.6f}), {st['sel_count']} windows + {M - Q - 7 * W} samples")
+    assert abs(g.mean - w.mean) <= 1e-12 * abs(w.mean)
+    assert abs(g.var - w.var) <= 1e-10 * max(abs(w.var), 1e-6)
+    assert abs(g.second - w.second) <= 1e-12 * abs(w.second)
+    assert g.mean > 0
+    # the state travels in a checkpoint
+    blob = b.checkpoint(c)
+    b2 = _batch(1, prec)
+    c2 = b2.restore_chain(ctx, blob)
+    st2 = b2.stats(c2)
+    for k in ("sel_window", "sel_count", "sel_base", "sel_mean", "sel_second", "sel_var"):
+        assert st2[k] == st[k], k
+    b2.close()
+    b.close()

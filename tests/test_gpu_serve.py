@@ -6,7 +6,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 KEYS = ("steps", "nmol", "energy", "accepted", "attempted", "sum_n", "sum_e",
-        "e_acc", "e_full", "win")
+        "e_acc", "e_full", "win", "sel_count", "sel_base", "sel_mean", "sel_var")
 
 
 def test_serve_same_alone_and_in_mix():
@@ -21,10 +21,16 @@ def test_serve_same_alone_and_in_mix():
     assert all(r.done >= r.admitted >= 0.0 and r.latency > 0 for r in res)
     # with 4 slots and 6 requests some waited for a slot
     assert max(r.admitted - r.arrival for r in res) > 0.0
-    # every result carries the end-of-run report of each guest type
+    # every result carries the end-of-run report of each guest type and,
+    # with several guest types, the selectivity of each pair
+    multi = 0
     for r in res:
-        assert len(r.report) == len(r.stats["nmol"])
+        ng = len(r.stats["nmol"])
+        assert len(r.report) == ng
         assert r.report[0].weight == 700 and r.report[0].avg[0] >= 0.0
+        assert sorted(r.selectivity) == [(j, i) for i in range(ng) for j in range(i)]
+        multi += ng > 1
+    assert multi >= 1
     srv.close()
     for k in (0, 3, 5):
         one = serve.Server(slots=1, slice_moves=300)

@@ -375,3 +375,81 @@ def test_chain_energy_bookkeeping(oracle):
     e = s.energy_full()
     sp = (e[1] + e[2] + e[3] + e[4] + e[5]) / 418.4
     assert abs(ch.energy(0) - sp) < 1e-8 * max(1.0, abs(sp))
+
+
+def test_reciprocal_change_rounding_on_the_large_cell(oracle):
+    """How far FP64 can pin a displacement's reciprocal energy change on the
+    3392-atom 51.6 A cell at kmax 12 (Nk 4192).  The exact change, summed
+    in 64-bit-mantissa extended precision, is compared with (a) the
+    oracle, which follows the reference's term-by-term form a_k (|S_new|^2 -
+    |S_old|^2) (ewald.f:311-319), and (b) the cancellation-free form a_k (2
+    S.dS + |dS|^2) in FP64 that the HIP kernels use.  The reference's form
+    cancels |S|^2 terms of the framework's structure factor, so its
+    rounding error reaches ~1e-11 of the component; the cancellation-free
+    form stays near 1e-14.  A GPU-vs-oracle comparison on this cell is
+    therefore bounded by the oracle's own rounding, relative to the
+    components rather than to a small estep."""
+    from fastmc_amd import synth
+    w = synth.c2()
+    A = w.system_arrays()
+    s = oracle.OracleSystem(A["imcon"], A["cell"], A["alpha"], A["kmax"], A["rcut"],
+                            A["delr"], A["ntpatm"], A["vdw"], A["fw"], A["guests"])
+    L = float(A["cell"][0])
+    km = 12
+    # the half space and spherical cut of ewald.f:81-84,141-213 on a cubic cell
+    lmn = [(l, m, n) for l in range(0, km + 1) for m in range(-km, km + 1)
+           for n in range(-km, km + 1)
+           if (l > 0 or m > 0 or (m == 0 and n > 0)) and l * l + m * m + n * n <= (1.05 * km) ** 2]
+    lmn = np.array(lmn, dtype=np.longdouble)
+    assert len(lmn) == s.nk == 4192
+    ld = np.longdouble
+    twopi = 2 * ld(np.pi)
+    k2 = (twopi / ld(L)) ** 2 * (lmn ** 2).sum(axis=1)
+    alpha = ld(A["alpha"])
+    ak = np.exp(-k2 / (4 * alpha * alpha)) / k2
+
+    def sfac(x, y, z, q):
+        f = np.stack([np.asarray(x, ld), np.asarray(y, ld), np.asarray(z, ld)], axis=1) / ld(L)
+        re = np.zeros(len(lmn), ld)
+        im = np.zeros(len(lmn), ld)
+        for a in range(0, len(f), 256):
+            ph = twopi * (f[a:a + 256] @ lmn.T)
+            qq = np.asarray(q[a:a + 256], ld)[:, None]
+            re += (qq * np.cos(ph)).sum(axis=0)
+            im += (qq * np.sin(ph)).sum(axis=0)
+        return re, im
+
+    fw = A["fw"]
+    Sre, Sim = sfac(fw["x"], fw["y"], fw["z"], fw["q"])
+    g = A["guests"][0]
+    q = np.asarray(g["q"])
+    factor = ld(2.0) * (twopi / ld(L) ** 3) * ld(138935.4835)
+    rng = np.random.default_rng(5)
+    worst_ref = worst_cf = 0.0
+    for t in range(6):
+        p0 = rng.uniform(2.0, L - 2.0, 3)
+        old = [np.asarray(g[c]) + p0[i] for i, c in enumerate("xyz")]
+        new = [o + rng.uniform(-0.3, 0.3) for o in old]
+        s.trial(1, 0, -1, *old)
+        s.commit()
+        e_orc = s.trial(3, 0, 0, *new)[0]
+        s.trial(2, 0, 0)
+        s.commit()
+        so_re, so_im = sfac(*old, q)
+        sn_re, sn_im = sfac(*new, q)
+        o_re, o_im = Sre + so_re, Sim + so_im
+        n_re, n_im = o_re - so_re + sn_re, o_im - so_im + sn_im
+        exact = float(factor * (ak * ((n_re ** 2 + n_im ** 2) - (o_re ** 2 + o_im ** 2))).sum())
+        # the cancellation-free form in FP64
+        d_re = (sn_re - so_re).astype(np.float64)
+        d_im = (sn_im - so_im).astype(np.float64)
+        O_re, O_im = o_re.astype(np.float64), o_im.astype(np.float64)
+        cf = float(factor) * float(np.sum(ak.astype(np.float64) *
+                                          (2.0 * (O_re * d_re + O_im * d_im) + d_re * d_re + d_im * d_im)))
+        scale = max(abs(exact), 1.0)
+        worst_ref = max(worst_ref, abs(e_orc - exact) / scale)
+        worst_cf = max(worst_cf, abs(cf - exact) / scale)
+        print(f"\ntrial {t}: exact {exact:.12f}, oracle - exact {e_orc - exact:.2e}, "
+              f"cancellation-free - exact {cf - exact:.2e} (internal units)")
+    assert worst_cf < 1e-12
+    assert worst_ref < 1e-9

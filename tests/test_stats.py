@@ -86,9 +86,17 @@ def test_union_of_replicas():
 
 
 def _report(path):
-    """The final-statistics block of the driver's OUTPUT."""
+    """The final-statistics block of the driver's OUTPUT (the last guest's
+    block, and every selectivity pair as ("<S>", j, i))."""
     vals = {}
+    pair = None
     for line in open(path):
+        if line.startswith("selectivity stats for guest"):
+            j, i = line.split("guest")[1].split("/")
+            pair = ("<S>", int(j), int(i))
+        if line.strip().startswith("<S>:") and pair:
+            a, b = line.split("<S>:")[1].split("+/-")
+            vals[pair] = (float(a), float(b))
         for key in ("<N>:", "<E>:", "<E*N>:", "<N*N>:", "<E*E>:"):
             if line.strip().startswith(key):
                 a, b = line.split(key)[1].split("+/-")
@@ -138,3 +146,112 @@ def test_driver_report_matches_the_restatement(oracle, tmp_path):
         else:
             assert abs(got[key] - nd.std[z]) <= 6e-7, key
     assert nd.std[0] > 0 and 0 < nd.avg[0]
+
+
+# ---------------------------------------------------------------------------
+# selectivity (gcmc.f:1887-1916)
+# ---------------------------------------------------------------------------
+
+def test_selectivity_of_loadings_in_constant_ratio():
+    """Two guests whose loadings keep the ratio 3:1 in every sample report
+    S(1/2) = 3 with zero variance, over full windows and a partial last one
+    (its weight only scales a zero change)."""
+    from fastmc_amd import stats
+    rng = np.random.default_rng(2)
+    k = rng.integers(1, 20, 2750).astype(float)
+    samples = np.stack([3 * k, k], axis=1)
+    sel = stats.selectivity_from_samples(samples, nwind=500)
+    s = sel[(0, 1)]
+    assert abs(s.mean - 3.0) < 1e-14
+    assert abs(s.var) < 1e-12
+    assert abs(s.weight - (5 + 250 / 500)) < 1e-15
+
+
+def test_selectivity_of_a_run_shorter_than_one_window():
+    """The reference takes a run shorter than its averaging window as one
+    partial window of weight samples / nwind, and its running mean moves
+    from zero by that weight: the printed <S> is the ratio times the
+    fraction of the window filled."""
+    from fastmc_amd import stats
+    samples = [[6.0, 2.0]] * 800
+    s = stats.selectivity_from_samples(samples, nwind=100000)[(0, 1)]
+    assert abs(s.mean - 3.0 * 800 / 100000) < 1e-15
+    assert abs(s.var - (9.0 - s.mean ** 2)) < 1e-12
+
+
+def test_selectivity_windows_against_numpy():
+    """Full windows only: the running update is the plain mean of the window
+    ratios <N_j>/<N_i> and the printed variance is their population
+    variance; three guests give the pairs 1/2, 1/3, 2/3."""
+    from fastmc_amd import stats
+    rng = np.random.default_rng(8)
+    nwind, nw = 400, 7
+    samples = rng.integers(1, 30, (nwind * nw, 3)).astype(float)
+    sel = stats.selectivity_from_samples(samples, nwind=nwind)
+    means = samples.reshape(nw, nwind, 3).mean(axis=1)
+    for (j, i), s in sel.items():
+        r = means[:, j] / means[:, i]
+        assert abs(s.mean - r.mean()) < 1e-13 * r.mean()
+        assert abs(s.var - r.var()) < 1e-10 * max(r.var(), 1e-12)
+    assert sorted(sel) == [(0, 1), (0, 2), (1, 2)]
+
+
+def test_selectivity_skips_windows_without_guest_i():
+    """A window whose <N_i> is zero updates no pair of guest i but counts
+    (avcount), so the next window's change is divided by the larger count:
+    windows with ratios 2, (skipped), 4 give 2 + (4 - 2) / 3."""
+    from fastmc_amd import stats
+    w1 = [[4.0, 2.0]] * 10
+    w2 = [[5.0, 0.0]] * 10
+    w3 = [[8.0, 2.0]] * 10
+    s = stats.selectivity_from_samples(w1 + w2 + w3, nwind=10)[(0, 1)]
+    assert abs(s.mean - (2.0 + 2.0 / 3.0)) < 1e-15
+    sec = 4.0 + (16.0 - 4.0) / 3.0        # selwin after window 3
+    assert abs(s.var - (sec - s.mean ** 2)) < 1e-13
+
+
+def test_selectivity_union_of_identical_replicas_is_the_replica():
+    from fastmc_amd import stats
+    rng = np.random.default_rng(4)
+    samples = rng.integers(1, 25, (3300, 2)).astype(float)
+    one = stats.selectivity_from_samples(samples, nwind=600)
+    u = stats.selectivity_union([one, one, one])
+    for key in one:
+        assert abs(u[key].mean - one[key].mean) < 1e-15
+        assert abs(u[key].var - (one[key].second - one[key].mean ** 2)) < 1e-13
+        assert abs(u[key].weight - 3 * one[key].weight) < 1e-12
+
+
+def test_driver_selectivity_matches_the_restatement(oracle, tmp_path):
+    """The driver (oracle ABI) on a two-guest system with an averaging
+    window of 300 samples: its printed selectivity equals the restatement
+    applied to the loadings of its move trace, to the printed digits."""
+    import os
+    import subprocess
+    from fastmc_amd import stats, synth
+    from conftest import ORACLE
+    exe = os.path.join(ORACLE, "_build", "fmc_gcmc_oracle")
+    if not os.path.exists(exe):
+        subprocess.check_call(["make", "-s", "-C", ORACLE])
+    rng = np.random.default_rng(21)
+    fw = synth.make_framework(rng, (26.0, 26.0, 26.0), 260)
+    w = synth.Workload("mix", fw, ["co2", "n2"], [4.0, 8.0], 298.0, (8, 8, 8))
+    w.write(str(tmp_path), steps=2000, eqsteps=0)
+    p = tmp_path / "CONTROL"
+    p.write_text(p.read_text().replace(
+        "finish", "averaging window 300\nseed 12 34 56 78\ntrace\nfinish"))
+    r = subprocess.run([exe], cwd=str(tmp_path), capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    tr = np.loadtxt(tmp_path / "branch01" / "trace.out", ndmin=2)
+    n = np.zeros(2)
+    samples = []
+    for row in tr:
+        n[int(row[1]) - 1] = row[4]
+        samples.append(n.copy())
+    assert len(samples) == 2400 and min(s[1] for s in samples[300:]) >= 0
+    sel = stats.selectivity_from_samples(samples, nwind=300)[(0, 1)]
+    got = _report(tmp_path / "OUTPUT")[("<S>", 1, 2)]
+    print(f"\ndriver <S> {got}, restatement {sel.mean:.6f} +/- {sel.var:.3f}")
+    assert abs(got[0] - sel.mean) <= 6e-7
+    assert abs(got[1] - sel.var) <= 6e-4
+    assert sel.mean > 0
