@@ -406,7 +406,7 @@ def main():
     drift = max(abs(a["drift"]) for a in after)
     recomputes = sum(a["recomputes"] - z["recomputes"] for a, z in zip(after, before))
     nmol = [a["nmol"][0] for a in after]
-    tile = b.tile()
+    tile = b.tile_shape()
     b.close()
 
     plain = None if args.no_plain else plain_fp64_rate()
@@ -436,7 +436,9 @@ def main():
                    "chains": n_chains, "step_budget_ms": args.budget_ms,
                    "moves_per_chain_per_step_mean": total_moves / n_chains / args.steps,
                    "recomputes_in_timed_region": recomputes,
-                   "tile": tile,
+                   "tile": {"threads": tile["threads"], "kpt": tile["kpt"],
+                            "kgroup": tile["kgroup"], "kunroll": tile["kunroll"],
+                            "chosen_by": {2: "autotuner", 1: "fixed", 0: "none"}[tile["tuned"]]},
                    "guest_molecules_mean": sum(nmol) / len(nmol),
                    "guest_molecules_max": max(nmol)},
         "plain_fp64_moves_per_s": plain,

@@ -1,31 +1,10 @@
-// Batched device-resident chains: many independent copies of the reference's
-// move loop (gcmc.f:589-1960) advance inside one persistent launch, one
-// chain per work-group.  Nothing returns to the host per move: duni, the
-// insert/delete/displace proposals, the energy of the trial, the acceptance
-// test and the commit all run on the GPU.
-//
-// Per chain, for the whole launch:
-//   * S(k) (FP64) and a_k sit in registers, KPT k-vectors per thread
-//     (k = tid + j*BT), so the reciprocal update reads no memory;
-//   * the trial's structure-factor change dS(k) stays in registers and is
-//     added to S(k) when the move is accepted;
-//   * per-position tables q X[l] Y[m] and Z[n] of the phase factors
-//     e^{i 2 pi l s} (ewald.f:97-137) sit in LDS;
-//   * real space walks a per-framework candidate list: the framework is cut
-//     into ~1.5 A cells and each cell lists every atom that can lie within
-//     rcut of a point in it (a superset, so the pair set -- and therefore the
-//     energy -- is exactly that of the reference's all-atom scan), plus all
-//     guest atoms;
-//   * the serial part runs wave-wide (RNG draws and proposal: every lane
-//     the same scalar steps, lane i draw i, lane s site s) or on one lane
-//     (acceptance and bookkeeping).
-// Precision FP64 reproduces the oracle's accept/reject stream; MIXED keeps
-// the per-position phase-factor products q X[l] Y[m], Z[n] and the trial's
-// dS(k) in FP32 (a displacement's as the change of each site's term, formed
-// in FP64 before rounding), with FP64 coordinates, S(k), real-space pair
-// terms, energies and acceptance.
+// Batched device-resident chains, host side: admission, systems and their
+// candidate records, launches (eager, time-budgeted, graph-captured
+// cycles), the autotuner over the kernel's tile shapes, checkpoints,
+// statistics and the FP64 recompute.  The device side -- the chain
+// structures and the persistent kernel that runs the reference's move
+// loop (gcmc.f:589-1960), one chain per work-group -- is fmc_batch_dev.h.
 #include <hip/hip_runtime.h>
-
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -40,1598 +19,12 @@
 #include <vector>
 
 #include "fastmc_amd.h"
+#include "fmc_batch_dev.h"
 #include "fmc_ctx.h"
 #include "fmc_device.h"
 #include "fmc_host.h"
 
 namespace fmc {
-
-constexpr int kBTab = 17;   // phase-table entries per dimension (kmax <= 16)
-constexpr int kLogFields = 8;
-constexpr int kWin = 5;          // statistics windows (gcmc.f:119)
-constexpr int kWinQ = 5;         // sums per window: N, E, E*N, N^2, E^2
-constexpr int kWinSnap = 2 * kWin - 1;   // running-sum snapshots that bound them
-constexpr int kWinDoubles = kWinSnap * kWinQ * kMaxGuestTypes;
-constexpr int kWinMax = 500000;  // samples beyond which the reference keeps
-                                 // rolling averages instead (gcmc.f:1971)
-constexpr int kLdsTypes = 32;   // atom types whose pair rows sit in LDS
-constexpr int kSelPairs = kMaxGuestTypes * (kMaxGuestTypes - 1) / 2;
-constexpr long long kSelWindow = 100000;   // readinputs.f:888-889
-
-// The small per-system constants the serial lane and the per-move set-up
-// read; the batch kernel keeps a copy in LDS for the whole launch.
-struct SysQ {
-  double cell[9], rcell[9];
-  double epsq;
-  double rvolm2;          // 2 * 2 pi / V
-  double volm;
-  int kmax[3];
-  int gdim[3];
-  int ngt;
-  int ns[kMaxGuestTypes];
-  double gq[kMaxGuestTypes][kMaxSites];
-  int gtype[kMaxGuestTypes][kMaxSites];  // 0-based
-  double gmass[kMaxGuestTypes][kMaxSites];
-  double tx[kMaxGuestTypes][kMaxSites], ty[kMaxGuestTypes][kMaxSites],
-      tz[kMaxGuestTypes][kMaxSites];
-  double e3[kMaxGuestTypes];
-  double engsic[kMaxGuestTypes];
-  // tail(n) = t0 + sum_g tb[g] n_g + sum_gh tA[g][h] n_g n_h
-  double t0, tb[kMaxGuestTypes], tA[kMaxGuestTypes][kMaxGuestTypes];
-};
-
-struct alignas(16) Rec {
-  double x, y, z, q;
-};
-
-struct BSys {
-  DevSys S;
-  int nk;
-  const int* kv;
-  const double* ak;
-  const double* sfw_re;   // framework S(k) (constant)
-  const double* sfw_im;
-  // candidate cells: cell c lists, in ascending atom order, every framework
-  // atom that can lie within rcut of a point in it, as records
-  // rec[c*gcap + i] (x, y, z, q) and rtype[c*gcap + i]; unused slots hold
-  // NaN coordinates, which every pair test skips
-  const Rec* grec;
-  const int* grec_t;
-  int gcap;
-  SysQ q;
-};
-
-struct BChain {
-  int sys;
-  int ntpguest;
-  int cap_atoms;
-  // end-of-run windows (gcmc.f:1969-2031): window j sums the production
-  // samples j*L ... j*L + L (L = win_len), the difference of the running
-  // sums after j*L + L + 1 samples and after j*L samples.  The running sums
-  // are copied to the snapshot area that follows S(k) in the sre buffer
-  // (sre + nk: [snapshot][quantity][guest]) when the production count
-  // reaches L, L+1, 2L, 2L+1, 3L, 3L+1, 4L, 4L+1, 5L+1.
-  int win_len;    // 0: no windows (no move limit, or too many samples)
-  double* sre;
-  double* sim;
-  double* dsre;   // dS scratch for the streaming (large-Nk) kernel
-  double* dsim;
-  double* gx[kMaxGuestTypes];
-  double* gy[kMaxGuestTypes];
-  double* gz[kMaxGuestTypes];
-  double temp, beta, statvolm, engunit;
-  double press[kMaxGuestTypes];
-  double cum[kMaxGuestTypes][9];
-  double disp_ratio[kMaxGuestTypes], tran_ratio[kMaxGuestTypes], rota_ratio;
-  long long eq_steps;
-  // RNG (gcmc.f:2712-2805)
-  float u[97];
-  float c, cd, cm;
-  int ir, jr;
-  // state
-  int nmol[kMaxGuestTypes];
-  double energy[kMaxGuestTypes];
-  double elrc;
-  double delrdisp[kMaxGuestTypes], tran_delr[kMaxGuestTypes], rotangle,
-      rota_rotangle;
-  int disp_count[kMaxGuestTypes], accept_disp[kMaxGuestTypes];
-  int tran_count[kMaxGuestTypes], accept_tran[kMaxGuestTypes];
-  int rota_count, accept_rota;
-  long long steps, prod_steps;
-  long long accepted[6], attempted[6];
-  double sum_n[kMaxGuestTypes], sum_n2[kMaxGuestTypes], sum_e[kMaxGuestTypes];
-  double sum_en[kMaxGuestTypes], sum_e2[kMaxGuestTypes];
-  double* log;
-  long long log_cap, log_n;
-  int fault;   // 1: guest capacity reached (reference error 75)
-  int win_snap;   // snapshots taken
-  long long stop_at;   // the chain idles once steps reaches this
-  long long recomp_every;   // > 0: FP64 recompute inside the launch every this many moves
-  // FP64 full-energy bookkeeping: e_acc is the running sum of accepted
-  // estep from the full energy at admission; e_full the last full
-  // recompute (kcal/mol); drift = e_acc - e_full at that recompute
-  double e_acc, e_full, drift, sk_dev;
-  long long n_recompute;
-  // selectivity (gcmc.f:1887-1916): windows of sel_nwind production
-  // samples; pair p = i (i - 1) / 2 + j of guests j < i
-  long long sel_nwind, sel_count;
-  double sel_base[kMaxGuestTypes];   // sum_n when the last window closed
-  double sel_mean[kSelPairs], sel_sec[kSelPairs], sel_var[kSelPairs];
-};
-
-// ---- RNG on one lane (gcmc.f:2786-2799) ----
-struct Rng {
-  float* u;
-  float c, cd, cm;
-  int ir, jr;
-  __device__ double next() {
-    float uni = u[ir - 1] - u[jr - 1];
-    if (uni < 0.0f) uni = uni + 1.0f;
-    u[ir - 1] = uni;
-    ir = ir - 1;
-    if (ir == 0) ir = 97;
-    jr = jr - 1;
-    if (jr == 0) jr = 97;
-    c = c - cd;
-    if (c < 0.0f) c = c + cm;
-    uni = uni - c;
-    if (uni < 0.0f) uni = uni + 1.0f;
-    return static_cast<double>(uni);
-  }
-};
-
-// Lane j's value of a wave-uniform computation (j uniform).
-__device__ __forceinline__ float lane_f(float v, int j) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
-}
-__device__ __forceinline__ double lane_d(double v, int j) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b), j));
-  const unsigned hi =
-      static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), j));
-  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
-}
-__device__ __forceinline__ int uni_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// Draw i of the duni stream (gcmc.f:2786-2799) ahead of state R, on lane i
-// of a wave.  Draw i reads u(ir_i), u(jr_i) and overwrites u(ir_i); both
-// indices step down together from 97 and 33, so jr = ir + 33 (mod 97)
-// throughout and draw i reads an entry that another draw overwrites only 33
-// draws later: within kLook draws every read sees the table as the
-// sequential draws would.  c, cd, cm and the table entries are multiples of
-// 2^-24 below 1, so every float step is exact and c after i+1 steps is
-// (c - (i+1) cd) mod cm in integers of 2^-24.
-constexpr int kLook = 10;   // most draws one proposal takes
-struct Ahead {
-  int ir;    // u index of this lane's draw
-  float w;   // table entry the draw leaves
-  float c;   // c after the draw
-  float d;   // the draw
-};
-
-__device__ __forceinline__ Ahead rng_ahead(const Rng& R, int lane) {
-  Ahead A;
-  const int i = lane < kLook ? lane : kLook - 1;
-  int ir = R.ir - i;
-  if (ir <= 0) ir += 97;
-  int jr = R.jr - i;
-  if (jr <= 0) jr += 97;
-  A.ir = ir;
-  float w = R.u[ir - 1] - R.u[jr - 1];
-  if (w < 0.0f) w = w + 1.0f;
-  A.w = w;
-  const int cd = static_cast<int>(R.cd * 16777216.0f);
-  const int cm = static_cast<int>(R.cm * 16777216.0f);
-  int ci = (static_cast<int>(R.c * 16777216.0f) - (i + 1) * cd) % cm;
-  if (ci < 0) ci += cm;
-  A.c = static_cast<float>(ci) * (1.0f / 16777216.0f);
-  float d = w - A.c;
-  if (d < 0.0f) d = d + 1.0f;
-  A.d = d;
-  return A;
-}
-
-// the first n draws taken: their table entries written (one per lane) and
-// the state after them, on every lane
-__device__ __forceinline__ void rng_commit(Rng& R, const Ahead& A, int lane, int n) {
-  if (lane < n) R.u[A.ir - 1] = A.w;
-  if (n > 0) {
-    R.c = lane_f(A.c, n - 1);
-    R.ir = R.ir - n;
-    if (R.ir <= 0) R.ir += 97;
-    R.jr = R.jr - n;
-    if (R.jr <= 0) R.jr += 97;
-  }
-}
-
-// rotation matrix of rotation (utility_pack.f:2870-2906) from the
-// quaternion built as random_ins / random_rot do
-__device__ __forceinline__ void rot_matrix(double theta, double r2, double r3, double r4,
-                                           double (&Q)[9]) {
-  const double b1 = 2.0 * r2 - 1.0, b2 = 2.0 * r3 - 1.0, b3 = 2.0 * r4 - 1.0;
-  double st, q1;
-  sincos(theta / 2.0, &st, &q1);
-  const double nrm = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
-  const double q2 = st * (b1 / nrm), q3 = st * (b2 / nrm), q4 = st * (b3 / nrm);
-  Q[0] = 2.0 * q2 * q2 + 2.0 * q1 * q1 - 1.0;
-  Q[1] = 2.0 * q2 * q3 - 2.0 * q1 * q4;
-  Q[2] = 2.0 * q2 * q4 + 2.0 * q1 * q3;
-  Q[3] = 2.0 * q2 * q3 + 2.0 * q1 * q4;
-  Q[4] = 2.0 * q1 * q1 + 2.0 * q3 * q3 - 1.0;
-  Q[5] = 2.0 * q3 * q4 - 2.0 * q1 * q2;
-  Q[6] = 2.0 * q2 * q4 - 2.0 * q1 * q3;
-  Q[7] = 2.0 * q3 * q4 + 2.0 * q1 * q2;
-  Q[8] = 2.0 * q4 * q4 + 2.0 * q1 * q1 - 1.0;
-}
-
-__device__ __forceinline__ void rot_apply(const double (&Q)[9], double& x, double& y,
-                                          double& z) {
-  const double tx = Q[0] * x + Q[1] * y + Q[2] * z;
-  const double ty = Q[3] * x + Q[4] * y + Q[5] * z;
-  const double tz = Q[6] * x + Q[7] * y + Q[8] * z;
-  x = tx;
-  y = ty;
-  z = tz;
-}
-
-// energy_eval (utility_pack.f:3049-3095)
-__device__ bool accept_test(double eng, double rande, double volm,
-                            double press, int ngsts, double temp, double beta,
-                            int kind) {
-  double test = 0.0;
-  if (-1.0 * beta * eng > 700.0) {
-    test = exp(700.0);
-  } else if (kind == FMC_DISPLACE) {
-    test = exp(-1.0 * beta * eng);
-  } else if (kind == FMC_INSERT) {
-    test = volm * press / (static_cast<double>(ngsts + 1) * kBoltz * temp) *
-           exp(-1.0 * beta * eng);
-  } else if (kind == FMC_DELETE) {
-    if (fabs(press) < 1e-7)
-      test = 1.0;
-    else
-      test = static_cast<double>(ngsts) * kBoltz * temp / (volm * press) *
-             exp(-1.0 * beta * eng);
-  }
-  return rande < test;
-}
-
-__device__ double tail_of(const SysQ& Z, const int* n) {
-  double t = Z.t0;
-  for (int g = 0; g < Z.ngt; ++g) t += Z.tb[g] * n[g];
-  for (int g = 0; g < Z.ngt; ++g)
-    for (int h = 0; h < Z.ngt; ++h) t += Z.tA[g][h] * n[g] * n[h];
-  return t;
-}
-
-// The move as the serial lane hands it to the work-group.
-struct MoveDesc {
-  int kind;     // 0 no-op, FMC_INSERT/DELETE/DISPLACE
-  int sub;      // 0 plain, 4 translate-only, 5 rotate-only
-  int ig, imol, ns, slot;
-  int accepted;
-  double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
-  double nx[kMaxSites], ny[kMaxSites], nz[kMaxSites];
-  // flattened real-space work: position p = (set, site) of the trial
-  int np;                       // active positions (ns or 2 ns)
-  int nguest;                   // guest atoms of every type
-  int gpre[kMaxGuestTypes + 1]; // guest-type prefix over atoms
-  int pset[2 * kMaxSites], psite[2 * kMaxSites];
-  int plo[2 * kMaxSites];        // first candidate record of the cell
-  // per position: coordinates, q/epsq*r4pie0, site type, sign of its terms
-  double ppx[2 * kMaxSites], ppy[2 * kMaxSites], ppz[2 * kMaxSites];
-  double pce[2 * kMaxSites];
-  double pq[2 * kMaxSites];     // +q (new set) or -q (old set)
-  int pts[2 * kMaxSites], psg[2 * kMaxSites];
-  // pairtab row of each position's site type (ntpatm <= kLdsTypes)
-  short ptab[2 * kMaxSites][kLdsTypes];
-};
-
-__device__ __forceinline__ int cell_of(const SysQ& Z, double x, double y,
-                                       double z) {
-  const double* rc = Z.rcell;
-  int idx[3];
-  for (int d = 0; d < 3; ++d) {
-    double f = rc[d] * x + rc[3 + d] * y + rc[6 + d] * z;
-    f -= floor(f);
-    int i = static_cast<int>(f * Z.gdim[d]);
-    idx[d] = i >= Z.gdim[d] ? Z.gdim[d] - 1 : (i < 0 ? 0 : i);
-  }
-  return (idx[0] * Z.gdim[1] + idx[1]) * Z.gdim[2] + idx[2];
-}
-
-template <typename T>
-struct Cplx {
-  T x, y;
-};
-
-__device__ __forceinline__ float fmat(float a, float b, float c) {
-  return __builtin_fmaf(a, b, c);
-}
-__device__ __forceinline__ double fmat(double a, double b, double c) {
-  return __builtin_fma(a, b, c);
-}
-
-// dS(k) of the trial's positions: sum_p (q_p X_p[l] Y_p[m]) Z_p[n], with
-// q X Y tabulated per (l, m) and Z per signed n (conjugates folded in); four
-// multiply-adds per position, accumulated in T.  Both kernel forms call
-// this one routine (contraction off, fused multiply-adds written out), so
-// they round identically.
-constexpr int kZRow = 2 * kBTab;   // Z table row: n = -16..16
-template <typename T>
-__device__ __forceinline__ void trial_ds(int np, const Cplx<T>* xy, int nlm, int lm,
-                                         const Cplx<T>* zt, int zoff, T& dr, T& di) {
-#pragma clang fp contract(off)
-  T ar = 0, ai = 0;
-  for (int p = 0; p < np; ++p) {
-    const Cplx<T> a = xy[p * nlm + lm];
-    const Cplx<T> c = zt[p * kZRow + zoff];
-    ar = fmat(a.x, c.x, ar);
-    ar = fmat(-a.y, c.y, ar);
-    ai = fmat(a.y, c.x, ai);
-    ai = fmat(a.x, c.y, ai);
-  }
-  dr = ar;
-  di = ai;
-}
-
-// trial_ds for the k-vectors a thread owns, kpt of its KPT slots (kpt the
-// same on every thread), positions outer: the LDS reads of one position for
-// a group of four k issue together and wait once, rather than once per
-// (k, position).  Each k accumulates over the positions in the same order
-// as trial_ds, so the sums are the same.  A slot of a group beyond the
-// thread's k-vectors carries index 0 and its sum is never used.
-template <typename T, int KPT>
-__device__ __forceinline__ void trial_ds_tile(int np, const Cplx<T>* xy, int nlm,
-                                              const Cplx<T>* zt, int zoff, int kpt,
-                                              const int (&kv)[KPT], T (&dr)[KPT],
-                                              T (&di)[KPT]) {
-#pragma clang fp contract(off)
-  constexpr int G = 4;
-  constexpr int NG = (KPT + G - 1) / G;
-#pragma unroll
-  for (int j = 0; j < KPT; ++j) {
-    dr[j] = 0;
-    di[j] = 0;
-  }
-  for (int p = 0; p < np; ++p) {
-    const Cplx<T>* xp = xy + p * nlm;
-    const Cplx<T>* zp = zt + p * kZRow + zoff;
-#pragma unroll
-    for (int g = 0; g < NG; ++g) {
-      if (g * G < kpt) {
-#pragma unroll
-        for (int jj = 0; jj < G; ++jj) {
-          const int j = g * G + jj;
-          if (j >= KPT) break;
-          const Cplx<T> a = xp[kv[j] & 0xffff];
-          const Cplx<T> c = zp[static_cast<signed char>((kv[j] >> 16) & 0xff)];
-          dr[j] = fmat(a.x, c.x, dr[j]);
-          dr[j] = fmat(-a.y, c.y, dr[j]);
-          di[j] = fmat(a.y, c.x, di[j]);
-          di[j] = fmat(a.x, c.y, di[j]);
-        }
-      }
-    }
-  }
-}
-
-// a_k (|S + dS|^2 - |S|^2) = a_k (2 S.dS + |dS|^2): no cancellation of |S|^2
-__device__ __forceinline__ double de_k(double ak, double s0, double s1, double dr, double di) {
-#pragma clang fp contract(off)
-  const double sd = fmat(s0, dr, s1 * di);
-  const double dd = fmat(dr, dr, di * di);
-  return ak * fmat(2.0, sd, dd);
-}
-
-// a / d for 0 <= a < 2^22 and 0 < d < 2^11 from rd = 1/d: the float
-// quotient is within one of the true one, and one correction step fixes it
-// (a runtime integer division costs tens of instructions)
-__device__ __forceinline__ int div_rc(int a, int d, float rd) {
-  int q = static_cast<int>(static_cast<float>(a) * rd);
-  const int r = a - q * d;
-  q += (r >= d) - (r < 0);
-  return q;
-}
-
-__device__ __forceinline__ int ul(int v) { return static_cast<signed char>(v & 0xff); }
-__device__ __forceinline__ int um(int v) { return static_cast<signed char>((v >> 8) & 0xff); }
-__device__ __forceinline__ int un(int v) { return static_cast<signed char>((v >> 16) & 0xff); }
-
-// ---------------------------------------------------------------------------
-// The proposal: move selection, adaptive steps, draws and trial positions
-// (gcmc.f:745-847, 855-877, 1010-1060, 1176-1248; utility_pack.f:2545-3047)
-// ---------------------------------------------------------------------------
-// The proposal as one wave computes it: every lane runs the same scalar
-// steps on the same values (wave-uniform, so no lane waits on another), lane
-// i holds draw i of the lookahead and lane s works on guest site s.  Lane 0
-// writes the chain's counters and the descriptor's scalars.  The
-// arithmetic and the draw order are those of the reference's sequential
-// form; the centre of mass is summed over the sites in order.
-__device__ void propose(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D, int lane) {
-  const double delrmin = 0.1, minangle = kPi / 18.0;
-  const Ahead A = rng_ahead(R, lane);
-  // draw 0 picks the guest type when there is more than one; draw g0 + j is
-  // the j-th after it
-  const int g0 = uni_i(C.ntpguest > 1 ? 1 : 0);
-  int ig = 0;
-  if (g0) ig = uni_i(static_cast<int>(floor(static_cast<double>(lane_f(A.d, 0)) * C.ntpguest)));
-  const double randmov = lane_f(A.d, g0);
-  const double* cum = C.cum[ig];
-  int kind = 0, sub = 0;
-  if (randmov < cum[0]) kind = FMC_INSERT;
-  else if (randmov < cum[1]) kind = FMC_DELETE;
-  else if (randmov < cum[2]) kind = FMC_DISPLACE;
-  else if (randmov < cum[5]) kind = 0;        // jump (rejected at set-up), flex, swap
-  else if (randmov < cum[6]) { kind = FMC_DISPLACE; sub = 4; }
-  else if (randmov < cum[7]) { kind = FMC_DISPLACE; sub = 5; }
-  else if (randmov < cum[8]) kind = 0;
-  else kind = FMC_DISPLACE;
-  const int ns = uni_i(Z.ns[ig]);
-  const int nmols = uni_i(C.nmol[ig]);
-  if (lane == 0) {
-    if (nmols >= 1 && C.disp_count[ig] >= 1) {
-      if (C.disp_count[ig] % 100 == 0) {
-        if ((C.accept_disp[ig] / static_cast<double>(C.disp_count[ig])) > C.disp_ratio[ig])
-          C.delrdisp[ig] = C.delrdisp[ig] * 1.05;
-        else if (C.delrdisp[ig] > delrmin)
-          C.delrdisp[ig] = C.delrdisp[ig] * 0.95;
-      } else if (C.disp_count[ig] % 50 == 0) {
-        if ((C.accept_disp[ig] / static_cast<double>(C.disp_count[ig])) > C.disp_ratio[ig])
-          C.rotangle = C.rotangle * 1.05;
-        else if (C.rotangle > minangle)
-          C.rotangle = C.rotangle * 0.95;
-      }
-    }
-    if (nmols >= 1 && C.tran_count[ig] >= 1 && C.tran_count[ig] % 100 == 0) {
-      if ((C.accept_tran[ig] / static_cast<double>(C.tran_count[ig])) > C.tran_ratio[ig])
-        C.tran_delr[ig] = C.tran_delr[ig] * 1.05;
-      else if (C.tran_delr[ig] > delrmin)
-        C.tran_delr[ig] = C.tran_delr[ig] * 0.95;
-    }
-    if (nmols >= 1 && C.rota_count >= 1 && C.rota_count % 100 == 0) {
-      if ((C.accept_rota / static_cast<double>(C.rota_count)) > C.rota_ratio)
-        C.rota_rotangle = C.rota_rotangle * 1.05;
-      else if (C.rota_rotangle > minangle)
-        C.rota_rotangle = C.rota_rotangle * 0.95;
-    }
-  }
-  if (nmols == 0) { kind = FMC_INSERT; sub = 0; }
-  kind = uni_i(kind);
-  sub = uni_i(sub);
-  // guest atoms of every type (the capacity check and the descriptor)
-  int na = 0;
-  for (int g = 0; g < Z.ngt; ++g) {
-    if (lane == 0) D.gpre[g] = na;
-    na += C.nmol[g] * Z.ns[g];
-  }
-  na = uni_i(na);
-  if (lane == 0) {
-    D.gpre[Z.ngt] = na;
-    D.nguest = na;
-    D.kind = kind;
-    D.sub = sub;
-    D.ig = ig;
-    D.ns = ns;
-    D.imol = -1;
-    D.accepted = 0;
-  }
-  int taken = g0 + 1;   // draws taken (type, move)
-  const bool site = lane < ns;
-  if (kind == FMC_INSERT) {
-    if (na + ns >= C.cap_atoms) {
-      if (lane == 0) {
-        C.fault = 1;
-        D.kind = 0;
-        D.np = 0;
-      }
-      rng_commit(R, A, lane, taken);
-      return;
-    }
-    const double ra = lane_f(A.d, g0 + 1), rb = lane_f(A.d, g0 + 2), rc = lane_f(A.d, g0 + 3);
-    const double r1 = lane_f(A.d, g0 + 4), r2 = lane_f(A.d, g0 + 5),
-                 r3 = lane_f(A.d, g0 + 6), r4 = lane_f(A.d, g0 + 7);
-    taken += 7;
-    const double* cell = Z.cell;
-    const double xc = ra * cell[0] + rb * cell[3] + rc * cell[6];
-    const double yc = ra * cell[1] + rb * cell[4] + rc * cell[7];
-    const double zc = ra * cell[2] + rb * cell[5] + rc * cell[8];
-    double Q[9];
-    rot_matrix(r1 * 2.0 * kPi, r2, r3, r4, Q);
-    if (site) {
-      double x = Z.tx[ig][lane], y = Z.ty[ig][lane], z = Z.tz[ig][lane];
-      rot_apply(Q, x, y, z);
-      D.nx[lane] = x + xc;
-      D.ny[lane] = y + yc;
-      D.nz[lane] = z + zc;
-    }
-    if (lane == 0) D.slot = nmols * ns;
-  } else if (kind == FMC_DELETE) {
-    const int imol = uni_i(static_cast<int>(floor(static_cast<double>(lane_f(A.d, g0 + 1)) * nmols)));
-    taken += 1;
-    if (lane == 0) D.imol = imol;
-    if (site) {
-      D.ox[lane] = gload(C.gx[ig] + imol * ns + lane);
-      D.oy[lane] = gload(C.gy[ig] + imol * ns + lane);
-      D.oz[lane] = gload(C.gz[ig] + imol * ns + lane);
-    }
-  } else if (kind == FMC_DISPLACE) {
-    double ra, rb, rc, r1, r2, r3, r4, dd, da, rm;
-    if (sub == 4) {
-      if (lane == 0) C.tran_count[ig]++;
-      ra = lane_f(A.d, g0 + 1); rb = lane_f(A.d, g0 + 2); rc = lane_f(A.d, g0 + 3);
-      r1 = r2 = r3 = r4 = 0.0;
-      dd = C.tran_delr[ig]; da = 0.0;
-      rm = lane_f(A.d, g0 + 4);
-      taken += 4;
-    } else if (sub == 5) {
-      if (lane == 0) C.rota_count++;
-      ra = rb = rc = 0.5;
-      r1 = lane_f(A.d, g0 + 1); r2 = lane_f(A.d, g0 + 2);
-      r3 = lane_f(A.d, g0 + 3); r4 = lane_f(A.d, g0 + 4);
-      dd = 0.0; da = C.rota_rotangle;
-      rm = lane_f(A.d, g0 + 5);
-      taken += 5;
-    } else {
-      if (lane == 0) C.disp_count[ig]++;
-      ra = lane_f(A.d, g0 + 1); rb = lane_f(A.d, g0 + 2); rc = lane_f(A.d, g0 + 3);
-      r1 = lane_f(A.d, g0 + 4); r2 = lane_f(A.d, g0 + 5);
-      r3 = lane_f(A.d, g0 + 6); r4 = lane_f(A.d, g0 + 7);
-      dd = C.delrdisp[ig]; da = C.rotangle;
-      rm = lane_f(A.d, g0 + 8);
-      taken += 8;
-    }
-    const int imol = uni_i(static_cast<int>(floor(rm * nmols)));
-    if (lane == 0) D.imol = imol;
-    double x = 0.0, y = 0.0, z = 0.0, m = 0.0;
-    if (site) {
-      x = gload(C.gx[ig] + imol * ns + lane);
-      y = gload(C.gy[ig] + imol * ns + lane);
-      z = gload(C.gz[ig] + imol * ns + lane);
-      m = Z.gmass[ig][lane];
-      D.ox[lane] = x;
-      D.oy[lane] = y;
-      D.oz[lane] = z;
-    }
-    const double h = dd * 0.5;
-    const double ta = (2.0 * ra - 1.0) * h, tb = (2.0 * rb - 1.0) * h,
-                 tc = (2.0 * rc - 1.0) * h;
-    x = ta + x;
-    y = tb + y;
-    z = tc + z;
-    double w = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int s = 0; s < ns; ++s) {
-      const double ms = lane_d(m, s);
-      w = w + ms;
-      cx = cx + lane_d(x, s) * ms;
-      cy = cy + lane_d(y, s) * ms;
-      cz = cz + lane_d(z, s) * ms;
-    }
-    cx = cx / w; cy = cy / w; cz = cz / w;
-    const double* rcl = Z.rcell;
-    const double* cell = Z.cell;
-    const double sx = (rcl[0] * cx + rcl[3] * cy + rcl[6] * cz);
-    const double sy = (rcl[1] * cx + rcl[4] * cy + rcl[7] * cz);
-    const double sz = (rcl[2] * cx + rcl[5] * cy + rcl[8] * cz);
-    const double xs = sx - floor(sx), ys = sy - floor(sy), zs = sz - floor(sz);
-    const double X = (cell[0] * xs + cell[3] * ys + cell[6] * zs);
-    const double Yv = (cell[1] * xs + cell[4] * ys + cell[7] * zs);
-    const double Zv = (cell[2] * xs + cell[5] * ys + cell[8] * zs);
-    x = x + (X - cx);
-    y = y + (Yv - cy);
-    z = z + (Zv - cz);
-    x = x - X;
-    y = y - Yv;
-    z = z - Zv;
-    double Q[9];
-    rot_matrix(r1 * da, r2, r3, r4, Q);
-    rot_apply(Q, x, y, z);
-    if (site) {
-      D.nx[lane] = x + X;
-      D.ny[lane] = y + Yv;
-      D.nz[lane] = z + Zv;
-    }
-  }
-  // trial positions: the old sites, then the new ones
-  int np = 0;
-  if (kind != 0) {
-    const int nold = kind != FMC_INSERT ? ns : 0;
-    np = nold + (kind != FMC_DELETE ? ns : 0);
-    if (lane < np) {
-      D.pset[lane] = lane < nold ? 0 : 1;
-      D.psite[lane] = lane < nold ? lane : lane - nold;
-    }
-  }
-  if (lane == 0) D.np = np;
-  rng_commit(R, A, lane, taken);
-}
-
-// Production count at which snapshot k of the running sums is taken.
-__host__ __device__ inline long long win_target(int k, int L) {
-  return k < 2 * kWin - 2 ? static_cast<long long>(k / 2 + 1) * L + (k & 1)
-                          : static_cast<long long>(kWin) * L + 1;
-}
-
-__device__ void window_snapshot(BChain& C, double* snaps) {
-  while (C.win_snap < kWinSnap && C.prod_steps == win_target(C.win_snap, C.win_len)) {
-    double* w = snaps + C.win_snap * kWinQ * kMaxGuestTypes;
-    for (int g = 0; g < kMaxGuestTypes; ++g) {
-      gstore(w + 0 * kMaxGuestTypes + g, C.sum_n[g]);
-      gstore(w + 1 * kMaxGuestTypes + g, C.sum_e[g]);
-      gstore(w + 2 * kMaxGuestTypes + g, C.sum_en[g]);
-      gstore(w + 3 * kMaxGuestTypes + g, C.sum_n2[g]);
-      gstore(w + 4 * kMaxGuestTypes + g, C.sum_e2[g]);
-    }
-    C.win_snap++;
-  }
-}
-
-// A selectivity window closes (gcmc.f:1887-1916 with weight 1): its <N_g>
-// are the production sums since the last close over the window's samples;
-// every closed window counts, a pair whose <N_i> is zero is not updated.
-__device__ __forceinline__ void selectivity_window(BChain& C) {
-#pragma clang fp contract(off)
-  C.sel_count++;
-  const double av = static_cast<double>(C.sel_count);
-  const double nw = static_cast<double>(C.sel_nwind);
-  const double weight = 1.0;
-  for (int i = 1; i < C.ntpguest; ++i) {
-    const double aN = (C.sum_n[i] - C.sel_base[i]) / nw;
-    if (aN == 0.0) continue;
-    for (int j = 0; j < i; ++j) {
-      const int p = i * (i - 1) / 2 + j;
-      const double sij = ((C.sum_n[j] - C.sel_base[j]) / nw) / aN;
-      const double delta = weight * (sij - C.sel_mean[p]);
-      C.sel_mean[p] = C.sel_mean[p] + delta / av;
-      C.sel_sec[p] = C.sel_sec[p] + weight * (sij * sij - C.sel_sec[p]) / (av - 1.0 + weight);
-      const double w = av - 1.0 + weight;
-      C.sel_var[p] = (w * C.sel_sec[p] - (w * C.sel_mean[p] * C.sel_mean[p])) / w;
-    }
-  }
-  for (int g = 0; g < C.ntpguest; ++g) C.sel_base[g] = C.sum_n[g];
-}
-
-// Production statistics of one move (gcmc.f:1814-1848), after C.steps has
-// counted it; true when it is a production sample.
-__device__ __forceinline__ bool production_sample(BChain& C) {
-  if (C.steps <= C.eq_steps) return false;
-  C.prod_steps++;
-  for (int g = 0; g < C.ntpguest; ++g) {
-    const double n = C.nmol[g];
-    const double e = C.energy[g];
-    C.sum_n[g] += n;
-    C.sum_n2[g] += n * n;
-    C.sum_e[g] += e;
-    C.sum_en[g] += e * n;
-    C.sum_e2[g] += e * e;
-  }
-  if (C.ntpguest > 1 && C.prod_steps % C.sel_nwind == 0) selectivity_window(C);
-  return true;
-}
-
-__device__ __forceinline__ void log_move(BChain& C, int slot, int ig, int acc, double estep,
-                                         double e1, double e2, double ev) {
-  if (C.log && C.log_n < C.log_cap) {
-    double* r = C.log + kLogFields * C.log_n;
-    r[0] = slot;
-    r[1] = ig;
-    r[2] = acc;
-    r[3] = C.nmol[ig];
-    r[4] = estep;
-    r[5] = e1;
-    r[6] = e2;
-    r[7] = ev;
-    C.log_n++;
-  }
-}
-
-__device__ __forceinline__ int move_slot(const MoveDesc& D) {
-  return D.kind == 0 ? 0 : (D.sub ? D.sub : D.kind);
-}
-
-// The decision alone (decide's arithmetic, gcmc.f:938-949, 1106-1116,
-// 1326-1338), with the acceptance draw given: estep, the logged e1 and
-// whether the move is accepted.  delrc is the tail change of an insert or
-// delete.
-__device__ __forceinline__ bool decision(const SysQ& Z, const BChain& C, const MoveDesc& D,
-                                         double e1, double e2, double ev, double delrc,
-                                         double rande, double& estep, double& e1out) {
-  const int ig = D.ig;
-  const int nmols = C.nmol[ig];
-  bool acc = false;
-  estep = 0.0;
-  if (D.kind == FMC_INSERT || D.kind == FMC_DELETE) {
-    if (D.kind == FMC_INSERT) {
-      e1 = e1 + Z.engsic[ig];
-      estep = (e1 + e2 + Z.e3[ig] + ev + delrc) / C.engunit;
-    } else {
-      e1 = e1 - Z.engsic[ig];
-      estep = (e1 - e2 - Z.e3[ig] - ev + delrc) / C.engunit;
-    }
-    acc = accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp, C.beta, D.kind);
-  } else if (D.kind == FMC_DISPLACE) {
-    estep = (e1 + e2 + ev) / C.engunit;
-    acc = estep < 0.0 ||
-          accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp, C.beta,
-                      FMC_DISPLACE);
-  }
-  e1out = e1;
-  return acc;
-}
-
-// A bound on estep above which the move is certainly rejected for the
-// acceptance draw rande: energy_eval accepts when rande < pref exp(-beta
-// estep) (pref 1 for a displacement, the volume-pressure factor for an
-// insert or delete), so estep > E* = -ln(rande / pref) / beta rejects.  The
-// bound adds a margin of 1e-6 (1 + |E*|), far beyond the rounding of exp and
-// log; a move at or below it is decided exactly.  Returns false when no
-// bound applies (rande 0, a tiny or zero prefactor, the zero-pressure
-// delete, a no-op).
-__device__ bool reject_bound(const BChain& C, const MoveDesc& D, double rande, double* thr) {
-  const int ig = D.ig;
-  double pref;
-  if (D.kind == FMC_DISPLACE) {
-    pref = 1.0;
-  } else if (D.kind == FMC_INSERT) {
-    pref = C.statvolm * C.press[ig] / (static_cast<double>(C.nmol[ig] + 1) * kBoltz * C.temp);
-  } else if (D.kind == FMC_DELETE) {
-    if (fabs(C.press[ig]) < 1e-7) return false;
-    pref = static_cast<double>(C.nmol[ig]) * kBoltz * C.temp / (C.statvolm * C.press[ig]);
-  } else {
-    return false;
-  }
-  if (!(rande > 0.0) || !(pref > 1e-300) || !(pref < 1e300)) return false;
-  const double e = -log(rande / pref) / C.beta;
-  *thr = e + 1e-6 * (1.0 + fabs(e));
-  return true;
-}
-
-__device__ __forceinline__ double tail_change(const SysQ& Z, const BChain& C, const MoveDesc& D) {
-  int n2[kMaxGuestTypes];
-  for (int g = 0; g < Z.ngt; ++g) n2[g] = C.nmol[g];
-  n2[D.ig] += (D.kind == FMC_INSERT) ? 1 : -1;
-  return tail_of(Z, n2) - C.elrc;
-}
-
-// Acceptance and bookkeeping on the serial lane (gcmc.f:938-1002, 1106-1172,
-// 1326-1378, 1814-1870).
-__device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
-                       double e1, double e2, double ev, int nk) {
-  const int ig = D.ig;
-  const int nmols = C.nmol[ig];
-  double estep = 0.0, delrc = 0.0;
-  bool acc = false;
-  const int slot = move_slot(D);
-  if (D.kind == FMC_INSERT || D.kind == FMC_DELETE) {
-    delrc = tail_change(Z, C, D);
-    if (D.kind == FMC_INSERT) {
-      e1 = e1 + Z.engsic[ig];
-      estep = (e1 + e2 + Z.e3[ig] + ev + delrc) / C.engunit;
-    } else {
-      e1 = e1 - Z.engsic[ig];
-      estep = (e1 - e2 - Z.e3[ig] - ev + delrc) / C.engunit;
-    }
-    const double rande = R.next();
-    acc = accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp,
-                      C.beta, D.kind);
-    if (acc) {
-      C.elrc = C.elrc + delrc;
-      C.nmol[ig] = nmols + (D.kind == FMC_INSERT ? 1 : -1);
-    }
-  } else if (D.kind == FMC_DISPLACE) {
-    estep = (e1 + e2 + ev) / C.engunit;
-    if (estep < 0.0) {
-      acc = true;
-    } else {
-      const double rande = R.next();
-      acc = accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp,
-                        C.beta, FMC_DISPLACE);
-    }
-    if (acc) {
-      if (D.sub == 4) C.accept_tran[ig]++;
-      else if (D.sub == 5) C.accept_rota++;
-      else C.accept_disp[ig]++;
-    }
-  }
-  if (acc) {
-    C.energy[ig] += estep;
-    C.e_acc += estep;
-  }
-  C.attempted[slot]++;
-  if (acc) C.accepted[slot]++;
-  D.accepted = acc ? 1 : 0;
-  C.steps++;
-  if (production_sample(C) && C.win_len > 0) window_snapshot(C, C.sre + nk);
-  log_move(C, slot, ig, D.accepted, estep, e1, e2, ev);
-}
-
-// Exact S(k) of a chain: framework part plus every guest atom, FP64 with
-// the phase reduced to [-1/2, 1/2] before sincospi (ewald.f:368-630 for the
-// guest atoms).  One code path for the recompute kernel and the batch
-// kernel's in-launch recompute, so both give the same bits.
-__device__ __forceinline__ void exact_sk(const BSys& Y, const SysQ& Z, const BChain& C,
-                                         int k, double& sx, double& sy) {
-  const int v = Y.kv[k];
-  const int l = ul(v), m = um(v), n = un(v);
-  const double* rc = Z.rcell;
-  sx = Y.sfw_re[k];
-  sy = Y.sfw_im[k];
-  for (int g = 0; g < Z.ngt; ++g) {
-    const int nat = C.nmol[g] * Z.ns[g];
-    for (int a = 0; a < nat; ++a) {
-      const double x = C.gx[g][a], y = C.gy[g][a], z = C.gz[g][a];
-      double ph = l * (rc[0] * x + rc[3] * y + rc[6] * z) +
-                  m * (rc[1] * x + rc[4] * y + rc[7] * z) +
-                  n * (rc[2] * x + rc[5] * y + rc[8] * z);
-      ph -= rint(ph);
-      double sv, cv;
-      sincospi(2.0 * ph, &sv, &cv);
-      const double q = Z.gq[g][a % Z.ns[g]];
-      sx += q * cv;
-      sy += q * sv;
-    }
-  }
-}
-
-// Real-space energy of all guests (single_point's pair part): guest atom i
-// against its cell's framework candidates and every later guest atom of
-// another molecule; threads tid, tid+nt, ... share the pairs of each i.
-__device__ __forceinline__ void guest_pairs(const BSys& Y, const SysQ& Z, const BChain& C,
-                                            int tid, int nt, double& e2, double& ev) {
-  int gpre[kMaxGuestTypes + 1];
-  int na = 0;
-  for (int g = 0; g < kMaxGuestTypes; ++g) {
-    gpre[g] = na;
-    if (g < Z.ngt) na += C.nmol[g] * Z.ns[g];
-  }
-  gpre[kMaxGuestTypes] = na;
-  for (int i = 0; i < na; ++i) {
-    int g = 0;
-    while (g + 1 < Z.ngt && gpre[g + 1] <= i) ++g;
-    const int la = i - gpre[g];
-    const int s = la % Z.ns[g];
-    const int mol = la / Z.ns[g];
-    const double xi = C.gx[g][la], yi = C.gy[g][la], zi = C.gz[g][la];
-    const double ce = Z.gq[g][s] / Z.epsq * kR4pie0;
-    const int ts = Z.gtype[g][s];
-    const int c = cell_of(Z, xi, yi, zi);
-    const int lo = c * Y.gcap;
-    for (int t = tid; t < Y.gcap; t += nt) {
-      const double* rr = &Y.grec[lo + t].x;
-      pair_fp64(Y.S, ce, ts, xi - gload(rr), yi - gload(rr + 1), zi - gload(rr + 2),
-                gload(rr + 3), gload(Y.grec_t + lo + t), e2, ev);
-    }
-    for (int j = i + 1 + tid; j < na; j += nt) {
-      int h = 0;
-      while (h + 1 < Z.ngt && gpre[h + 1] <= j) ++h;
-      const int lb = j - gpre[h];
-      if (h == g && lb / Z.ns[h] == mol) continue;
-      const int sb = lb % Z.ns[h];
-      pair_fp64(Y.S, ce, ts, xi - C.gx[h][lb], yi - C.gy[h][lb], zi - C.gz[h][lb],
-                Z.gq[h][sb], Z.gtype[h][sb], e2, ev);
-    }
-  }
-}
-
-// Thread 0's part of a recompute: the FP64 totals into the chain's
-// bookkeeping (see recompute_kernel).
-__device__ __forceinline__ void recompute_finish(const SysQ& Z, BChain& C, double erec,
-                                                 double e2, double ev, double dev, int base) {
-  double e = Z.rvolm2 * kR4pie0 * erec / Z.epsq;
-  for (int g = 0; g < Z.ngt; ++g) e += C.nmol[g] * (Z.engsic[g] + Z.e3[g]);
-  e += e2 + ev;
-  // the move loop's tail bookkeeping starts from elrc = 0 and adds
-  // delrc = tail(N +/- 1) - elrc (gcmc.f:891-901): compare the running
-  // sum with the same tail it carries, report the true one in e_full
-  const double book = (e + C.elrc) / C.engunit;
-  C.e_full = (e + tail_of(Z, C.nmol)) / C.engunit;
-  if (base) {
-    C.e_acc = book;
-    for (int g = 0; g < Z.ngt; ++g) C.energy[g] = C.e_full;
-  }
-  C.drift = C.e_acc - book;
-  C.sk_dev = dev;
-  C.n_recompute++;
-}
-
-// The per-move tables of a trial's positions, one position per wave (waves
-// w < nw of the caller's range take positions p = w, w + nw, ...): its
-// phase factors e^{i 2 pi l f} for every dimension and l (ewald.f:97-137,
-// one sincospi per lane, into the wave's own rows of tab), then, read back
-// within the wave, its q X[l] Y[m] row and signed Z[n] row for the
-// reciprocal sum, its pair-table row, and its coordinates, charge factors
-// and candidate-list start for real space.  Positions are independent, so
-// no work-group barrier is needed inside; the caller's next barrier
-// publishes the tables.
-//
-// A displacement in DELTA form (mixed precision) sums, per site, the change
-// of q e^{ik.r} rather than the new term minus the old one:
-//   q (P_n Z_n - P_o Z_o) = q P_o (Z_n - Z_o) + q (P_n - P_o) Z_n,
-// with P = X[l] Y[m].  The old position's rows are (q P_o, Z_n - Z_o), the
-// new position's (q (P_n - P_o), Z_n).  The differences are formed from
-// FP64 phase factors in FP64 and only then rounded to T, so their rounding
-// error is relative to the change of the term, not to the term: the FP32
-// difference of two nearly equal phase factors would lose the digits that
-// the framework's large |S(k)| amplifies in 2 a_k S.dS.  Each position's
-// wave computes the partner position's phase factors it needs into tabp.
-template <typename T, bool DELTA>
-__device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveDesc& D,
-                                             int wave, int nw, int lane,
-                                             Cplx<double> (*tab)[3][kBTab],
-                                             Cplx<double> (*tabp)[3][kBTab], Cplx<T>* xy_l,
-                                             Cplx<T>* zt, int nlm) {
-  const int np = D.np, ig = D.ig, kind = D.kind;
-  const int ntp = Y.S.ntpatm;
-  const int w2 = 2 * Z.kmax[1] + 1, k3 = Z.kmax[2], nz = 2 * k3 + 1;
-  const float rw2 = 1.0f / w2;
-  const bool delta = DELTA && kind == FMC_DISPLACE;
-  for (int p = wave; p < np; p += nw) {
-    const int set = D.pset[p], s = D.psite[p];
-    const double x = set ? D.nx[s] : D.ox[s];
-    const double y = set ? D.ny[s] : D.oy[s];
-    const double z = set ? D.nz[s] : D.oz[s];
-    const double q = Z.gq[ig][s];
-    const double pq = (set || delta) ? q : -q;
-    const int ts = Z.gtype[ig][s];
-    // one sincospi per lane: first the position's own phase factors (kmax+1
-    // per dimension, packed), then, in delta form, the partner position's
-    // that the rows below need (the new Z for the old position, the old X
-    // and Y for the new one); both fit one pass of the wave while the
-    // entries number at most 64
-    const int k0 = Z.kmax[0] + 1, k1 = Z.kmax[1] + 1, k2 = Z.kmax[2] + 1;
-    const int nown = k0 + k1 + k2;
-    const int d0 = set ? 0 : 2;   // the partner's first dimension
-    const int npart = delta ? (set ? k0 + k1 : k2) : 0;
-    for (int j = lane; j < nown + npart; j += kWave) {
-      const bool own = j < nown;
-      int d, l;
-      if (own) {
-        d = j < k0 ? 0 : (j < k0 + k1 ? 1 : 2);
-        l = j - (d == 0 ? 0 : (d == 1 ? k0 : k0 + k1));
-      } else {
-        const int jj = j - nown;
-        d = d0 + (d0 == 0 && jj >= k0 ? 1 : 0);
-        l = jj - (d == 1 ? k0 : 0);
-      }
-      const double px = own ? x : (set ? D.ox[s] : D.nx[s]);
-      const double py = own ? y : (set ? D.oy[s] : D.ny[s]);
-      const double pz = own ? z : (set ? D.oz[s] : D.nz[s]);
-      double f = Z.rcell[d] * px + Z.rcell[3 + d] * py + Z.rcell[6 + d] * pz;
-      f = f - floor(f);
-      double ph = l * f;
-      ph = ph - rint(ph);
-      double sv, cv;
-      sincospi(2.0 * ph, &sv, &cv);
-      if (!DELTA || own)
-        tab[p][d][l] = {cv, sv};
-      else
-        tabp[p][d][l] = {cv, sv};
-    }
-    if (lane == 0) {
-      D.plo[p] = cell_of(Z, x, y, z) * Y.gcap;
-      D.ppx[p] = x;
-      D.ppy[p] = y;
-      D.ppz[p] = z;
-      D.pce[p] = q / Z.epsq * kR4pie0;
-      D.pq[p] = pq;
-      D.pts[p] = ts;
-      D.psg[p] = (kind == FMC_DISPLACE && set == 0) ? -1 : 1;
-    }
-    if (ntp <= kLdsTypes)
-      for (int tj = lane; tj < ntp; tj += kWave)
-        D.ptab[p][tj] = static_cast<short>(Y.S.pairtab[ts * ntp + tj]);
-    // the phase factors of this position, written by this wave, read back
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // X[l] * Y[m] for every (l, m) (m < 0 conjugates Y); in delta form the
-    // new position's row is X_n Y_n - X_o Y_o
-    Cplx<T>* xr = xy_l + p * nlm;
-    const bool dxy = delta && set;
-    for (int r = lane; r < nlm; r += kWave) {
-      const int l = div_rc(r, w2, rw2), m = r - l * w2 - Z.kmax[1];
-      const int am = m < 0 ? -m : m;
-      const Cplx<double> a = tab[p][0][l];
-      const Cplx<double> b = tab[p][1][am];
-      double ox, oy;
-      if (m >= 0) {
-        ox = a.x * b.x - a.y * b.y;
-        oy = a.y * b.x + b.y * a.x;
-      } else {
-        ox = a.x * b.x + a.y * b.y;
-        oy = a.y * b.x - b.y * a.x;
-      }
-      if (dxy) {
-        const Cplx<double> c = tabp[p][0][l];
-        const Cplx<double> e = tabp[p][1][am];
-        if (m >= 0) {
-          ox -= c.x * e.x - c.y * e.y;
-          oy -= c.y * e.x + e.y * c.x;
-        } else {
-          ox -= c.x * e.x + c.y * e.y;
-          oy -= c.y * e.x - e.y * c.x;
-        }
-      }
-      xr[r] = {static_cast<T>(pq * ox), static_cast<T>(pq * oy)};
-    }
-    // signed Z[n]; in delta form the old position's row is Z_n - Z_o
-    const bool dz = delta && !set;
-    for (int i = lane; i < nz; i += kWave) {
-      const int n = i - k3;
-      const int an = n < 0 ? -n : n;
-      Cplx<double> c = tab[p][2][an];
-      if (dz) {
-        const Cplx<double> e = tabp[p][2][an];
-        c = {e.x - c.x, e.y - c.y};
-      }
-      zt[p * kZRow + i] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// The persistent kernel
-// ---------------------------------------------------------------------------
-// STREAM: S(k), a_k and the k indices stay in global memory (L2-resident)
-// and dS(k) goes to the chain's scratch; for Nk beyond what registers and
-// LDS hold (kmax 16: Nk ~ 10^4).  Otherwise S(k) lives in registers, KPT
-// per thread, a_k and k indices in LDS.  Both forms give a thread the same
-// k = tid + j*BT (coalesced global and conflict-free LDS access) and reduce
-// in the same order, so a chain's results do not depend on the form its
-// launch used.  The register form keeps the trial's S + dS in registers
-// too; the streaming form writes dS to the chain's scratch.
-template <int BT, int KPT, bool MIXED, bool STREAM>
-__global__ void __launch_bounds__(BT)
-    batch_kernel(const BSys* __restrict__ systems, BChain* __restrict__ chains,
-                 const int* __restrict__ ids, long long nmoves, int nkcap,
-                 long long budget_ticks) {
-  using T = typename std::conditional<MIXED, float, double>::type;
-  __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
-  // a displacement's partner phase factors (mixed precision: delta form)
-  __shared__ Cplx<double> tabp[MIXED ? 2 * kMaxSites : 1][3][kBTab];
-  __shared__ Cplx<T> zt[2 * kMaxSites * kZRow];           // Z_p[n], signed n
-  // Move descriptors and duni tables, two of each: while the waves work on
-  // one move, the last wave proposes the next one on copies of the chain
-  // state as if the current move were rejected (most are); the serial lane
-  // adopts that proposal when the move is indeed rejected.
-  // The chain state is double-buffered as well: the last wave also books
-  // the current move as rejected in the other buffer, so that a rejection
-  // costs the serial lane its decision and a buffer flip.
-  __shared__ MoveDesc Dbuf[2];
-  __shared__ float rng_u[2][97];
-  __shared__ BChain Cbuf[2];
-  __shared__ int csel, dsel, rsel, spec_hit;   // current buffers, adopted?
-  __shared__ float cur_c, spec_c;       // duni state after this / the next proposal
-  __shared__ int cur_ir, cur_jr, spec_ir, spec_jr;
-  __shared__ double pre_rande, pre_delrc;   // this move's draw and tail change
-  __shared__ double pre_thr;                // estep above which it is rejected
-  __shared__ int pre_fast;                  // pre_thr applies
-  __shared__ double red[4][BT / kWave];
-  __shared__ int stop_flag;
-  __shared__ int shift_lo, shift_hi;
-  __shared__ SysQ Z;   // the system's small constants, for the whole launch
-  BChain& C = Cbuf[0];   // the launch's first state (the loop rebinds it)
-
-  const int tid = threadIdx.x;
-  // The chain's mutable state and the system's small constants are copied
-  // to LDS for the launch; tables and the framework are read from global
-  // memory.
-  BChain& Cg = chains[ids[blockIdx.x]];
-  const BSys& Y = systems[Cg.sys];
-  {
-    const int* src = reinterpret_cast<const int*>(&Cg);
-    int* dst = reinterpret_cast<int*>(&C);
-    for (int i = tid; i < static_cast<int>(sizeof(BChain) / 4); i += BT) dst[i] = src[i];
-  }
-  {
-    const int* src = reinterpret_cast<const int*>(&Y.q);
-    int* dst = reinterpret_cast<int*>(&Z);
-    for (int i = tid; i < static_cast<int>(sizeof(SysQ) / 4); i += BT) dst[i] = src[i];
-  }
-  __syncthreads();
-  const int nk = Y.nk;
-  // dynamic LDS: a_k [nkcap], packed k index [nkcap] (register form), then
-  // the XY tables
-  extern __shared__ double dyn_lds[];
-  double* ak_l = dyn_lds;
-  int* kv_l = reinterpret_cast<int*>(dyn_lds + (STREAM ? 0 : nkcap));
-  // X[l]*Y[m] for every (l, m) of the half space, per trial position:
-  // [p][l*(2*kmax2+1) + m + kmax2]
-  const int nlm = (Z.kmax[0] + 1) * (2 * Z.kmax[1] + 1);
-  Cplx<T>* xy_l =
-      reinterpret_cast<Cplx<T>*>(STREAM ? reinterpret_cast<int*>(dyn_lds)
-                                        : kv_l + ((nkcap + 3) & ~3));
-  // this thread owns k = tid + j*BT, j < kpt = ceil(nk/BT); the index
-  // (l, m) is precomputed into kv's upper bits
-  const int kpt = (nk + BT - 1) / BT;
-  if (!STREAM) {
-    for (int k = tid; k < nk; k += BT) {
-      ak_l[k] = Y.ak[k];
-      const int v = Y.kv[k];
-      const int lm = ul(v) * (2 * Z.kmax[1] + 1) + um(v) + Z.kmax[1];
-      kv_l[k] = (v & 0xff0000) | lm;   // n in bits 16..23, (l,m) index below
-    }
-  }
-  double sre[KPT], sim[KPT];
-  T dre[KPT], dim[KPT];   // the trial's dS, as trial_ds summed it (exact)
-  if (!STREAM) {
-#pragma unroll
-    for (int j = 0; j < KPT; ++j) {
-      const int k = tid + j * BT;
-      const bool in = j < kpt && k < nk;
-      sre[j] = in ? C.sre[k] : 0.0;
-      sim[j] = in ? C.sim[k] : 0.0;
-    }
-  }
-  Rng R = {};
-  // time budget (constant-rate counter): thread 0 raises stop_flag after
-  // the move that crosses it; the flag is read after a barrier
-  const long long t_start = (tid == 0 && budget_ticks > 0)
-                                ? static_cast<long long>(__builtin_amdgcn_s_memrealtime())
-                                : 0;
-  if (tid == 0) stop_flag = 0;
-  __syncthreads();
-  if (tid == 0) {
-    for (int i = 0; i < 97; ++i) rng_u[0][i] = C.u[i];
-    R.u = rng_u[0];
-    csel = 0;
-    dsel = 0;
-    rsel = 0;
-    spec_hit = 0;
-    R.c = C.c; R.cd = C.cd; R.cm = C.cm; R.ir = C.ir; R.jr = C.jr;
-  }
-  __syncthreads();
-
-  // moves left until the next in-launch recompute (the chain's cadence)
-
-  for (long long mv = 0; mv < nmoves; ++mv) {
-    BChain& C = Cbuf[csel];
-    // C.steps and stop_flag were last written before the barrier ending the
-    // previous move: every thread takes the same branch
-    if (C.steps >= C.stop_at || stop_flag) break;
-    // a prepared move is in place already (written before the last
-    // barrier); only a fresh proposal needs one
-    const bool fresh = !spec_hit;
-    if (tid < kWave) {
-      if (fresh) {
-        // wave 0 proposes; its lanes take thread 0's duni state
-        R.u = rng_u[rsel];
-        R.c = lane_f(R.c, 0);
-        R.ir = __builtin_amdgcn_readlane(R.ir, 0);
-        R.jr = __builtin_amdgcn_readlane(R.jr, 0);
-        R.cd = C.cd;
-        R.cm = C.cm;
-        propose(Z, C, R, Dbuf[dsel], tid);
-      }
-      if (tid == 0) {
-        cur_c = R.c;
-        cur_ir = R.ir;
-        cur_jr = R.jr;
-      }
-    }
-    if (fresh) __syncthreads();
-    // dsel is written only by thread 0's decision, after the reduction barrier
-    const int dcur = dsel;
-    MoveDesc& D = Dbuf[dcur];
-    const int kind = D.kind;
-    const int ns = D.ns;
-    const int ig = D.ig;
-    double e1 = 0.0, e2 = 0.0, ev = 0.0;
-    if (kind != 0) {
-      const int np = D.np;
-      // the positions' tables: built by every wave for a fresh proposal;
-      // a prepared one had them built while the previous move was decided
-      if (fresh) {
-        build_tables<T, MIXED>(Z, Y, D, tid / kWave, BT / kWave, tid % kWave, tab, tabp, xy_l,
-                               zt, nlm);
-        __syncthreads();
-      }
-      const int ntp = Y.S.ntpatm;
-      const bool ldsrow = ntp <= kLdsTypes;
-      // ---- reciprocal space (ewald1_guest, ewald.f:141-319) ----
-      // dS(k) = sum_new q e^{ik.r} - sum_old q e^{ik.r}
-      if (STREAM) {
-        const int w2 = 2 * Z.kmax[1] + 1;
-        for (int k = tid; k < nk; k += BT) {
-          const int v = Y.kv[k];
-          const int lm = ul(v) * w2 + um(v) + Z.kmax[1];
-          T dr32, di32;
-          trial_ds<T>(np, xy_l, nlm, lm, zt, un(v) + Z.kmax[2], dr32, di32);
-          const double dr = dr32, di = di32;
-          const double s0 = gload(C.sre + k), s1 = gload(C.sim + k);
-          gstore(C.dsre + k, dr);
-          gstore(C.dsim + k, di);
-          e1 += de_k(Y.ak[k], s0, s1, dr, di);
-        }
-      } else {
-        // fully unrolled (predicated, no early exit) so that sre/sim/dre/dim
-        // keep static register indices
-        int kvj[KPT];
-#pragma unroll
-        for (int j = 0; j < KPT; ++j) {
-          const int k = tid + j * BT;
-          kvj[j] = (j < kpt && k < nk) ? kv_l[k] : 0;
-        }
-        trial_ds_tile<T, KPT>(np, xy_l, nlm, zt, Z.kmax[2], __builtin_amdgcn_readfirstlane(kpt),
-                              kvj, dre, dim);
-#pragma unroll
-        for (int j = 0; j < KPT; ++j) {
-          const int k = tid + j * BT;
-          if (j < kpt && k < nk) {
-            const double dr = dre[j], di = dim[j];
-            e1 += de_k(ak_l[k], sre[j], sim[j], dr, di);
-          }
-        }
-      }
-      // ---- the next move's proposal, by the last wave, on copies of the
-      //      chain state and duni table as this move leaves them if it is
-      //      rejected (its acceptance draw taken, if it makes one) ----
-      constexpr int RT = BT - kWave;   // threads on real space
-      if (tid >= RT) {
-        const int lane = tid - RT;
-        BChain& Cn = Cbuf[csel ^ 1];
-        const long long* src = reinterpret_cast<const long long*>(&C);
-        long long* dst = reinterpret_cast<long long*>(&Cn);
-        static_assert(sizeof(BChain) % 8 == 0, "BChain copies in 8-byte words");
-        for (int i = lane; i < static_cast<int>(sizeof(BChain) / 8); i += kWave) dst[i] = src[i];
-        for (int i = lane; i < 97; i += kWave) rng_u[rsel ^ 1][i] = rng_u[rsel][i];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        __builtin_amdgcn_wave_barrier();
-        {
-          // every lane holds the same duni state
-          Rng Rs;
-          Rs.u = rng_u[rsel ^ 1];
-          Rs.c = cur_c;
-          Rs.cd = C.cd;
-          Rs.cm = C.cm;
-          Rs.ir = cur_ir;
-          Rs.jr = cur_jr;
-          // this move booked as rejected: its acceptance draw (every
-          // rejected move with a parallel phase takes one), the attempt,
-          // the step and the production sample
-          (void)Rs.next();
-          if (lane == 0) {
-            Cn.attempted[move_slot(D)]++;
-            Cn.steps++;
-            (void)production_sample(Cn);
-          }
-          propose(Z, Cn, Rs, Dbuf[dsel ^ 1], lane);
-          if (lane == 0) {
-            spec_c = Rs.c;
-            spec_ir = Rs.ir;
-            spec_jr = Rs.jr;
-          }
-        }
-      }
-      // ---- real space: framework candidates of every position's cell and
-      //      every guest atom, one flattened loop (ewald2 + srfrce), on all
-      //      but the last wave.  It reads only what the set-up phases
-      //      wrote, so no barrier separates it from the reciprocal part: a
-      //      wave that finishes its k-vectors starts its pairs at once ----
-      // the decision's inputs, by the first lane of the wave before it (it
-      // finishes its real-space share ahead of the preparing wave): the
-      // acceptance draw (read without taking it), the tail change, the
-      // rejection bound
-      if (tid == RT - kWave) {
-        const float* u = rng_u[rsel];
-        float uni = u[cur_ir - 1] - u[cur_jr - 1];
-        if (uni < 0.0f) uni = uni + 1.0f;
-        float c = cur_c - C.cd;
-        if (c < 0.0f) c = c + C.cm;
-        uni = uni - c;
-        if (uni < 0.0f) uni = uni + 1.0f;
-        const double rande = uni;
-        pre_rande = rande;
-        pre_delrc = (kind == FMC_INSERT || kind == FMC_DELETE) ? tail_change(Z, C, D) : 0.0;
-        double thr = 0.0;
-        pre_fast = reject_bound(C, D, rande, &thr) ? 1 : 0;
-        pre_thr = thr;
-      }
-      if (tid < RT) {
-        const int gcap = Y.gcap;
-        const int nfwp = np * gcap;
-        const int ng = D.nguest;
-        const float rgcap = 1.0f / gcap, rng = 1.0f / max(ng, 1);
-        const int total = nfwp + np * ng;
-        const int own0 = (kind != FMC_INSERT) ? D.gpre[ig] + D.imol * ns : -1;
-        double a2 = 0, av = 0;
-        // kU items per thread per pass: their index loads, then their
-        // coordinate loads, issue together (one memory latency per pass)
-        constexpr int kU = 4;
-        for (int base = tid; base < total; base += kU * RT) {
-          int pp[kU], aa[kU];
-#pragma unroll
-          for (int u = 0; u < kU; ++u) {
-            const int idx = base + u * RT;
-            pp[u] = -1;
-            aa[u] = 0;
-            if (idx < nfwp) {
-              const int p = div_rc(idx, gcap, rgcap);
-              pp[u] = p;
-              aa[u] = D.plo[p] + (idx - p * gcap);   // record slot
-            } else if (idx < total) {
-              const int r = idx - nfwp;
-              const int p = div_rc(r, ng, rng);
-              const int a = r - p * ng;
-              if (!(own0 >= 0 && a >= own0 && a < own0 + ns)) {
-                pp[u] = p;
-                aa[u] = -1 - a;   // guest atom a
-              }
-            }
-          }
-          double ax[kU], ay[kU], az[kU], aq[kU];
-          int at[kU];
-#pragma unroll
-          for (int u = 0; u < kU; ++u) {
-            ax[u] = ay[u] = az[u] = aq[u] = 0.0;
-            at[u] = 0;
-            if (pp[u] < 0) continue;
-            if (aa[u] >= 0) {
-              const double* rr = &Y.grec[aa[u]].x;
-              ax[u] = gload(rr);
-              ay[u] = gload(rr + 1);
-              az[u] = gload(rr + 2);
-              aq[u] = gload(rr + 3);
-              at[u] = gload(Y.grec_t + aa[u]);
-            } else {
-              const int a = -1 - aa[u];
-              int g = 0;
-              while (g + 1 < Z.ngt && D.gpre[g + 1] <= a) ++g;
-              const int la = a - D.gpre[g];
-              const int sa =
-                  la - div_rc(la, Z.ns[g], __builtin_amdgcn_rcpf(static_cast<float>(Z.ns[g]))) *
-                           Z.ns[g];
-              ax[u] = gload(C.gx[g] + la);
-              ay[u] = gload(C.gy[g] + la);
-              az[u] = gload(C.gz[g] + la);
-              aq[u] = Z.gq[g][sa];
-              at[u] = Z.gtype[g][sa];
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < kU; ++u) {
-            if (pp[u] < 0) continue;
-            const int p = pp[u];
-            // FP64 in both precisions: the nearest pairs' terms reach tens
-            // of kcal/mol and cancel against each other and against the
-            // reciprocal part, so an FP32 rounding of each would dominate
-            // the error of the move's energy change, and on this path FP64
-            // pair arithmetic costs about what FP32 does
-            double b2 = 0, bv = 0;
-            const int vk =
-                ldsrow ? D.ptab[p][at[u]] : gload(Y.S.pairtab + D.pts[p] * ntp + at[u]);
-            pair_fp64_vk(Y.S, D.pce[p], vk, D.ppx[p] - ax[u], D.ppy[p] - ay[u],
-                         D.ppz[p] - az[u], aq[u], b2, bv);
-            if (D.psg[p] < 0) {
-              a2 -= b2;
-              av -= bv;
-            } else {
-              a2 += b2;
-              av += bv;
-            }
-          }
-        }
-        e2 = a2;
-        ev = av;
-      }
-    }
-    // block reductions (fixed tree): DPP within each wave, then the waves'
-    // totals in order
-    e1 = wave_sum_dpp(e1);
-    e2 = wave_sum_dpp(e2);
-    ev = wave_sum_dpp(ev);
-    if ((tid % kWave) == kWave - 1) {
-      red[0][tid / kWave] = e1;
-      red[1][tid / kWave] = e2;
-      red[2][tid / kWave] = ev;
-    }
-    __syncthreads();
-    // while thread 0 decides, waves 1.. build the tables of the prepared
-    // next move (this move's are read no more); the next move uses them if
-    // it adopts the prepared proposal, and rebuilds them otherwise
-    if (kind != 0 && tid >= kWave) {
-      MoveDesc& Dn = Dbuf[dcur ^ 1];
-      if (Dn.kind != 0)
-        build_tables<T, MIXED>(Z, Y, Dn, tid / kWave - 1, BT / kWave - 1, tid % kWave, tab,
-                               tabp, xy_l, zt, nlm);
-    }
-    if (tid == 0) {
-      double a = 0.0, b = 0.0, c = 0.0;
-      for (int w = 0; w < BT / kWave; ++w) {
-        a += red[0][w];
-        b += red[1][w];
-        c += red[2][w];
-      }
-      a = Z.rvolm2 * kR4pie0 * a / Z.epsq;
-      if (budget_ticks > 0 &&
-          static_cast<long long>(__builtin_amdgcn_s_memrealtime()) - t_start >= budget_ticks)
-        stop_flag = 1;
-      // A rejected move, when another move follows in this launch (a
-      // launch never ends holding draws of a move it has not made), takes
-      // the other buffers: the move booked as rejected and the next move
-      // proposed.  Otherwise the serial lane books the move itself.  (A
-      // no-op move has no parallel phase, hence nothing prepared.)
-      spec_hit = 0;
-      if (kind != 0 && !stop_flag && C.steps + 1 < C.stop_at && mv + 1 < nmoves) {
-        double estep, e1x;
-        bool rej;
-        if (pre_fast) {
-          // estep and the logged e1 as decide forms them; the bound first
-          e1x = a;
-          if (kind == FMC_INSERT) {
-            e1x = e1x + Z.engsic[ig];
-            estep = (e1x + b + Z.e3[ig] + c + pre_delrc) / C.engunit;
-          } else if (kind == FMC_DELETE) {
-            e1x = e1x - Z.engsic[ig];
-            estep = (e1x - b - Z.e3[ig] - c + pre_delrc) / C.engunit;
-          } else {
-            estep = (e1x + b + c) / C.engunit;
-          }
-          rej = estep > pre_thr ||
-                !decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x);
-        } else {
-          rej = !decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x);
-        }
-        if (rej && !(kind == FMC_DISPLACE && estep < 0.0)) {
-          spec_hit = 1;
-          D.accepted = 0;
-          csel ^= 1;
-          rsel ^= 1;
-          dsel ^= 1;
-          R.u = rng_u[rsel];
-          R.c = spec_c;
-          R.ir = spec_ir;
-          R.jr = spec_jr;
-          BChain& Cn = Cbuf[csel];
-          if (Cn.win_len > 0 && Cn.steps > Cn.eq_steps) window_snapshot(Cn, Cn.sre + nk);
-          log_move(Cn, move_slot(D), ig, 0, estep, e1x, b, c);
-        }
-      }
-      if (!spec_hit) decide(Z, C, R, D, a, b, c, nk);
-      shift_lo = 0;
-      shift_hi = 0;
-      if (D.accepted && kind == FMC_DELETE) {
-        shift_lo = D.imol * ns;
-        shift_hi = C.nmol[ig] * ns;   // already decremented
-      }
-    }
-    __syncthreads();
-    {
-    // the chain state after this move (the other buffer if it was rejected)
-    BChain& C = Cbuf[csel];
-    if (D.accepted) {
-      // S(k) <- S_new (ewald.f:260-293, gcmc.f:971-974)
-      if (STREAM) {
-        for (int k = tid; k < nk; k += BT) {
-          gstore(C.sre + k, gload(C.sre + k) + gload(C.dsre + k));
-          gstore(C.sim + k, gload(C.sim + k) + gload(C.dsim + k));
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < KPT; ++j) {
-          const int k = tid + j * BT;
-          if (j < kpt && k < nk) {
-            sre[j] = sre[j] + static_cast<double>(dre[j]);
-            sim[j] = sim[j] + static_cast<double>(dim[j]);
-          }
-        }
-      }
-      if (kind == FMC_INSERT) {
-        if (tid < ns) {
-          C.gx[ig][D.slot + tid] = D.nx[tid];
-          C.gy[ig][D.slot + tid] = D.ny[tid];
-          C.gz[ig][D.slot + tid] = D.nz[tid];
-        }
-      } else if (kind == FMC_DISPLACE) {
-        if (tid < ns) {
-          C.gx[ig][D.imol * ns + tid] = D.nx[tid];
-          C.gy[ig][D.imol * ns + tid] = D.ny[tid];
-          C.gz[ig][D.imol * ns + tid] = D.nz[tid];
-        }
-      } else if (kind == FMC_DELETE) {
-        // successors shift down by one molecule (gcmc.f:1127-1131)
-        for (int base = shift_lo; base < shift_hi; base += BT) {
-          const int i = base + tid;
-          double x = 0, y = 0, z = 0;
-          if (i < shift_hi) {
-            x = C.gx[ig][i + ns];
-            y = C.gy[ig][i + ns];
-            z = C.gz[ig][i + ns];
-          }
-          __syncthreads();
-          if (i < shift_hi) {
-            C.gx[ig][i] = x;
-            C.gy[ig][i] = y;
-            C.gz[ig][i] = z;
-          }
-          __syncthreads();
-        }
-      }
-      __syncthreads();
-    }
-    // FP64 recompute every recomp_every moves of this chain, inside the
-    // launch: S(k) <- exact, full energy, drift (as recompute_kernel; the
-    // same exact_sk, so the same S bits).  Uniform branch (LDS values
-    // written before the last barrier).
-    if (C.recomp_every > 0 && C.steps % C.recomp_every == 0) {
-      double erec = 0.0, dev = 0.0;
-      if (STREAM) {
-        for (int k = tid; k < nk; k += BT) {
-          double sx, sy;
-          exact_sk(Y, Z, C, k, sx, sy);
-          const double fr = Y.sfw_re[k], fi = Y.sfw_im[k];
-          dev = fmax(dev, fmax(fabs(sx - gload(C.sre + k)), fabs(sy - gload(C.sim + k))));
-          gstore(C.sre + k, sx);
-          gstore(C.sim + k, sy);
-          erec += Y.ak[k] * ((sx - fr) * (sx + fr) + (sy - fi) * (sy + fi));
-        }
-      } else {
-        for (int j = 0; j < kpt; ++j) {   // not unrolled: one copy of exact_sk
-          const int k = tid + j * BT;
-          if (k >= nk) break;
-          double sx, sy;
-          exact_sk(Y, Z, C, k, sx, sy);
-          const double fr = Y.sfw_re[k], fi = Y.sfw_im[k];
-#pragma unroll
-          for (int jj = 0; jj < KPT; ++jj)
-            if (jj == j) {
-              dev = fmax(dev, fmax(fabs(sx - sre[jj]), fabs(sy - sim[jj])));
-              sre[jj] = sx;
-              sim[jj] = sy;
-            }
-          erec += ak_l[k] * ((sx - fr) * (sx + fr) + (sy - fi) * (sy + fi));
-        }
-      }
-      double e2r = 0.0, evr = 0.0;
-      guest_pairs(Y, Z, C, tid, BT, e2r, evr);
-      erec = wave_sum_dpp(erec);
-      e2r = wave_sum_dpp(e2r);
-      evr = wave_sum_dpp(evr);
-      for (int off = kWave / 2; off > 0; off >>= 1) dev = fmax(dev, __shfl_xor(dev, off, kWave));
-      if ((tid % kWave) == kWave - 1) {
-        red[0][tid / kWave] = erec;
-        red[1][tid / kWave] = e2r;
-        red[2][tid / kWave] = evr;
-        red[3][tid / kWave] = dev;
-      }
-      __syncthreads();
-      if (tid == 0) {
-        double a = 0.0, b = 0.0, cc = 0.0, d = 0.0;
-        for (int w = 0; w < BT / kWave; ++w) {
-          a += red[0][w];
-          b += red[1][w];
-          cc += red[2][w];
-          d = fmax(d, red[3][w]);
-        }
-        recompute_finish(Z, C, a, b, cc, d, 0);
-      }
-      __syncthreads();
-    }
-    }
-  }
-  // write the state back
-  BChain& Cf = Cbuf[csel];
-  if (!STREAM) {
-#pragma unroll
-    for (int j = 0; j < KPT; ++j) {
-      const int k = tid + j * BT;
-      if (j < kpt && k < nk) {
-        Cf.sre[k] = sre[j];
-        Cf.sim[k] = sim[j];
-      }
-    }
-  }
-  if (tid == 0) {
-    for (int i = 0; i < 97; ++i) Cf.u[i] = rng_u[rsel][i];
-    Cf.c = R.c; Cf.ir = R.ir; Cf.jr = R.jr;
-  }
-  __syncthreads();
-  {
-    const int* src = reinterpret_cast<const int*>(&Cf);
-    int* dst = reinterpret_cast<int*>(&Cg);
-    for (int i = tid; i < static_cast<int>(sizeof(BChain) / 4); i += BT) dst[i] = src[i];
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Candidate grid (built once per framework)
@@ -1955,9 +348,13 @@ void duni_setup(int i, int j, int k, int l, BChain* C) {
 struct fmc_batch {
   int max_chains = 0;
   int precision = FMC_PREC_FP64;
-  int bt = 0, kpt = 0;   // block size; register tile (0: streaming form)
+  // tile shape: block size, register tile (0: streaming form), k-group
+  // width and real-space items per pass
+  int bt = 0, kpt = 0, kg = 4, ku = 4;
   bool tuned = false;    // false: autotune at the next launch
   bool force_stream = false;
+  int tune_probes = 0;   // shapes the last autotune timed
+  bool fixed_kpt = false; // the register tile was set, not derived
   std::vector<std::unique_ptr<SysHost>> systems;
   std::vector<std::unique_ptr<ChainHost>> chains;
   BSys* d_sys = nullptr;
@@ -1977,6 +374,7 @@ struct fmc_batch {
   std::mutex admit_mu;
   std::vector<char> reserved;   // slots being admitted
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t tev0 = nullptr, tev1 = nullptr;   // the autotuner's timings
   // graph replays: one event pair around each replay of the move graph
   std::vector<std::pair<hipEvent_t, hipEvent_t>> cyc_ev;
   long long cyc_pending = 0;
@@ -1989,7 +387,7 @@ struct fmc_batch {
   hipGraphExec_t graph_rec_exec = nullptr;
   std::vector<int> graph_ids;
   long long graph_moves = -1;
-  int graph_bt = 0, graph_kpt = -1;
+  int graph_bt = 0, graph_kpt = -1, graph_kg = 0, graph_ku = 0;
   bool timing_pending = false;
   double kernel_ms = 0.0;
   long long launches = 0;
@@ -1999,39 +397,57 @@ struct fmc_batch {
   std::vector<int> active_ids;
 };
 
+// the tile shapes fmc_batch_tiles.hip instantiates, one object per
+// (block size, register tile)
+namespace fmc {
+#define FMC_TILE_DECL(BT_, KPT_)                 \
+  extern const KCfg kTiles_##BT_##_##KPT_[];    \
+  extern const int kNTiles_##BT_##_##KPT_;
+FMC_TILE_DECL(256, 8)
+FMC_TILE_DECL(256, 16)
+FMC_TILE_DECL(256, 28)
+FMC_TILE_DECL(256, 0)
+FMC_TILE_DECL(512, 4)
+FMC_TILE_DECL(512, 8)
+FMC_TILE_DECL(512, 12)
+FMC_TILE_DECL(512, 14)
+FMC_TILE_DECL(512, 0)
+#undef FMC_TILE_DECL
+}  // namespace fmc
+
 namespace {
 
-using KernelFn = void (*)(const BSys*, BChain*, const int*, long long, int, long long);
+const std::vector<KCfg>& all_cfgs() {
+  static const std::vector<KCfg> v = [] {
+    std::vector<KCfg> out;
+    auto add = [&out](const KCfg* t, int n) { out.insert(out.end(), t, t + n); };
+    add(kTiles_256_8, kNTiles_256_8);
+    add(kTiles_256_16, kNTiles_256_16);
+    add(kTiles_256_28, kNTiles_256_28);
+    add(kTiles_256_0, kNTiles_256_0);
+    add(kTiles_512_4, kNTiles_512_4);
+    add(kTiles_512_8, kNTiles_512_8);
+    add(kTiles_512_12, kNTiles_512_12);
+    add(kTiles_512_14, kNTiles_512_14);
+    add(kTiles_512_0, kNTiles_512_0);
+    return out;
+  }();
+  return v;
+}
 
-struct KCfg {
-  int bt, kpt;   // kpt 0: the streaming form
-  bool mixed;
-  KernelFn fn;
-};
-
-#define KC(BT_, KPT_)                                                      \
-  {BT_, KPT_, false, batch_kernel<BT_, KPT_, false, false>},                \
-      {BT_, KPT_, true, batch_kernel<BT_, KPT_, true, false>}
-#define KS(BT_)                                                            \
-  {BT_, 0, false, batch_kernel<BT_, 1, false, true>},                       \
-      {BT_, 0, true, batch_kernel<BT_, 1, true, true>}
-
-const KCfg kConfigs[] = {
-    KC(256, 8), KC(256, 16), KC(256, 28), KC(512, 4), KC(512, 8), KC(512, 12),
-    KC(512, 14),
-    KS(256),    KS(512),
-};
-
-const KCfg* find_cfg(int bt, int kpt, bool mixed) {
-  for (const auto& c : kConfigs)
-    if (c.bt == bt && c.kpt == kpt && c.mixed == mixed) return &c;
+// kg is ignored by the streaming form (kpt 0)
+const KCfg* find_cfg(int bt, int kpt, int kg, int ku, bool mixed) {
+  for (const auto& c : all_cfgs())
+    if (c.bt == bt && c.kpt == kpt && (kpt == 0 || c.kg == kg) && c.ku == ku &&
+        c.mixed == mixed)
+      return &c;
   return nullptr;
 }
 
 // smallest template KPT for block size bt that covers nk
 int kpt_for(int bt, int nk) {
   int best = 0;
-  for (const auto& c : kConfigs)
+  for (const auto& c : all_cfgs())
     if (c.bt == bt && !c.mixed && c.kpt > 0 && c.bt * c.kpt >= nk &&
         (best == 0 || c.kpt < best))
       best = c.kpt;
@@ -2314,8 +730,9 @@ int lds_need(fmc_batch* b, const KCfg* cfg, size_t* out) {
   return FMC_OK;
 }
 
-int launch(fmc_batch* b, long long n, int bt, int kpt, long long budget_ticks = 0) {
-  const KCfg* cfg = find_cfg(bt, kpt, b->precision == FMC_PREC_MIXED);
+int launch(fmc_batch* b, long long n, int bt, int kpt, int kg, int ku,
+           long long budget_ticks = 0) {
+  const KCfg* cfg = find_cfg(bt, kpt, kg, ku, b->precision == FMC_PREC_MIXED);
   if (!cfg) return FMC_ERR_UNSUPPORTED;
   if (b->active_ids.empty()) return FMC_OK;
   if (kpt > 0 && bt * kpt < b->max_nk) return FMC_ERR_UNSUPPORTED;
@@ -2329,17 +746,22 @@ int launch(fmc_batch* b, long long n, int bt, int kpt, long long budget_ticks = 
   return FMC_OK;
 }
 
+int launch(fmc_batch* b, long long n, long long budget_ticks = 0) {
+  return launch(b, n, b->bt, b->kpt, b->kg, b->ku, budget_ticks);
+}
+
 // the register form for block size bt if it covers the largest Nk and fits
 // in LDS, else the streaming form (256/512 threads); 0 if neither
 int pick_kpt(fmc_batch* b, int bt, bool allow_stream, int* kpt) {
   const bool mixed = b->precision == FMC_PREC_MIXED;
   const int k = kpt_for(bt, std::max(1, b->max_nk));
   size_t lds;
-  if (k > 0 && lds_need(b, find_cfg(bt, k, mixed), &lds) == FMC_OK) {
+  const KCfg* rc = k > 0 ? find_cfg(bt, k, b->kg, b->ku, mixed) : nullptr;
+  if (rc && lds_need(b, rc, &lds) == FMC_OK) {
     *kpt = k;
     return FMC_OK;
   }
-  const KCfg* sc = find_cfg(bt, 0, mixed);
+  const KCfg* sc = find_cfg(bt, 0, b->kg, b->ku, mixed);
   if (allow_stream && sc && lds_need(b, sc, &lds) == FMC_OK) {
     *kpt = 0;
     return FMC_OK;
@@ -2395,38 +817,63 @@ int snapshot(fmc_batch* b, Snapshot* s, bool restore) {
   return FMC_OK;
 }
 
-// Project autotuner: time each block size (with the smallest register tile
-// that covers the largest Nk) on real moves, keep the fastest, restore state.
+// Project autotuner.  Every compiled shape that can run the batch's systems
+// is timed on real moves: block size 256 or 512, with the smallest register
+// tile that covers the largest Nk or the streaming form, every k-group
+// width (the k-vectors whose LDS reads issue together) and real-space items
+// per pass.  Each shape first runs a short launch (loading its code), then
+// a timed one, GPU time from HIP events on the batch stream; the three
+// fastest are timed again with twice the moves and the fastest of those is
+// kept.  The chains are restored to their state before tuning, so the
+// tuned run starts where it would have.  Only the block size changes a
+// chain's arithmetic (the reductions over threads); the chains record the
+// shape their launches used (fmc_chain_stats, checkpoints).
 int autotune(fmc_batch* b) {
-  std::vector<int> bts = {256, 512};
+  const bool mixed = b->precision == FMC_PREC_MIXED;
+  struct Shape {
+    int bt, kpt, kg, ku;
+    float ms;
+  };
+  std::vector<Shape> cand;
+  for (int bt : {256, 512}) {
+    const int k = kpt_for(bt, std::max(1, b->max_nk));
+    for (const auto& c : all_cfgs()) {
+      if (c.bt != bt || c.mixed != mixed || !(c.kpt == 0 || c.kpt == k)) continue;
+      size_t lds;
+      if (lds_need(b, &c, &lds) != FMC_OK) continue;
+      cand.push_back({c.bt, c.kpt, c.kg, c.ku, 0.0f});
+    }
+  }
+  if (cand.empty()) return FMC_ERR_UNSUPPORTED;
   Snapshot snap;
   int r;
   if ((r = snapshot(b, &snap, false))) return r;
-  double best = 1e300;
-  int bbt = 0, bkpt = 0;
-  for (int bt : bts) {
-    int kpt = 0;
-    if (pick_kpt(b, bt, true, &kpt)) continue;
-    // warm-up + timed run
-    r = launch(b, 4, bt, kpt);
-    if (r == FMC_ERR_UNSUPPORTED) continue;
-    if (r) return r;
-    BCHECK(hipStreamSynchronize(b->stream));
-    const auto t0 = std::chrono::steady_clock::now();
-    if ((r = launch(b, 24, bt, kpt))) return r;
-    BCHECK(hipStreamSynchronize(b->stream));
-    const double dt =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    if (dt < best) {
-      best = dt;
-      bbt = bt;
-      bkpt = kpt;
-    }
-    if ((r = snapshot(b, &snap, true))) return r;
-  }
-  if (!bbt) return FMC_ERR_UNSUPPORTED;
-  b->bt = bbt;
-  b->kpt = bkpt;
+  auto time_shape = [&](Shape& sh, long long moves) -> int {
+    int rr = launch(b, 2, sh.bt, sh.kpt, sh.kg, sh.ku);
+    if (rr) return rr;
+    BCHECK(hipEventRecord(b->tev0, b->stream));
+    if ((rr = launch(b, moves, sh.bt, sh.kpt, sh.kg, sh.ku))) return rr;
+    BCHECK(hipEventRecord(b->tev1, b->stream));
+    BCHECK(hipEventSynchronize(b->tev1));
+    BCHECK(hipEventElapsedTime(&sh.ms, b->tev0, b->tev1));
+    return FMC_OK;
+  };
+  for (auto& sh : cand)
+    if ((r = time_shape(sh, 32))) return r;
+  std::sort(cand.begin(), cand.end(), [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
+  const size_t top = std::min<size_t>(3, cand.size());
+  for (size_t i = 0; i < top; ++i)
+    if ((r = time_shape(cand[i], 64))) return r;
+  std::sort(cand.begin(), cand.begin() + top,
+            [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
+  if ((r = snapshot(b, &snap, true))) return r;
+  b->bt = cand[0].bt;
+  b->kpt = cand[0].kpt;
+  b->kg = cand[0].kg;
+  b->ku = cand[0].ku;
+  b->force_stream = cand[0].kpt == 0;
+  b->fixed_kpt = false;
+  b->tune_probes = static_cast<int>(cand.size());
   b->tuned = true;
   return FMC_OK;
 }
@@ -2437,11 +884,12 @@ int ensure_tile(fmc_batch* b) {
   if (!b->tuned) return autotune(b);
   if (b->force_stream) {
     size_t lds;
-    const KCfg* sc = find_cfg(b->bt, 0, b->precision == FMC_PREC_MIXED);
+    const KCfg* sc = find_cfg(b->bt, 0, b->kg, b->ku, b->precision == FMC_PREC_MIXED);
     if (!sc) return FMC_ERR_UNSUPPORTED;
     b->kpt = 0;
     return lds_need(b, sc, &lds);
   }
+  if (b->fixed_kpt) return b->bt * b->kpt >= b->max_nk ? FMC_OK : FMC_ERR_UNSUPPORTED;
   return pick_kpt(b, b->bt, true, &b->kpt);
 }
 
@@ -2462,6 +910,8 @@ extern "C" int fmc_batch_create(fmc_batch** out, int max_chains, int precision) 
   BCHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
   BCHECK(hipEventCreate(&b->ev0));
   BCHECK(hipEventCreate(&b->ev1));
+  BCHECK(hipEventCreate(&b->tev0));
+  BCHECK(hipEventCreate(&b->tev1));
   BCHECK(hipMalloc(&b->d_chains, max_chains * sizeof(BChain)));
   BCHECK(hipMemset(b->d_chains, 0, max_chains * sizeof(BChain)));
   BCHECK(hipMalloc(&b->d_ids, max_chains * sizeof(int)));
@@ -2491,6 +941,8 @@ extern "C" int fmc_batch_destroy(fmc_batch* b) {
     (void)hipEventDestroy(e.second);
   }
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->tev0) (void)hipEventDestroy(b->tev0);
+  if (b->tev1) (void)hipEventDestroy(b->tev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return FMC_OK;
@@ -2831,6 +1283,30 @@ extern "C" int fmc_batch_restore_chain(fmc_batch* b, const fmc_ctx* ctx, const v
         static_cast<long long>(C.nmol[t]) * h.ns[t] > C.cap_atoms)
       return FMC_ERR_ARG;
   if (len != ck_size(C, h.nk, h.ns)) return FMC_ERR_ARG;
+  // a chain's arithmetic depends on the block size of the launches that
+  // move it: a batch without a tile yet takes the chain's; a batch whose
+  // tile has another block size refuses it
+  if (C.tile_code) {
+    const int cbt = (C.tile_code & 63) * 64;
+    std::lock_guard<std::mutex> g(b->mu);
+    if (b->tuned && b->bt != cbt) {
+      std::fprintf(stderr,
+                   "fastmc_amd batch: checkpoint of a chain moved by %d-thread launches, "
+                   "batch tile %d threads\n", cbt, b->bt);
+      return FMC_ERR_STATE;
+    }
+    if (!b->tuned) {
+      const int ckpt = (C.tile_code >> 6) & 63;
+      b->bt = cbt;
+      b->kg = (C.tile_code >> 12) & 15;
+      b->ku = (C.tile_code >> 16) & 15;
+      b->kpt = ckpt;
+      b->force_stream = ckpt == 0;
+      b->fixed_kpt = false;
+      b->tune_probes = 0;
+      b->tuned = true;
+    }
+  }
   std::lock_guard<std::mutex> admit(b->admit_mu);
   int slot = -1, sys = -1;
   int r = reserve_slot(b, key, &slot, &sys);
@@ -2899,27 +1375,58 @@ extern "C" int fmc_batch_get_molecules(fmc_batch* b, int id, int gtype, double* 
 
 extern "C" int fmc_batch_set_tile(fmc_batch* b, int threads) {
   if (!b) return FMC_ERR_ARG;
-  std::lock_guard<std::mutex> g(b->mu);
   if (threads == 0) {
+    std::lock_guard<std::mutex> g(b->mu);
     b->bt = b->kpt = 0;
+    b->kg = b->ku = 4;
     b->tuned = false;
     b->force_stream = false;
+    b->fixed_kpt = false;
     return FMC_OK;
   }
   const bool stream = (threads & FMC_TILE_STREAM) != 0;
-  const int bt = threads & ~FMC_TILE_STREAM;
-  if (bt != 256 && bt != 512) return FMC_ERR_UNSUPPORTED;
-  b->bt = bt;
-  b->kpt = 0;
-  b->tuned = true;
-  b->force_stream = stream;
-  return FMC_OK;
+  return fmc_batch_set_tile_shape(b, threads & ~FMC_TILE_STREAM, stream ? 0 : -1, 4, 4);
 }
 
 extern "C" int fmc_batch_get_tile(const fmc_batch* b, int* threads) {
   if (!b || !threads) return FMC_ERR_ARG;
   std::lock_guard<std::mutex> g(const_cast<fmc_batch*>(b)->mu);
   *threads = b->bt | ((b->tuned && b->kpt == 0 && b->bt) ? FMC_TILE_STREAM : 0);
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_set_tile_shape(fmc_batch* b, int threads, int kpt, int kgroup,
+                                        int kunroll) {
+  if (!b) return FMC_ERR_ARG;
+  if (threads != 256 && threads != 512) return FMC_ERR_UNSUPPORTED;
+  if (kgroup != 2 && kgroup != 4 && kgroup != 8) return FMC_ERR_UNSUPPORTED;
+  if (kunroll != 2 && kunroll != 4) return FMC_ERR_UNSUPPORTED;
+  if (kpt > 0) {
+    bool have = false;
+    for (const auto& c : all_cfgs())
+      if (c.bt == threads && c.kpt == kpt && c.kg == kgroup) have = true;
+    if (!have) return FMC_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> g(b->mu);
+  b->bt = threads;
+  b->kpt = kpt > 0 ? kpt : 0;
+  b->kg = kgroup;
+  b->ku = kunroll;
+  b->tuned = true;
+  b->force_stream = kpt == 0;
+  b->fixed_kpt = kpt > 0;
+  return FMC_OK;
+}
+
+extern "C" int fmc_batch_get_tile_shape(const fmc_batch* b, int* threads, int* kpt,
+                                        int* kgroup, int* kunroll, int* tuned) {
+  if (!b) return FMC_ERR_ARG;
+  std::lock_guard<std::mutex> g(const_cast<fmc_batch*>(b)->mu);
+  if (threads) *threads = b->bt;
+  if (kpt) *kpt = b->kpt;
+  if (kgroup) *kgroup = b->kg;
+  if (kunroll) *kunroll = b->ku;
+  if (tuned) *tuned = b->tuned ? (b->tune_probes > 0 ? 2 : 1) : 0;
   return FMC_OK;
 }
 
@@ -2933,7 +1440,7 @@ extern "C" int fmc_batch_launch(fmc_batch* b, long long n) {
   if ((r = ensure_tile(b))) return r;
   // kernel time on the batch's own stream
   BCHECK(hipEventRecord(b->ev0, b->stream));
-  if ((r = launch(b, n, b->bt, b->kpt))) return r;
+  if ((r = launch(b, n))) return r;
   BCHECK(hipEventRecord(b->ev1, b->stream));
   b->timing_pending = true;
   b->timing_mode = 1;
@@ -2955,7 +1462,7 @@ extern "C" int fmc_batch_launch_budget(fmc_batch* b, long long max_moves,
   if (khz <= 0) return FMC_ERR_UNSUPPORTED;
   const long long ticks = std::max(1LL, static_cast<long long>(budget_us * 1e-3 * khz));
   BCHECK(hipEventRecord(b->ev0, b->stream));
-  if ((r = launch(b, max_moves, b->bt, b->kpt, ticks))) return r;
+  if ((r = launch(b, max_moves, ticks))) return r;
   BCHECK(hipEventRecord(b->ev1, b->stream));
   b->timing_pending = true;
   b->timing_mode = 1;
@@ -3054,6 +1561,10 @@ extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
   st->drift = C.drift;
   st->sk_dev = C.sk_dev;
   st->recomputes = C.n_recompute;
+  st->tile_threads = (C.tile_code & 63) * 64;
+  st->tile_kpt = (C.tile_code >> 6) & 63;
+  st->tile_kgroup = (C.tile_code >> 12) & 15;
+  st->tile_kunroll = (C.tile_code >> 16) & 15;
   st->sel_window = C.sel_nwind;
   st->sel_count = C.sel_count;
   for (int g = 0; g < kMaxGuestTypes; ++g) st->sel_base[g] = C.sel_base[g];
@@ -3179,11 +1690,12 @@ extern "C" int fmc_batch_launch_cycles(fmc_batch* b, long long n_cycles,
   if (b->active_ids.empty() || n_cycles == 0) return FMC_OK;
   if ((r = ensure_tile(b))) return r;
   if (!b->graph_exec || b->graph_ids != b->active_ids || b->graph_moves != moves_per_cycle ||
-      b->graph_bt != b->bt || b->graph_kpt != b->kpt) {
+      b->graph_bt != b->bt || b->graph_kpt != b->kpt || b->graph_kg != b->kg ||
+      b->graph_ku != b->ku) {
     invalidate_graph(b);
     for (int part = 0; part < 2; ++part) {
       BCHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-      const int r1 = part == 0 ? launch(b, moves_per_cycle, b->bt, b->kpt)
+      const int r1 = part == 0 ? launch(b, moves_per_cycle)
                                : enqueue_recompute(b);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(b->stream, &g);
@@ -3207,6 +1719,8 @@ extern "C" int fmc_batch_launch_cycles(fmc_batch* b, long long n_cycles,
     b->graph_moves = moves_per_cycle;
     b->graph_bt = b->bt;
     b->graph_kpt = b->kpt;
+    b->graph_kg = b->kg;
+    b->graph_ku = b->ku;
   }
   while (static_cast<long long>(b->cyc_ev.size()) < n_cycles) {
     hipEvent_t e0, e1;

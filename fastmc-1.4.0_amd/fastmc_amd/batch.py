@@ -181,6 +181,8 @@ class Batch:
                     sel_window=st.sel_window, sel_count=st.sel_count,
                     sel_base=list(st.sel_base)[:ng], sel_mean=list(st.sel_mean),
                     sel_second=list(st.sel_second), sel_var=list(st.sel_var),
+                    tile=dict(threads=st.tile_threads, kpt=st.tile_kpt,
+                              kgroup=st.tile_kgroup, kunroll=st.tile_kunroll),
                     capacity_fault=(r == -4))
 
     def report(self, cid: int, temp: float):
@@ -283,6 +285,20 @@ class Batch:
         v = C.c_int()
         check(self.L.fmc_batch_get_tile(self.h, C.byref(v)), "fmc_batch_get_tile")
         return v.value
+
+    def set_tile_shape(self, threads: int, kpt: int = -1, kgroup: int = 4, kunroll: int = 4):
+        """The whole tile shape (fmc_batch_set_tile_shape)."""
+        check(self.L.fmc_batch_set_tile_shape(self.h, threads, kpt, kgroup, kunroll),
+              "fmc_batch_set_tile_shape")
+
+    def tile_shape(self) -> dict:
+        """The tile shape in use: threads, kpt (0 streaming), kgroup, kunroll,
+        tuned (0 none yet, 1 set, 2 autotuned)."""
+        v = [C.c_int() for _ in range(5)]
+        check(self.L.fmc_batch_get_tile_shape(self.h, *[C.byref(x) for x in v]),
+              "fmc_batch_get_tile_shape")
+        return dict(zip(("threads", "kpt", "kgroup", "kunroll", "tuned"),
+                        (x.value for x in v)))
 
 
 def footprint(ctx, precision: int = PREC_FP64) -> int:

@@ -54,7 +54,9 @@ class ChainStats(C.Structure):
                 ("win", ((C.c_double * 4) * 5) * 5),
                 ("sel_window", C.c_longlong), ("sel_count", C.c_longlong),
                 ("sel_base", C.c_double * 4), ("sel_mean", C.c_double * 6),
-                ("sel_second", C.c_double * 6), ("sel_var", C.c_double * 6)]
+                ("sel_second", C.c_double * 6), ("sel_var", C.c_double * 6),
+                ("tile_threads", C.c_int), ("tile_kpt", C.c_int),
+                ("tile_kgroup", C.c_int), ("tile_kunroll", C.c_int)]
 
 
 _lib = None
@@ -107,6 +109,8 @@ _SIGS = {
     "fmc_batch_timing": [C.c_void_p, C.c_int, DP, C.POINTER(C.c_longlong)],
     "fmc_batch_set_tile": [C.c_void_p, C.c_int],
     "fmc_batch_get_tile": [C.c_void_p, IP],
+    "fmc_batch_set_tile_shape": [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int],
+    "fmc_batch_get_tile_shape": [C.c_void_p, IP, IP, IP, IP, IP],
     "fmc_chain_footprint": [C.c_void_p, C.c_int, C.POINTER(C.c_longlong)],
     "fmc_version": [IP, IP],
     "fmc_device_ok": [],

@@ -227,6 +227,9 @@ typedef struct fmc_chain_stats {
   long long sel_count;
   double sel_base[4];
   double sel_mean[6], sel_second[6], sel_var[6];
+  /* the tile shape of the launches that moved the chain (all 0 before its
+   * first launch); see fmc_batch_set_tile_shape */
+  int tile_threads, tile_kpt, tile_kgroup, tile_kunroll;
 } fmc_chain_stats;
 
 /* max_chains chains; precision FMC_PREC_*.  */
@@ -315,6 +318,22 @@ int fmc_batch_timing(fmc_batch *b, int reset, double *ms, long long *launches);
 #define FMC_TILE_STREAM 0x10000
 int fmc_batch_set_tile(fmc_batch *b, int threads);
 int fmc_batch_get_tile(const fmc_batch *b, int *threads);
+/* the whole tile shape: threads per work-group (256, 512); kpt, the
+ * register tile's k-vectors per thread (-1: the smallest that covers the
+ * largest Nk while it fits, else the streaming form; 0: the streaming
+ * form); kgroup, the k-vectors whose LDS reads issue together (2, 4, 8);
+ * kunroll, the real-space items each thread loads per pass (2, 4).  Only
+ * threads changes a chain's arithmetic.  fmc_batch_set_tile(b, 0) lets the
+ * autotuner time every compiled shape on real moves before the next launch
+ * (the k-vector enumeration it tiles: ewald.f:141-213); get reports the
+ * shape in use and *tuned = 0 (none yet), 1 (set) or 2 (autotuned).  A
+ * checkpoint restored into a batch without a tile sets the chain's; into a
+ * batch whose tile has another thread count it is refused
+ * (FMC_ERR_STATE). */
+int fmc_batch_set_tile_shape(fmc_batch *b, int threads, int kpt, int kgroup,
+                             int kunroll);
+int fmc_batch_get_tile_shape(const fmc_batch *b, int *threads, int *kpt,
+                             int *kgroup, int *kunroll, int *tuned);
 /* bytes of device memory one chain of this context needs (admission) */
 int fmc_chain_footprint(const fmc_ctx *ctx, int precision,
                         long long *bytes);
