@@ -1155,6 +1155,7 @@ __global__ void __launch_bounds__(BT)
   __shared__ int pin_fast[2], pin_ok[2];
   __shared__ double red[4][BT / kWave];
   __shared__ int stop_flag;
+  __shared__ int recomp_now;   // this move ends the chain's recompute cadence
   __shared__ int shift_lo, shift_hi;
   __shared__ SysQ Z;   // the system's small constants, for the whole launch
   BChain& C = Cbuf[0];   // the launch's first state (the loop rebinds it)
@@ -1553,6 +1554,11 @@ __global__ void __launch_bounds__(BT)
         }
       }
       if (!spec_hit) decide(Z, C, R, D, a, b, c, nk);
+      {
+        // one 64-bit modulo on thread 0 instead of one per thread
+        const BChain& Ca = Cbuf[csel];
+        recomp_now = (Ca.recomp_every > 0 && Ca.steps % Ca.recomp_every == 0) ? 1 : 0;
+      }
       shift_lo = 0;
       shift_hi = 0;
       if (D.accepted && kind == FMC_DELETE) {
@@ -1616,9 +1622,9 @@ __global__ void __launch_bounds__(BT)
     }
     // FP64 recompute every recomp_every moves of this chain, inside the
     // launch: S(k) <- exact, full energy, drift (as recompute_kernel; the
-    // same exact_sk, so the same S bits).  Uniform branch (LDS values
-    // written before the last barrier).
-    if (C.recomp_every > 0 && C.steps % C.recomp_every == 0) {
+    // same exact_sk, so the same S bits).  Uniform branch (thread 0 set the
+    // flag before the last barrier).
+    if (recomp_now) {
       double erec = 0.0, dev = 0.0;
       if (STREAM) {
         for (int k = tid; k < nk; k += BT) {
