@@ -41,6 +41,10 @@ struct fmc_ctx {
   unsigned long long trial_seq = 0;
   hipStream_t stream = nullptr;
   int nb_recip_max = 64, nb_real_max = 64;
+  // where fmc_trial / fmc_commit spend their wall time (fmc_ctx_timing):
+  // argument set-up on the host, the launch call, the wait for the sums
+  long long n_trials = 0, n_commits = 0;
+  double t_prep = 0.0, t_launch = 0.0, t_wait = 0.0, t_commit = 0.0;
   // pending trial
   int pending = 0, pend_gt = 0, pend_mol = 0;
   double pend_x[fmc::kMaxSites], pend_y[fmc::kMaxSites], pend_z[fmc::kMaxSites];

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -206,6 +207,17 @@ __global__ void __launch_bounds__(kTrialBS) trial_kernel(TrialParams P) {
   double e1 = 0.0, e2 = 0.0, ev = 0.0;
   if (static_cast<int>(blockIdx.x) < P.nb_recip) {
     // ---- reciprocal part (ewald1_guest, ewald.f:97-331) ----
+    // the first k of this thread is loaded before the phase tables are
+    // built, so its loads overlap the tables' arithmetic
+    const int k0 = blockIdx.x * kTrialBS + tid;
+    int v0 = 0;
+    double ak0 = 0.0, sre0 = 0.0, sim0 = 0.0;
+    if (k0 < P.nk) {
+      v0 = P.kv[k0];
+      ak0 = P.ak[k0];
+      sre0 = P.sre[k0];
+      sim0 = P.sim[k0];
+    }
     const int ntab = 2 * P.ns * 3;
     if (tid < ntab) {
       const int set = tid / (P.ns * 3);
@@ -218,8 +230,9 @@ __global__ void __launch_bounds__(kTrialBS) trial_kernel(TrialParams P) {
       phase_table(f, P.kmax[d], &tab[set][s][d][0]);
     }
     __syncthreads();
-    for (int k = blockIdx.x * kTrialBS + tid; k < P.nk; k += P.nb_recip * kTrialBS) {
-      const int v = P.kv[k];
+    for (int k = k0; k < P.nk; k += P.nb_recip * kTrialBS) {
+      const bool first = k == k0;
+      const int v = first ? v0 : P.kv[k];
       const int l = unpack_l(v), m = unpack_m(v), n = unpack_n(v);
       const int am = m < 0 ? -m : m, an = n < 0 ? -n : n;
       double2 so = make_double2(0.0, 0.0), sn = make_double2(0.0, 0.0);
@@ -237,7 +250,7 @@ __global__ void __launch_bounds__(kTrialBS) trial_kernel(TrialParams P) {
           sn.x = sn.x + t.x;
           sn.y = sn.y + t.y;
         }
-      const double cold = P.sre[k], sold = P.sim[k];
+      const double cold = first ? sre0 : P.sre[k], sold = first ? sim0 : P.sim[k];
       double c, s;
       if (P.kind == FMC_INSERT) {
         c = sn.x + cold;
@@ -251,7 +264,7 @@ __global__ void __launch_bounds__(kTrialBS) trial_kernel(TrialParams P) {
       }
       P.nre[k] = c;
       P.nim[k] = s;
-      e1 += P.ak[k] * (c * c + s * s - cold * cold - sold * sold);
+      e1 += (first ? ak0 : P.ak[k]) * (c * c + s * s - cold * cold - sold * sold);
     }
   } else {
     // ---- real space: the move's halves (old sites for a delete, new for
@@ -1042,9 +1055,16 @@ static int wait_trial(fmc_ctx* c, unsigned long long seq, int nb, double v[3]) {
   return FMC_OK;
 }
 
+namespace {
+double secs(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+  return std::chrono::duration<double>(b - a).count();
+}
+}  // namespace
+
 extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
                          const double* x, const double* y, const double* z,
                          double* out) {
+  const auto t0 = std::chrono::steady_clock::now();
   if (!c || !c->sys.finalized) return FMC_ERR_STATE;
   {
     const int r0 = ensure_device(c);
@@ -1105,14 +1125,17 @@ extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
   static void (*const kTrialForms[5])(TrialParams) = {trial_kernel<0>, trial_kernel<1>,
                                                       trial_kernel<2>, trial_kernel<3>,
                                                       trial_kernel<4>};
+  const auto t1 = std::chrono::steady_clock::now();
   hipLaunchKernelGGL(kTrialForms[ns <= 4 ? ns : 0], dim3(P.nb_recip + P.nb_real), dim3(kTrialBS), 0,
                      c->stream, P);
   HIPCHECK(hipGetLastError());
+  const auto t2 = std::chrono::steady_clock::now();
   double v[3];
   {
     const int rw = wait_trial(c, P.seq, P.nb_recip + P.nb_real, v);
     if (rw) return rw;
   }
+  const auto t3 = std::chrono::steady_clock::now();
   double e1 = P.rvolm2 * kR4pie0 * v[0] / P.S.epsq;
   if (kind == FMC_INSERT) e1 = e1 + P.engsic;
   if (kind == FMC_DELETE) e1 = e1 - P.engsic;
@@ -1127,10 +1150,31 @@ extern "C" int fmc_trial(fmc_ctx* c, int kind, int gt, int imol,
     c->pend_y[s] = kind != FMC_DELETE ? y[s] : 0.0;
     c->pend_z[s] = kind != FMC_DELETE ? z[s] : 0.0;
   }
+  const auto t4 = std::chrono::steady_clock::now();
+  c->n_trials++;
+  c->t_prep += secs(t0, t1) + secs(t3, t4);
+  c->t_launch += secs(t1, t2);
+  c->t_wait += secs(t2, t3);
+  return FMC_OK;
+}
+
+extern "C" int fmc_ctx_timing(fmc_ctx* c, int reset, double* out) {
+  if (!c || !out) return FMC_ERR_ARG;
+  out[0] = static_cast<double>(c->n_trials);
+  out[1] = c->t_prep;
+  out[2] = c->t_launch;
+  out[3] = c->t_wait;
+  out[4] = static_cast<double>(c->n_commits);
+  out[5] = c->t_commit;
+  if (reset) {
+    c->n_trials = c->n_commits = 0;
+    c->t_prep = c->t_launch = c->t_wait = c->t_commit = 0.0;
+  }
   return FMC_OK;
 }
 
 extern "C" int fmc_commit(fmc_ctx* c) {
+  const auto t0 = std::chrono::steady_clock::now();
   if (!c || !c->pending) return FMC_ERR_STATE;
   const int gt = c->pend_gt;
   const int ns = c->sys.guests[gt].ns;
@@ -1189,6 +1233,8 @@ extern "C" int fmc_commit(fmc_ctx* c) {
   }
   c->cur = 1 - c->cur;  // S(k) <- S_new
   c->pending = 0;
+  c->n_commits++;
+  c->t_commit += secs(t0, std::chrono::steady_clock::now());
   return FMC_OK;
 }
 

@@ -102,6 +102,12 @@ module fmc_abi
       integer(c_int), value :: kind, gt, imol
       real(c_double) :: x(*), y(*), z(*), out(3)
     end function
+    integer(c_int) function fmc_ctx_timing(ctx, reset, out) bind(C, name='fmc_ctx_timing')
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: reset
+      real(c_double) :: out(6)
+    end function
     integer(c_int) function fmc_commit(ctx) bind(C, name='fmc_commit')
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx

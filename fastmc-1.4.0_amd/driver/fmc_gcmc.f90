@@ -207,6 +207,12 @@ program fmc_gcmc
   integer :: nfw
   ! control
   integer :: mcsteps, eqsteps, nnumg, maxguest, seed(4), values(8)
+  real(8) :: tim(6), t_rst
+  integer(8) :: cw0, cw1
+  ! the rigid framework's REVCON lines (label, atom number, coordinates),
+  ! formatted once; the atom numbers are rewritten when they shift
+  character(len=1), allocatable :: fw_text(:)
+  integer :: fw_first = -1
   ! selectivity windows (gcmc.f:1887-1916)
   integer :: nwind, avcount
   integer(8) :: rollcount
@@ -669,6 +675,8 @@ program fmc_gcmc
   allocate(gasmol(maxsamp, ntpguest), gaseng(maxsamp, ntpguest))
   st_n = 0.d0; st_e = 0.d0; st_en = 0.d0; st_n2 = 0.d0; st_e2 = 0.d0
   avcount = 0; rollcount = 0; wsum = 0.d0; seltot = 0.d0; selwin = 0.d0
+  call ck(fmc_ctx_timing(ctx, 1, tim), 'fmc_ctx_timing')
+  t_rst = 0.d0
   call system_clock(c0, crate)
   do while (lgchk)
     gcmccount = gcmccount + 1
@@ -697,7 +705,12 @@ program fmc_gcmc
     end if
 
     ! restart files every 1000 moves (gcmc.f:737-740)
-    if (mod(gcmccount, 1000_8) == 0) call write_restart()
+    if (mod(gcmccount, 1000_8) == 0) then
+      call system_clock(cw0)
+      call write_restart()
+      call system_clock(cw1)
+      t_rst = t_rst + dble(cw1 - cw0) / dble(crate)
+    end if
 
     ig = 1
     if (ntpguest > 1) ig = floor(duni() * ntpguest) + 1
@@ -921,6 +934,16 @@ program fmc_gcmc
   ! ------------------------------------------------------------------
   write(6, "(/,a17,i12,a15,f13.3,a8)") 'time elapsed for ', gcmccount, ' gcmc steps : ', t1, ' seconds'
   write(6, "(/,a30,f16.2)") 'trial moves per second : ', dble(gcmccount) / max(t1, 1d-9)
+  ! where a move's time goes: the C ABI's trial and commit calls (host
+  ! set-up, launch, wait for the sums) and the rest of the move loop
+  call ck(fmc_ctx_timing(ctx, 0, tim), 'fmc_ctx_timing')
+  if (tim(1) > 0.d0) then
+    write(6, "(/,'microseconds per move:',f10.3,'  in fmc_trial: set-up',f8.3,'  launch',f8.3, &
+      & '  wait',f8.3,'  fmc_commit',f8.3,'  restart files',f8.3,'  the rest',f8.3)") &
+      1d6 * t1 / gcmccount, 1d6 * tim(2) / gcmccount, 1d6 * tim(3) / gcmccount, &
+      1d6 * tim(4) / gcmccount, 1d6 * tim(6) / gcmccount, 1d6 * t_rst / gcmccount, &
+      1d6 * (t1 - tim(2) - tim(3) - tim(4) - tim(6) - t_rst) / gcmccount
+  end if
   write(6, "(/,a30,i12)") 'total accepted steps : ', accept_ins + accept_del + &
     sum(accept_disp) + sum(accept_tran) + accept_rota
   if (ins_count > 0) write(6, "(/,3x,a21,f15.9)") 'insertion ratio: ', dble(accept_ins) / dble(ins_count)
@@ -1171,10 +1194,26 @@ contains
     ! 3380), and the unformatted REVIVE
     integer :: u, nn, ng, m2, s2, kk, a, ns0
     integer, allocatable :: num(:)
-    open(newunit=u, file='branch01/REVCON', form='formatted', status='replace')
-    write(u, '(a80)') cfgname
-    write(u, '(2i10,e20.10)') 0, imcon, dele_last
-    if (imcon > 0) write(u, '(3f20.12)') cell
+    character(len=80) :: title
+    character(len=40) :: head
+    character(len=60) :: vec(3)
+    character(len=18) :: lab
+    character(len=1), parameter :: nl = achar(10)
+    ! REVCON's records are made by internal writes with the reference's
+    ! edit descriptors and go out as bytes, each ended by a newline (the
+    ! record a formatted sequential write makes); the framework block is
+    ! one string of ready-made lines.  Both files are rewritten in place
+    ! and cut at their new end (ENDFILE), which costs less than deleting
+    ! and creating them every 1000 moves
+    open(newunit=u, file='branch01/REVCON', form='unformatted', access='stream', &
+         status='unknown', position='rewind')
+    write(title, '(a80)') cfgname
+    write(head, '(2i10,e20.10)') 0, imcon, dele_last
+    write(u) title, nl, head, nl
+    if (imcon > 0) then
+      write(vec, '(3f20.12)') cell
+      write(u) vec(1), nl, vec(2), nl, vec(3), nl
+    end if
     a = 0
     do nn = 1, ntpmls
       ng = mol_gi(nn)
@@ -1183,23 +1222,27 @@ contains
         do s2 = 1, gns(ng)
           a = a + 1
           kk = (m2 - 1) * gns(ng) + s2
-          write(u, '(a8,i10)') mol_name(s2, nn), a
-          write(u, '(3g20.10)') gx(kk, ng), gy(kk, ng), gz(kk, ng)
+          write(lab, '(a8,i10)') mol_name(s2, nn), a
+          write(vec(1), '(3g20.10)') gx(kk, ng), gy(kk, ng), gz(kk, ng)
+          write(u) lab, nl, vec(1), nl
         end do
       end do
     end do
-    kk = 0
-    do nn = 1, ntpmls
-      if (mol_gi(nn) > 0) cycle
-      do m2 = 1, mol_nummols(nn)
-        do s2 = 1, mol_nsite(nn)
-          a = a + 1
-          kk = kk + 1
-          write(u, '(a8,i10)') mol_name(s2, nn), a
-          write(u, '(3g20.10)') fx(kk), fy(kk), fz(kk)
+    ! the framework never moves: its lines ('(a8,i10)' then '(3g20.10)',
+    ! 80 bytes with the newlines) are formatted on the first write; later
+    ! writes only renumber the atoms when the guest count has changed
+    if (nfw > 0) then
+      if (.not. allocated(fw_text)) call frame_text()
+      if (fw_first /= a) then
+        do kk = 1, nfw
+          call put_i10(fw_text, 80 * (kk - 1) + 9, a + kk)
         end do
-      end do
-    end do
+        fw_first = a
+      end if
+      write(u) fw_text
+      a = a + nfw
+    end if
+    endfile(u)
     close(u)
     allocate(num(ntpmls))
     do nn = 1, ntpmls
@@ -1215,7 +1258,8 @@ contains
       chainstats(kk + 4) = st_n2(ng)
       chainstats(kk + 5) = st_e2(ng)
     end do
-    open(newunit=u, file='branch01/REVIVE', form='unformatted', status='replace')
+    open(newunit=u, file='branch01/REVIVE', form='unformatted', status='unknown', &
+         position='rewind')
     write(u) production
     write(u) num
     write(u) chainstats(1:14 * ntpguest + 1)
@@ -1236,7 +1280,54 @@ contains
       end if
     end do
     write(u) energy(1:ntpguest)
+    endfile(u)
     close(u)
+  end subroutine
+
+  subroutine frame_text()
+    ! the framework's REVCON lines in FIELD order, atom numbers zero
+    character(len=18) :: lab
+    character(len=60) :: xyz
+    integer :: nn, m2, s2, kk, o
+    integer :: j
+    allocate(fw_text(80 * nfw))
+    kk = 0
+    do nn = 1, ntpmls
+      if (mol_gi(nn) > 0) cycle
+      do m2 = 1, mol_nummols(nn)
+        do s2 = 1, mol_nsite(nn)
+          kk = kk + 1
+          write(lab, '(a8,i10)') mol_name(s2, nn), 0
+          write(xyz, '(3g20.10)') fx(kk), fy(kk), fz(kk)
+          o = 80 * (kk - 1)
+          do j = 1, 18
+            fw_text(o + j) = lab(j:j)
+          end do
+          fw_text(o + 19) = achar(10)
+          do j = 1, 60
+            fw_text(o + 19 + j) = xyz(j:j)
+          end do
+          fw_text(o + 80) = achar(10)
+        end do
+      end do
+    end do
+    fw_first = -1
+  end subroutine
+
+  subroutine put_i10(buf, pos, v)
+    ! v >= 0 right-justified in buf(pos:pos+9), as the I10 edit writes it
+    character(len=1), intent(inout) :: buf(:)
+    integer, intent(in) :: pos, v
+    integer :: p, x
+    x = v
+    do p = pos + 9, pos, -1
+      if (x == 0 .and. p < pos + 9) then
+        buf(p) = ' '
+      else
+        buf(p) = achar(48 + mod(x, 10))
+        x = x / 10
+      end if
+    end do
   end subroutine
 
   subroutine block_stats(g)

@@ -86,6 +86,7 @@ _SIGS = {
     "fmc_tail_delta": [C.c_void_p, C.c_int, C.c_int, C.c_double, DP],
     "fmc_energy_full": [C.c_void_p, DP],
     "fmc_energy_full_noalloc": [C.c_void_p, DP],
+    "fmc_ctx_timing": [C.c_void_p, C.c_int, DP],
     "fmc_batch_create": [C.POINTER(C.c_void_p), C.c_int, C.c_int],
     "fmc_batch_destroy": [C.c_void_p],
     "fmc_batch_add_chain": [C.c_void_p, C.c_void_p, C.POINTER(ChainParams), IP],

@@ -126,6 +126,13 @@ int fmc_trial(fmc_ctx *ctx, int kind, int gtype, int imol, const double *x,
  * condense, utility_pack.f:3325-3423): S(k) <- S_new, coordinates updated;
  * a deleted molecule's successors shift down by one, as in gcmc.f:1127 */
 int fmc_commit(fmc_ctx *ctx);
+/* Wall time of this context's trials and commits (no reference
+ * counterpart; the reference times only the whole run, gcmc.f:2433-2490):
+ * out[0] trials, out[1] seconds of host work in fmc_trial (argument
+ * set-up, reading the sums), out[2] seconds in the kernel launch call,
+ * out[3] seconds waiting for the work-groups' sums, out[4] commits,
+ * out[5] seconds in fmc_commit; reset != 0 clears them */
+int fmc_ctx_timing(fmc_ctx *ctx, int reset, double *out);
 /* gstlrcorrect (vdw_terms.f:684-745) for N(gtype)+sign molecules, minus
  * elrc (gcmc.f:891-901, 1056-1069) */
 int fmc_tail_delta(fmc_ctx *ctx, int gtype, int sign, double elrc,

@@ -145,6 +145,14 @@ int fmc_commit(fmc_ctx *c) {
   return orc_sys_commit(c->s);
 }
 
+/* the CPU restatement keeps no per-call timers: zeros */
+int fmc_ctx_timing(fmc_ctx *c, int reset, double *out) {
+  (void)c;
+  (void)reset;
+  for (int i = 0; i < 6; i++) out[i] = 0.0;
+  return FMC_OK;
+}
+
 int fmc_tail_delta(fmc_ctx *c, int gt, int sign, double elrc, double *d) {
   *d = orc_sys_tail_delta(c->s, gt, sign, elrc);
   return FMC_OK;

@@ -145,6 +145,53 @@ def test_gpu_driver_matches_oracle_driver(oracle_driver, tmp_path, name):
         assert np.allclose(a, b, rtol=0, atol=2e-6, equal_nan=True), (k, a, b)
 
 
+def test_revcon_after_the_guest_count_changes(oracle_driver, tmp_path):
+    """REVCON is rewritten every 1000 moves (gcmc.f:737-740).  The driver
+    formats the framework's lines once and renumbers their atoms when the
+    guest count moves them, so the REVCON of the last write (three writes
+    in a row at different loadings) must equal, byte for byte, the file the
+    Python writer makes from the REVIVE written with it.  Both files are
+    rewritten in place and cut at their end."""
+    from fastmc_amd import inputs
+    w, extra, _, _ = _workload("ar")
+    w.write(str(tmp_path), steps=3000, eqsteps=500)
+    p = tmp_path / "CONTROL"
+    p.write_text(p.read_text().replace("finish", "seed 12 34 56 78\ntrace\nfinish"))
+    # the files are rewritten in place: older, longer files in their place
+    # must be cut at the new end
+    (tmp_path / "branch01").mkdir()
+    for f in ("REVCON", "REVIVE"):
+        (tmp_path / "branch01" / f).write_bytes(b"\x07" * 2_000_000)
+    tr = _run(oracle_driver, str(tmp_path))
+    field = inputs.read_field(str(tmp_path / "FIELD"))
+    config = inputs.read_config(str(tmp_path / "CONFIG"), field)
+    ns = [len(m.names) for m in field.molecules if m.guest]
+
+    def guest_atoms(upto):
+        # trace rows: move number, guest (1-based), ..., that guest's count
+        n = [0] * len(ns)
+        for row in tr[tr[:, 0] < upto]:
+            n[int(row[1]) - 1] = int(row[4])
+        return sum(a * b for a, b in zip(n, ns)), n
+
+    # the files on disk come from the last write, made before that move
+    lastw = int(tr[-1, 0]) // 1000 * 1000
+    at = [guest_atoms(m) for m in (lastw - 2000, lastw - 1000, lastw)]
+    assert len({a for a, _ in at}) == 3, at
+    rev = inputs.read_revive(str(tmp_path / "branch01" / "REVIVE"), field)
+    assert rev.nummols[:len(ns)] == at[2][1]
+    # production, counts, chain statistics, atom count, 3 arrays per
+    # molecule type, energies: nothing after them
+    recs = inputs.read_fortran_records(str(tmp_path / "branch01" / "REVIVE"))
+    assert len(recs) == 4 + 3 * len(field.molecules) + 1
+    last = tr[tr[:, 0] < lastw][-1]
+    want = tmp_path / "REVCON.want"
+    inputs.write_revcon(str(want), config.title, config.imcon, config.cell,
+                        inputs.revcon_atoms(field, rev.coords),
+                        eng=last[5] if last[3] else 0.0)
+    assert (tmp_path / "branch01" / "REVCON").read_bytes() == want.read_bytes()
+
+
 def _restart_run_dir(d, ex_dir, steps=1000):
     """The reference's restart example set up as a restart: CONFIG, FIELD,
     branch01/REVIVE, and a CONTROL asking for `restart`, a fixed seed and a
