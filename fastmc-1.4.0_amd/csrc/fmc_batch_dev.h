@@ -763,30 +763,27 @@ __device__ __forceinline__ bool decision(const SysQ& Z, const BChain& C, const M
 // A bound on estep above which the move is certainly rejected for the
 // acceptance draw rande: energy_eval accepts when rande < pref exp(-beta
 // estep) (pref 1 for a displacement, the volume-pressure factor for an
-// insert or delete), so estep > E* = -ln(rande / pref) / beta rejects.  E* is
-// estimated in FP32 (relative error ~1e-7 of |ln rande| + |ln pref|, over
-// beta) and the bound adds a margin of 1e-4 (1 + |E*|), several hundred
-// times that; a move at or below the bound is decided exactly.  Returns
-// false when no bound applies (rande 0, a zero or non-finite prefactor, the
-// zero-pressure delete, a no-op).
+// insert or delete), so estep > E* = -ln(rande / pref) / beta rejects.  The
+// bound adds a margin of 1e-6 (1 + |E*|), far beyond the rounding of exp and
+// log; a move at or below it is decided exactly.  Returns false when no
+// bound applies (rande 0, a tiny or zero prefactor, the zero-pressure
+// delete, a no-op).
 static __device__ bool reject_bound(const BChain& C, const MoveDesc& D, double rande, double* thr) {
   const int ig = D.ig;
-  float pref;
+  double pref;
   if (D.kind == FMC_DISPLACE) {
-    pref = 1.0f;
+    pref = 1.0;
   } else if (D.kind == FMC_INSERT) {
-    pref = static_cast<float>(C.statvolm) * static_cast<float>(C.press[ig]) /
-           (static_cast<float>(C.nmol[ig] + 1) * static_cast<float>(kBoltz * C.temp));
+    pref = C.statvolm * C.press[ig] / (static_cast<double>(C.nmol[ig] + 1) * kBoltz * C.temp);
   } else if (D.kind == FMC_DELETE) {
     if (fabs(C.press[ig]) < 1e-7) return false;
-    pref = static_cast<float>(C.nmol[ig]) * static_cast<float>(kBoltz * C.temp) /
-           (static_cast<float>(C.statvolm) * static_cast<float>(C.press[ig]));
+    pref = static_cast<double>(C.nmol[ig]) * kBoltz * C.temp / (C.statvolm * C.press[ig]);
   } else {
     return false;
   }
-  if (!(rande > 0.0) || !(pref > 0.0f) || !(pref < 3.0e38f)) return false;
-  const float e = -(logf(static_cast<float>(rande)) - logf(pref)) / static_cast<float>(C.beta);
-  *thr = static_cast<double>(e) + 1e-4 * (1.0 + fabs(static_cast<double>(e)));
+  if (!(rande > 0.0) || !(pref > 1e-300) || !(pref < 1e300)) return false;
+  const double e = -log(rande / pref) / C.beta;
+  *thr = e + 1e-6 * (1.0 + fabs(e));
   return true;
 }
 
@@ -795,28 +792,6 @@ __device__ __forceinline__ double tail_change(const SysQ& Z, const BChain& C, co
   for (int g = 0; g < Z.ngt; ++g) n2[g] = C.nmol[g];
   n2[D.ig] += (D.kind == FMC_INSERT) ? 1 : -1;
   return tail_of(Z, n2) - C.elrc;
-}
-
-// A move's decision inputs, from its chain state C, its proposal D and the
-// duni state after the proposal (table u, carry c0, indices ir, jr): the
-// acceptance draw (gcmc.f:949, 1116, 1336; read, not taken), the tail change
-// of an insert or delete (gcmc.f:891-901, 1056-1069) and the rejection
-// bound.
-__device__ __forceinline__ void decision_inputs(const SysQ& Z, const BChain& C,
-                                                const MoveDesc& D, const float* u, float c0,
-                                                int ir, int jr, double& rande, double& delrc,
-                                                int& fast, double& thr) {
-  float uni = u[ir - 1] - u[jr - 1];
-  if (uni < 0.0f) uni = uni + 1.0f;
-  float c = c0 - C.cd;
-  if (c < 0.0f) c = c + C.cm;
-  uni = uni - c;
-  if (uni < 0.0f) uni = uni + 1.0f;
-  rande = uni;
-  delrc = (D.kind == FMC_INSERT || D.kind == FMC_DELETE) ? tail_change(Z, C, D) : 0.0;
-  double t = 0.0;
-  fast = reject_bound(C, D, rande, &t) ? 1 : 0;
-  thr = t;
 }
 
 // Acceptance and bookkeeping on the serial lane (gcmc.f:938-1002, 1106-1172,
@@ -985,20 +960,13 @@ __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveD
                                              int wave, int nw, int lane,
                                              Cplx<double> (*tab)[3][kBTab],
                                              Cplx<double> (*tabp)[3][kBTab], Cplx<T>* xy_l,
-                                             Cplx<T>* zt, int nlm,
-                                             const short (*prow)[kLdsTypes]) {
+                                             Cplx<T>* zt, int nlm) {
   const int np = D.np, ig = D.ig, kind = D.kind;
   const int ntp = Y.S.ntpatm;
   const int w2 = 2 * Z.kmax[1] + 1, k3 = Z.kmax[2], nz = 2 * k3 + 1;
   const float rw2 = 1.0f / w2;
   const bool delta = DELTA && kind == FMC_DISPLACE;
-  // with waves to spare, two waves share a position: each computes the
-  // phase factors into a table slot of its own and half of the q X Y row
-  const int spp = (2 * np <= nw && np <= kMaxSites) ? 2 : 1;
-  for (int pw = wave; pw < np * spp; pw += nw) {
-    const int p = spp == 2 ? pw >> 1 : pw;
-    const int part = spp == 2 ? (pw & 1) : 0;
-    const int tp = p + part * kMaxSites;   // the table slot of this wave
+  for (int p = wave; p < np; p += nw) {
     const int set = D.pset[p], s = D.psite[p];
     const double x = set ? D.nx[s] : D.ox[s];
     const double y = set ? D.ny[s] : D.oy[s];
@@ -1036,26 +1004,24 @@ __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveD
       double sv, cv;
       sincospi(2.0 * ph, &sv, &cv);
       if (!DELTA || own)
-        tab[tp][d][l] = {cv, sv};
+        tab[p][d][l] = {cv, sv};
       else
-        tabp[tp][d][l] = {cv, sv};
+        tabp[p][d][l] = {cv, sv};
     }
-    if (part == 0) {
-      if (lane == 0) {
-        D.plo[p] = cell_of(Z, x, y, z) * Y.gcap;
-        D.ppx[p] = x;
-        D.ppy[p] = y;
-        D.ppz[p] = z;
-        D.pce[p] = q / Z.epsq * kR4pie0;
-        D.pq[p] = pq;
-        D.pts[p] = ts;
-        D.psg[p] = (kind == FMC_DISPLACE && set == 0) ? -1 : 1;
-      }
-      // the site type's pair-table row, from the launch's copy in LDS
-      if (ntp <= kLdsTypes)
-        for (int tj = lane; tj < ntp; tj += kWave) D.ptab[p][tj] = prow[ig * kMaxSites + s][tj];
+    if (lane == 0) {
+      D.plo[p] = cell_of(Z, x, y, z) * Y.gcap;
+      D.ppx[p] = x;
+      D.ppy[p] = y;
+      D.ppz[p] = z;
+      D.pce[p] = q / Z.epsq * kR4pie0;
+      D.pq[p] = pq;
+      D.pts[p] = ts;
+      D.psg[p] = (kind == FMC_DISPLACE && set == 0) ? -1 : 1;
     }
-    // the phase factors of this slot, written by this wave, read back
+    if (ntp <= kLdsTypes)
+      for (int tj = lane; tj < ntp; tj += kWave)
+        D.ptab[p][tj] = static_cast<short>(Y.S.pairtab[ts * ntp + tj]);
+    // the phase factors of this position, written by this wave, read back
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -1063,11 +1029,11 @@ __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveD
     // new position's row is X_n Y_n - X_o Y_o
     Cplx<T>* xr = xy_l + p * nlm;
     const bool dxy = delta && set;
-    for (int r = lane + part * kWave; r < nlm; r += spp * kWave) {
+    for (int r = lane; r < nlm; r += kWave) {
       const int l = div_rc(r, w2, rw2), m = r - l * w2 - Z.kmax[1];
       const int am = m < 0 ? -m : m;
-      const Cplx<double> a = tab[tp][0][l];
-      const Cplx<double> b = tab[tp][1][am];
+      const Cplx<double> a = tab[p][0][l];
+      const Cplx<double> b = tab[p][1][am];
       double ox, oy;
       if (m >= 0) {
         ox = a.x * b.x - a.y * b.y;
@@ -1077,8 +1043,8 @@ __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveD
         oy = a.y * b.x - b.y * a.x;
       }
       if (dxy) {
-        const Cplx<double> c = tabp[tp][0][l];
-        const Cplx<double> e = tabp[tp][1][am];
+        const Cplx<double> c = tabp[p][0][l];
+        const Cplx<double> e = tabp[p][1][am];
         if (m >= 0) {
           ox -= c.x * e.x - c.y * e.y;
           oy -= c.y * e.x + e.y * c.x;
@@ -1090,18 +1056,16 @@ __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveD
       xr[r] = {static_cast<T>(pq * ox), static_cast<T>(pq * oy)};
     }
     // signed Z[n]; in delta form the old position's row is Z_n - Z_o
-    if (part == 0) {
-      const bool dz = delta && !set;
-      for (int i = lane; i < nz; i += kWave) {
-        const int n = i - k3;
-        const int an = n < 0 ? -n : n;
-        Cplx<double> c = tab[tp][2][an];
-        if (dz) {
-          const Cplx<double> e = tabp[tp][2][an];
-          c = {e.x - c.x, e.y - c.y};
-        }
-        zt[p * kZRow + i] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
+    const bool dz = delta && !set;
+    for (int i = lane; i < nz; i += kWave) {
+      const int n = i - k3;
+      const int an = n < 0 ? -n : n;
+      Cplx<double> c = tab[p][2][an];
+      if (dz) {
+        const Cplx<double> e = tabp[p][2][an];
+        c = {e.x - c.x, e.y - c.y};
       }
+      zt[p * kZRow + i] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
     }
   }
 }
@@ -1131,8 +1095,6 @@ __global__ void __launch_bounds__(BT)
   __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
   // a displacement's partner phase factors (mixed precision: delta form)
   __shared__ Cplx<double> tabp[MIXED ? 2 * kMaxSites : 1][3][kBTab];
-  // each guest site's row of the vdW pair table (ntpatm <= kLdsTypes)
-  __shared__ short prow[kMaxGuestTypes * kMaxSites][kLdsTypes];
   __shared__ Cplx<T> zt[2 * kMaxSites * kZRow];           // Z_p[n], signed n
   // Move descriptors and duni tables, two of each: while the waves work on
   // one move, the last wave proposes the next one on copies of the chain
@@ -1147,15 +1109,11 @@ __global__ void __launch_bounds__(BT)
   __shared__ int csel, dsel, rsel, spec_hit;   // current buffers, adopted?
   __shared__ float cur_c, spec_c;       // duni state after this / the next proposal
   __shared__ int cur_ir, cur_jr, spec_ir, spec_jr;
-  // a move's decision inputs, per descriptor buffer: its acceptance draw,
-  // tail change and the estep above which it is rejected (pin_fast: the
-  // bound applies); pin_ok: computed already (a prepared move's are
-  // computed while the previous move is decided)
-  __shared__ double pin_rande[2], pin_delrc[2], pin_thr[2];
-  __shared__ int pin_fast[2], pin_ok[2];
+  __shared__ double pre_rande, pre_delrc;   // this move's draw and tail change
+  __shared__ double pre_thr;                // estep above which it is rejected
+  __shared__ int pre_fast;                  // pre_thr applies
   __shared__ double red[4][BT / kWave];
   __shared__ int stop_flag;
-  __shared__ int recomp_now;   // this move ends the chain's recompute cadence
   __shared__ int shift_lo, shift_hi;
   __shared__ SysQ Z;   // the system's small constants, for the whole launch
   BChain& C = Cbuf[0];   // the launch's first state (the loop rebinds it)
@@ -1179,15 +1137,6 @@ __global__ void __launch_bounds__(BT)
   __syncthreads();
   const int nk = Y.nk;
   if (tid == 0) C.tile_code = pack_tile(BT, STREAM ? 0 : KPT, KG, KU);
-  if (Y.S.ntpatm <= kLdsTypes) {
-    const int ntp = Y.S.ntpatm;
-    for (int i = tid; i < kMaxGuestTypes * kMaxSites * ntp; i += BT) {
-      const int gs = i / ntp, tj = i - gs * ntp;
-      const int g = gs / kMaxSites, s = gs - g * kMaxSites;
-      if (g < Z.ngt && s < Z.ns[g])
-        prow[gs][tj] = static_cast<short>(Y.S.pairtab[Z.gtype[g][s] * ntp + tj]);
-    }
-  }
   // dynamic LDS: a_k [nkcap], packed k index [nkcap] (register form), then
   // the XY tables
   extern __shared__ double dyn_lds[];
@@ -1236,22 +1185,14 @@ __global__ void __launch_bounds__(BT)
     dsel = 0;
     rsel = 0;
     spec_hit = 0;
-    pin_ok[0] = pin_ok[1] = 0;
     R.c = C.c; R.cd = C.cd; R.cm = C.cm; R.ir = C.ir; R.jr = C.jr;
   }
   __syncthreads();
 
   // moves left until the next in-launch recompute (the chain's cadence)
 
-  // 512-thread tiles compute a prepared move's decision inputs on the last
-  // wave while the previous move is decided (the other waves build its
-  // tables); smaller tiles compute them with the move's energy
-  constexpr bool kPinAhead = BT >= 512;
   for (long long mv = 0; mv < nmoves; ++mv) {
     BChain& C = Cbuf[csel];
-    // the buffers this move starts from (thread 0 flips them only in the
-    // decision, after the reduction barrier)
-    const int ccur = csel, rcur = rsel;
     // C.steps and stop_flag were last written before the barrier ending the
     // previous move: every thread takes the same branch
     if (C.steps >= C.stop_at || stop_flag) break;
@@ -1268,12 +1209,15 @@ __global__ void __launch_bounds__(BT)
         R.cd = C.cd;
         R.cm = C.cm;
         propose(Z, C, R, Dbuf[dsel], tid);
-      }
-      if (tid == 0) {
-        cur_c = R.c;
-        cur_ir = R.ir;
-        cur_jr = R.jr;
-        if (fresh) pin_ok[dsel] = 0;
+        // the duni state after this proposal, published by the barrier
+        // below (an adopted proposal's was published by the decision that
+        // adopted it: no barrier follows here, and the waves that read it
+        // may be ahead of wave 0)
+        if (tid == 0) {
+          cur_c = R.c;
+          cur_ir = R.ir;
+          cur_jr = R.jr;
+        }
       }
     }
     if (fresh) __syncthreads();
@@ -1290,7 +1234,7 @@ __global__ void __launch_bounds__(BT)
       // a prepared one had them built while the previous move was decided
       if (fresh) {
         build_tables<T, MIXED>(Z, Y, D, tid / kWave, BT / kWave, tid % kWave, tab, tabp, xy_l,
-                               zt, nlm, prow);
+                               zt, nlm);
         __syncthreads();
       }
       const int ntp = Y.S.ntpatm;
@@ -1362,12 +1306,11 @@ __global__ void __launch_bounds__(BT)
             Cn.steps++;
             (void)production_sample(Cn);
           }
-          propose(Z, Cn, Rs, Dbuf[dcur ^ 1], lane);
+          propose(Z, Cn, Rs, Dbuf[dsel ^ 1], lane);
           if (lane == 0) {
             spec_c = Rs.c;
             spec_ir = Rs.ir;
             spec_jr = Rs.jr;
-            pin_ok[dcur ^ 1] = 0;
           }
         }
       }
@@ -1376,12 +1319,25 @@ __global__ void __launch_bounds__(BT)
       //      but the last wave.  It reads only what the set-up phases
       //      wrote, so no barrier separates it from the reciprocal part: a
       //      wave that finishes its k-vectors starts its pairs at once ----
-      // the decision's inputs, unless they were computed with the previous
-      // move's decision: by the first lane of the wave before the
-      // preparing one (it finishes its real-space share ahead of it)
-      if (tid == RT - kWave && !pin_ok[dcur])
-        decision_inputs(Z, C, D, rng_u[rcur], cur_c, cur_ir, cur_jr, pin_rande[dcur],
-                        pin_delrc[dcur], pin_fast[dcur], pin_thr[dcur]);
+      // the decision's inputs, by the first lane of the wave before it (it
+      // finishes its real-space share ahead of the preparing wave): the
+      // acceptance draw (read without taking it), the tail change, the
+      // rejection bound
+      if (tid == RT - kWave) {
+        const float* u = rng_u[rsel];
+        float uni = u[cur_ir - 1] - u[cur_jr - 1];
+        if (uni < 0.0f) uni = uni + 1.0f;
+        float c = cur_c - C.cd;
+        if (c < 0.0f) c = c + C.cm;
+        uni = uni - c;
+        if (uni < 0.0f) uni = uni + 1.0f;
+        const double rande = uni;
+        pre_rande = rande;
+        pre_delrc = (kind == FMC_INSERT || kind == FMC_DELETE) ? tail_change(Z, C, D) : 0.0;
+        double thr = 0.0;
+        pre_fast = reject_bound(C, D, rande, &thr) ? 1 : 0;
+        pre_thr = thr;
+      }
       if (tid < RT) {
         const int gcap = Y.gcap;
         const int nfwp = np * gcap;
@@ -1486,20 +1442,9 @@ __global__ void __launch_bounds__(BT)
     // it adopts the prepared proposal, and rebuilds them otherwise
     if (kind != 0 && tid >= kWave) {
       MoveDesc& Dn = Dbuf[dcur ^ 1];
-      if (Dn.kind != 0) {
-        const int w = tid / kWave;
-        if (!kPinAhead || w < BT / kWave - 1) {
-          build_tables<T, MIXED>(Z, Y, Dn, w - 1, BT / kWave - (kPinAhead ? 2 : 1), tid % kWave,
-                                 tab, tabp, xy_l, zt, nlm, prow);
-        } else if (tid % kWave == 0) {
-          // the prepared move's decision inputs: its chain state and duni
-          // state as the rejection of this move leaves them
-          decision_inputs(Z, Cbuf[ccur ^ 1], Dn, rng_u[rcur ^ 1], spec_c, spec_ir, spec_jr,
-                          pin_rande[dcur ^ 1], pin_delrc[dcur ^ 1], pin_fast[dcur ^ 1],
-                          pin_thr[dcur ^ 1]);
-          pin_ok[dcur ^ 1] = 1;
-        }
-      }
+      if (Dn.kind != 0)
+        build_tables<T, MIXED>(Z, Y, Dn, tid / kWave - 1, BT / kWave - 1, tid % kWave, tab,
+                               tabp, xy_l, zt, nlm);
     }
     if (tid == 0) {
       double a = 0.0, b = 0.0, c = 0.0;
@@ -1521,22 +1466,22 @@ __global__ void __launch_bounds__(BT)
       if (kind != 0 && !stop_flag && C.steps + 1 < C.stop_at && mv + 1 < nmoves) {
         double estep, e1x;
         bool rej;
-        if (pin_fast[dcur]) {
+        if (pre_fast) {
           // estep and the logged e1 as decide forms them; the bound first
           e1x = a;
           if (kind == FMC_INSERT) {
             e1x = e1x + Z.engsic[ig];
-            estep = (e1x + b + Z.e3[ig] + c + pin_delrc[dcur]) / C.engunit;
+            estep = (e1x + b + Z.e3[ig] + c + pre_delrc) / C.engunit;
           } else if (kind == FMC_DELETE) {
             e1x = e1x - Z.engsic[ig];
-            estep = (e1x - b - Z.e3[ig] - c + pin_delrc[dcur]) / C.engunit;
+            estep = (e1x - b - Z.e3[ig] - c + pre_delrc) / C.engunit;
           } else {
             estep = (e1x + b + c) / C.engunit;
           }
-          rej = estep > pin_thr[dcur] ||
-                !decision(Z, C, D, a, b, c, pin_delrc[dcur], pin_rande[dcur], estep, e1x);
+          rej = estep > pre_thr ||
+                !decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x);
         } else {
-          rej = !decision(Z, C, D, a, b, c, pin_delrc[dcur], pin_rande[dcur], estep, e1x);
+          rej = !decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x);
         }
         if (rej && !(kind == FMC_DISPLACE && estep < 0.0)) {
           spec_hit = 1;
@@ -1548,17 +1493,15 @@ __global__ void __launch_bounds__(BT)
           R.c = spec_c;
           R.ir = spec_ir;
           R.jr = spec_jr;
+          cur_c = spec_c;
+          cur_ir = spec_ir;
+          cur_jr = spec_jr;
           BChain& Cn = Cbuf[csel];
           if (Cn.win_len > 0 && Cn.steps > Cn.eq_steps) window_snapshot(Cn, Cn.sre + nk);
           log_move(Cn, move_slot(D), ig, 0, estep, e1x, b, c);
         }
       }
       if (!spec_hit) decide(Z, C, R, D, a, b, c, nk);
-      {
-        // one 64-bit modulo on thread 0 instead of one per thread
-        const BChain& Ca = Cbuf[csel];
-        recomp_now = (Ca.recomp_every > 0 && Ca.steps % Ca.recomp_every == 0) ? 1 : 0;
-      }
       shift_lo = 0;
       shift_hi = 0;
       if (D.accepted && kind == FMC_DELETE) {
@@ -1622,9 +1565,9 @@ __global__ void __launch_bounds__(BT)
     }
     // FP64 recompute every recomp_every moves of this chain, inside the
     // launch: S(k) <- exact, full energy, drift (as recompute_kernel; the
-    // same exact_sk, so the same S bits).  Uniform branch (thread 0 set the
-    // flag before the last barrier).
-    if (recomp_now) {
+    // same exact_sk, so the same S bits).  Uniform branch (LDS values
+    // written before the last barrier).
+    if (C.recomp_every > 0 && C.steps % C.recomp_every == 0) {
       double erec = 0.0, dev = 0.0;
       if (STREAM) {
         for (int k = tid; k < nk; k += BT) {
