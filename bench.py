@@ -160,7 +160,15 @@ def cpu_baseline(budget_s=20.0):
 # plain FP64 drop-in: Fortran driver over the C ABI on the 3392-atom supercell
 # ---------------------------------------------------------------------------
 
-def plain_fp64_rate(moves=1600):
+PLAIN_PARTS = ("fmc_trial set-up", "launch", "wait", "fmc_commit", "restart files",
+               "the rest")
+
+
+def plain_fp64_rate(moves=1600, parts=None):
+    """Moves/s of the driver's run; with a dict `parts`, also its
+    microseconds per move by part, as the driver's OUTPUT breaks them down
+    (the host time of fmc_trial's set-up, its kernel launch and its wait
+    for the sums; fmc_commit; the restart files; everything else)."""
     from fastmc_amd import synth
     exe = os.path.join(PKG, "bin", "fmc_gcmc")
     if not os.path.exists(exe):
@@ -177,6 +185,13 @@ def plain_fp64_rate(moves=1600):
             raise RuntimeError(r.stderr)
         out = open(os.path.join(d, "OUTPUT")).read()
         rate = float(re.search(r"trial moves per second :\s+([0-9.Ee+-]+)", out).group(1))
+        m = re.search(r"microseconds per move:\s*([0-9.]+)\s+in fmc_trial: set-up\s*([0-9.]+)"
+                      r"\s+launch\s*([0-9.]+)\s+wait\s*([0-9.]+)\s+fmc_commit\s*([0-9.]+)"
+                      r"\s+restart files\s*([0-9.]+)\s+the rest\s*(-?[0-9.]+)", out)
+        if parts is not None and m:
+            parts["total"] = float(m.group(1))
+            for k, v in zip(PLAIN_PARTS, m.groups()[1:]):
+                parts[k] = float(v)
         return rate
     finally:
         shutil.rmtree(d, ignore_errors=True)
@@ -409,7 +424,8 @@ def main():
     tile = b.tile_shape()
     b.close()
 
-    plain = None if args.no_plain else plain_fp64_rate()
+    plain_us = {}
+    plain = None if args.no_plain else plain_fp64_rate(parts=plain_us)
     fp64 = None if args.no_fp64 else batched_fp64_rate(
         budget_us=budget_us, recompute_every=args.recompute_every)
     de = de_error()
@@ -442,6 +458,7 @@ def main():
                    "guest_molecules_mean": sum(nmol) / len(nmol),
                    "guest_molecules_max": max(nmol)},
         "plain_fp64_moves_per_s": plain,
+        "plain_fp64_us_per_move": plain_us or None,
         "speedup_vs_plain_fp64": (value / plain) if plain else None,
         "batched_fp64_moves_per_s": fp64,
         "mixed_vs_fp64_dE": de,
