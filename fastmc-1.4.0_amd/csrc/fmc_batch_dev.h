@@ -342,6 +342,22 @@ __device__ __forceinline__ int cell_of(const SysQ& Z, double x, double y,
   return (idx[0] * Z.gdim[1] + idx[1]) * Z.gdim[2] + idx[2];
 }
 
+// Stage clocks (a profiling build only, -DFMC_STAGE_CLOCKS): each chain's
+// work-group sums s_memtime cycles per stage of its moves, and per wave the
+// end of its parallel-stage work and of its decision-stage work, into
+// g_stage_clocks[work-group][slot] (slots: kClk* below).  Each compiled
+// tile has its own copy, read by fmc_stage_clocks_<threads>_<kpt>.
+constexpr int kClkPropose = 0, kClkTables = 1, kClkParallel = 2, kClkDecision = 3,
+              kClkCommit = 4, kClkMoves = 5, kClkFresh = 6, kClkDisplace = 7,
+              kClkWaveParallel = 8, kClkWaveDecision = 16, kClkWaveRecip = 24, kClkSlots = 32;
+constexpr int kClkGroups = 4096;
+#ifdef FMC_STAGE_CLOCKS
+static __device__ unsigned long long g_stage_clocks[kClkGroups][kClkSlots];
+#define FMC_CLOCK(v) const unsigned long long v = __builtin_amdgcn_s_memtime()
+#else
+#define FMC_CLOCK(v)
+#endif
+
 template <typename T>
 struct Cplx {
   T x, y;
@@ -1114,6 +1130,9 @@ __global__ void __launch_bounds__(BT)
   __shared__ int pre_fast;                  // pre_thr applies
   __shared__ double red[4][BT / kWave];
   __shared__ int stop_flag;
+#ifdef FMC_STAGE_CLOCKS
+  __shared__ unsigned long long clk[kClkSlots];
+#endif
   __shared__ int shift_lo, shift_hi;
   __shared__ SysQ Z;   // the system's small constants, for the whole launch
   BChain& C = Cbuf[0];   // the launch's first state (the loop rebinds it)
@@ -1177,6 +1196,9 @@ __global__ void __launch_bounds__(BT)
                                 ? static_cast<long long>(__builtin_amdgcn_s_memrealtime())
                                 : 0;
   if (tid == 0) stop_flag = 0;
+#ifdef FMC_STAGE_CLOCKS
+  if (tid < kClkSlots) clk[tid] = 0;
+#endif
   __syncthreads();
   if (tid == 0) {
     for (int i = 0; i < 97; ++i) rng_u[0][i] = C.u[i];
@@ -1196,6 +1218,7 @@ __global__ void __launch_bounds__(BT)
     // C.steps and stop_flag were last written before the barrier ending the
     // previous move: every thread takes the same branch
     if (C.steps >= C.stop_at || stop_flag) break;
+    FMC_CLOCK(clk_start);
     // a prepared move is in place already (written before the last
     // barrier); only a fresh proposal needs one
     const bool fresh = !spec_hit;
@@ -1221,6 +1244,10 @@ __global__ void __launch_bounds__(BT)
       }
     }
     if (fresh) __syncthreads();
+    FMC_CLOCK(clk_proposed);
+#ifdef FMC_STAGE_CLOCKS
+    unsigned long long clk_tables = clk_proposed;
+#endif
     // dsel is written only by thread 0's decision, after the reduction barrier
     const int dcur = dsel;
     MoveDesc& D = Dbuf[dcur];
@@ -1236,6 +1263,9 @@ __global__ void __launch_bounds__(BT)
         build_tables<T, MIXED>(Z, Y, D, tid / kWave, BT / kWave, tid % kWave, tab, tabp, xy_l,
                                zt, nlm);
         __syncthreads();
+#ifdef FMC_STAGE_CLOCKS
+        clk_tables = __builtin_amdgcn_s_memtime();
+#endif
       }
       const int ntp = Y.S.ntpatm;
       const bool ldsrow = ntp <= kLdsTypes;
@@ -1274,6 +1304,10 @@ __global__ void __launch_bounds__(BT)
           }
         }
       }
+#ifdef FMC_STAGE_CLOCKS
+      if ((tid % kWave) == 0)
+        clk[kClkWaveRecip + tid / kWave] += __builtin_amdgcn_s_memtime() - clk_tables;
+#endif
       // ---- the next move's proposal, by the last wave, on copies of the
       //      chain state and duni table as this move leaves them if it is
       //      rejected (its acceptance draw taken, if it makes one) ----
@@ -1428,6 +1462,7 @@ __global__ void __launch_bounds__(BT)
     }
     // block reductions (fixed tree): DPP within each wave, then the waves'
     // totals in order
+    FMC_CLOCK(clk_wave_done);
     e1 = wave_sum_dpp(e1);
     e2 = wave_sum_dpp(e2);
     ev = wave_sum_dpp(ev);
@@ -1437,6 +1472,7 @@ __global__ void __launch_bounds__(BT)
       red[2][tid / kWave] = ev;
     }
     __syncthreads();
+    FMC_CLOCK(clk_reduced);
     // while thread 0 decides, waves 1.. build the tables of the prepared
     // next move (this move's are read no more); the next move uses them if
     // it adopts the prepared proposal, and rebuilds them otherwise
@@ -1509,7 +1545,9 @@ __global__ void __launch_bounds__(BT)
         shift_hi = C.nmol[ig] * ns;   // already decremented
       }
     }
+    FMC_CLOCK(clk_decided);
     __syncthreads();
+    FMC_CLOCK(clk_after_decision);
     {
     // the chain state after this move (the other buffer if it was rejected)
     BChain& C = Cbuf[csel];
@@ -1622,7 +1660,29 @@ __global__ void __launch_bounds__(BT)
       __syncthreads();
     }
     }
+#ifdef FMC_STAGE_CLOCKS
+    {
+      const unsigned long long clk_end = __builtin_amdgcn_s_memtime();
+      if (tid == 0) {
+        clk[kClkPropose] += clk_proposed - clk_start;
+        clk[kClkTables] += clk_tables - clk_proposed;
+        clk[kClkParallel] += clk_reduced - clk_tables;
+        clk[kClkDecision] += clk_after_decision - clk_reduced;
+        clk[kClkCommit] += clk_end - clk_after_decision;
+        clk[kClkMoves] += 1;
+        clk[kClkFresh] += fresh ? 1 : 0;
+        clk[kClkDisplace] += (kind == FMC_DISPLACE) ? 1 : 0;
+      }
+      if ((tid % kWave) == 0 && tid / kWave < 8) {
+        clk[kClkWaveParallel + tid / kWave] += clk_wave_done - clk_tables;
+        clk[kClkWaveDecision + tid / kWave] += clk_decided - clk_reduced;
+      }
+    }
+#endif
   }
+#ifdef FMC_STAGE_CLOCKS
+  if (tid < kClkSlots && blockIdx.x < kClkGroups) g_stage_clocks[blockIdx.x][tid] += clk[tid];
+#endif
   // write the state back
   BChain& Cf = Cbuf[csel];
   if (!STREAM) {

@@ -38,3 +38,24 @@ extern const KCfg FMC_TILES[] = {FMC_KG(2), FMC_KG(4)};
 extern const int FMC_NTILES = static_cast<int>(sizeof(FMC_TILES) / sizeof(FMC_TILES[0]));
 
 }  // namespace fmc
+
+#ifdef FMC_STAGE_CLOCKS
+// The profiling build's reader of this tile's stage clocks: the slots of
+// work-groups 0..ngroups-1 (the batch's active chains in order) into out
+// [ngroups][kClkSlots]; reset != 0 then clears them.  Returns 0, or -5 on a
+// HIP error.
+#define FMC_CLOCKS_FN FMC_CAT(fmc_stage_clocks, FMC_TILE_BT, FMC_TILE_KPT)
+extern "C" int FMC_CLOCKS_FN(int ngroups, int reset, unsigned long long* out) {
+  if (ngroups < 0 || ngroups > fmc::kClkGroups) return -1;
+  const size_t bytes = sizeof(unsigned long long) * fmc::kClkSlots * ngroups;
+  if (out && ngroups > 0 &&
+      hipMemcpyFromSymbol(out, HIP_SYMBOL(fmc::g_stage_clocks), bytes) != hipSuccess)
+    return -5;
+  if (reset) {
+    static unsigned long long zero[fmc::kClkGroups][fmc::kClkSlots];
+    if (hipMemcpyToSymbol(HIP_SYMBOL(fmc::g_stage_clocks), zero, sizeof(zero)) != hipSuccess)
+      return -5;
+  }
+  return 0;
+}
+#endif
