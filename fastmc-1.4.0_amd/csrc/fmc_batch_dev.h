@@ -779,27 +779,30 @@ __device__ __forceinline__ bool decision(const SysQ& Z, const BChain& C, const M
 // A bound on estep above which the move is certainly rejected for the
 // acceptance draw rande: energy_eval accepts when rande < pref exp(-beta
 // estep) (pref 1 for a displacement, the volume-pressure factor for an
-// insert or delete), so estep > E* = -ln(rande / pref) / beta rejects.  The
-// bound adds a margin of 1e-6 (1 + |E*|), far beyond the rounding of exp and
-// log; a move at or below it is decided exactly.  Returns false when no
-// bound applies (rande 0, a tiny or zero prefactor, the zero-pressure
-// delete, a no-op).
+// insert or delete), so estep > E* = -ln(rande / pref) / beta rejects.  E* is
+// estimated in FP32 (relative error ~1e-7 of |ln rande| + |ln pref|, over
+// beta, in kcal/mol) and the bound adds a margin of 1e-4 (1 + |E*|), several
+// hundred times that; a move at or below the bound is decided exactly.
+// Returns false when no bound applies (rande 0, a zero or non-finite
+// prefactor, the zero-pressure delete, a no-op).
 static __device__ bool reject_bound(const BChain& C, const MoveDesc& D, double rande, double* thr) {
   const int ig = D.ig;
-  double pref;
+  float pref;
   if (D.kind == FMC_DISPLACE) {
-    pref = 1.0;
+    pref = 1.0f;
   } else if (D.kind == FMC_INSERT) {
-    pref = C.statvolm * C.press[ig] / (static_cast<double>(C.nmol[ig] + 1) * kBoltz * C.temp);
+    pref = static_cast<float>(C.statvolm) * static_cast<float>(C.press[ig]) /
+           (static_cast<float>(C.nmol[ig] + 1) * static_cast<float>(kBoltz * C.temp));
   } else if (D.kind == FMC_DELETE) {
     if (fabs(C.press[ig]) < 1e-7) return false;
-    pref = static_cast<double>(C.nmol[ig]) * kBoltz * C.temp / (C.statvolm * C.press[ig]);
+    pref = static_cast<float>(C.nmol[ig]) * static_cast<float>(kBoltz * C.temp) /
+           (static_cast<float>(C.statvolm) * static_cast<float>(C.press[ig]));
   } else {
     return false;
   }
-  if (!(rande > 0.0) || !(pref > 1e-300) || !(pref < 1e300)) return false;
-  const double e = -log(rande / pref) / C.beta;
-  *thr = e + 1e-6 * (1.0 + fabs(e));
+  if (!(rande > 0.0) || !(pref > 0.0f) || !(pref < 3.0e38f)) return false;
+  const float e = -(logf(static_cast<float>(rande)) - logf(pref)) / static_cast<float>(C.beta);
+  *thr = static_cast<double>(e) + 1e-4 * (1.0 + fabs(static_cast<double>(e)));
   return true;
 }
 
