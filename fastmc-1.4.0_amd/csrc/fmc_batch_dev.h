@@ -417,16 +417,26 @@ __device__ __forceinline__ void trial_ds_tile(int np, const Cplx<T>* xy, int nlm
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
       if (g * G < kpt) {
+        Cplx<T> a[G], c[G];
 #pragma unroll
         for (int jj = 0; jj < G; ++jj) {
           const int j = g * G + jj;
           if (j >= KPT) break;
-          const Cplx<T> a = xp[kv[j] & 0xffff];
-          const Cplx<T> c = zp[static_cast<signed char>((kv[j] >> 16) & 0xff)];
-          dr[j] = fmat(a.x, c.x, dr[j]);
-          dr[j] = fmat(-a.y, c.y, dr[j]);
-          di[j] = fmat(a.y, c.x, di[j]);
-          di[j] = fmat(a.x, c.y, di[j]);
+          a[jj] = xp[kv[j] & 0xffff];
+          c[jj] = zp[static_cast<signed char>((kv[j] >> 16) & 0xff)];
+        }
+        // every read of the group is issued before the first multiply-add
+        // waits on one (the scheduler would otherwise interleave them
+        // pairwise under register pressure)
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int jj = 0; jj < G; ++jj) {
+          const int j = g * G + jj;
+          if (j >= KPT) break;
+          dr[j] = fmat(a[jj].x, c[jj].x, dr[j]);
+          dr[j] = fmat(-a[jj].y, c[jj].y, dr[j]);
+          di[j] = fmat(a[jj].y, c[jj].x, di[j]);
+          di[j] = fmat(a[jj].x, c[jj].y, di[j]);
         }
       }
     }
@@ -813,6 +823,33 @@ __device__ __forceinline__ double tail_change(const SysQ& Z, const BChain& C, co
   return tail_of(Z, n2) - C.elrc;
 }
 
+// The decision's inputs of move D on chain state C, from the duni state
+// (table u, indices ir and jr, carry c0) its proposal left: the acceptance
+// draw (read without taking it), the tail change of an insert or delete,
+// and the estep above which the move is certainly rejected (fast: the
+// bound applies).
+struct DecisionIn {
+  double rande, delrc, thr;
+  int fast;
+};
+
+__device__ __forceinline__ DecisionIn decision_inputs(const SysQ& Z, const BChain& C,
+                                                      const MoveDesc& D, const float* u,
+                                                      int ir, int jr, float c0) {
+  float uni = u[ir - 1] - u[jr - 1];
+  if (uni < 0.0f) uni = uni + 1.0f;
+  float c = c0 - C.cd;
+  if (c < 0.0f) c = c + C.cm;
+  uni = uni - c;
+  if (uni < 0.0f) uni = uni + 1.0f;
+  DecisionIn in;
+  in.rande = uni;
+  in.delrc = (D.kind == FMC_INSERT || D.kind == FMC_DELETE) ? tail_change(Z, C, D) : 0.0;
+  in.thr = 0.0;
+  in.fast = reject_bound(C, D, in.rande, &in.thr) ? 1 : 0;
+  return in;
+}
+
 // Acceptance and bookkeeping on the serial lane (gcmc.f:938-1002, 1106-1172,
 // 1326-1378, 1814-1870).
 static __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
@@ -1090,6 +1127,315 @@ __device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveD
 }
 
 // ---------------------------------------------------------------------------
+// Real space of a trial: ewald2 + srfrce (ewald.f:632-694,
+// vdw_terms.f:398-498) of every trial position against the framework atoms
+// of its candidate cell and against every guest atom
+// ---------------------------------------------------------------------------
+// Minimum image (min_image's operations, utility_pack.f:1138-1238): a
+// cubic or orthorhombic cell (imcon 1-2) from cell lengths and inverses
+// held in registers, a parallelepiped (imcon 3) through fractional
+// coordinates.
+struct OrthoCell {
+  double a0, a1, a2, i0, i1, i2;
+};
+
+__device__ __forceinline__ OrthoCell ortho_cell(const DevSys& S) {
+  const bool cubic = S.imcon != 2;   // imcon 1 takes cell(1) on every axis
+  return {S.cell[0], cubic ? S.cell[0] : S.cell[4], cubic ? S.cell[0] : S.cell[8],
+          S.inv[0], cubic ? S.inv[0] : S.inv[1], cubic ? S.inv[0] : S.inv[2]};
+}
+
+template <bool TRI>
+__device__ __forceinline__ void image(const DevSys& S, const OrthoCell& o, double& x, double& y,
+                                      double& z) {
+#pragma clang fp contract(off)
+  if (TRI) {
+    const double sx = S.rcell[0] * x + S.rcell[3] * y + S.rcell[6] * z;
+    const double sy = S.rcell[1] * x + S.rcell[4] * y + S.rcell[7] * z;
+    const double sz = S.rcell[2] * x + S.rcell[5] * y + S.rcell[8] * z;
+    const double ax = sx - round(sx), ay = sy - round(sy), az = sz - round(sz);
+    x = S.cell[0] * ax + S.cell[3] * ay + S.cell[6] * az;
+    y = S.cell[1] * ax + S.cell[4] * ay + S.cell[7] * az;
+    z = S.cell[2] * ax + S.cell[5] * ay + S.cell[8] * az;
+  } else {
+    x -= o.a0 * round(o.i0 * x);
+    y -= o.a1 * round(o.i1 * y);
+    z -= o.a2 * round(o.i2 * z);
+  }
+}
+
+// One pair's distance-dependent inputs: whether it counts (r^2 < rcut^2),
+// and the interpolation rows and fractions of its erfc and vdW tables (a
+// pair that does not count reads entry 0 and contributes nothing).
+// two doubles from a 16-byte aligned global address, one 16-byte load
+typedef double Dbl2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ Dbl2 gload2(const double* p) {
+  return *(const __attribute__((address_space(1))) Dbl2*)(p);
+}
+
+struct PairSlot {
+  double pe, pv;   // fractional grid offsets
+  int le, lv;      // first table entries read
+  bool in;
+};
+
+__device__ __forceinline__ PairSlot pair_slot(double rsq, bool ok, double rcut2, double rdrewd,
+                                              double rdlrpot, int imax) {
+#pragma clang fp contract(off)
+  PairSlot s;
+  s.in = ok && rcut2 > rsq;
+  const double r = sqrt(s.in ? rsq : 1.0);
+  const double ue = r * rdrewd;
+  int l = static_cast<int>(ue);
+  l = l > imax ? imax : l;
+  s.pe = ue - static_cast<double>(l);
+  s.le = s.in ? l : 0;
+  const double uv = r * rdlrpot;
+  l = static_cast<int>(uv);
+  l = l > imax ? imax : l;
+  s.pv = uv - static_cast<double>(l);
+  s.lv = s.in ? l : 0;
+  return s;
+}
+
+// the vdW table of each item's (site type, atom type) pair, -1 for none:
+// from the position's pairtab row in LDS, or from pairtab itself
+template <int KU>
+__device__ __forceinline__ void pair_rows(const DevSys& S, const MoveDesc& D, bool ldsrow,
+                                          const int (&pp)[KU], const int (&ty)[KU],
+                                          int (&vk)[KU]) {
+  if (ldsrow) {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) vk[u] = D.ptab[pp[u]][ty[u]];
+  } else {
+#pragma unroll
+    for (int u = 0; u < KU; ++u) vk[u] = gload(S.pairtab + D.pts[pp[u]] * S.ntpatm + ty[u]);
+  }
+}
+
+// interp3's formula on table values already loaded
+__device__ __forceinline__ double interp3v(double v0, double v1, double v2, double p) {
+#pragma clang fp contract(off)
+  const double t1 = v0 + (v1 - v0) * p;
+  const double t2 = v1 + (v2 - v1) * (p - 1.0);
+  return t1 + (t2 - t1) * p * 0.5;
+}
+
+// Thread t of nt takes items t, t + nt, t + 2 nt, ... of the flattened list
+// (the candidate records of every position, np * gcap of them, then every
+// guest atom of every position) and adds their terms in that order, signed
+// per position, into e2 (Coulomb) and ev (vdW).  KU items per pass: the
+// pass issues every record load, then every table load, before it uses
+// any of them, and no load sits under a branch (an item past the end or
+// excluded reads a valid slot and is masked), so a pass waits for one
+// record latency and one table latency.  The terms and their order are
+// those of pair_fp64_vk per item, whatever KU is.
+template <int KU, bool TRI>
+__device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const BChain& C,
+                                           const MoveDesc& D, int t, int nt, double& e2,
+                                           double& ev) {
+  // no contraction: an item's terms round the same way in every kernel
+  // shape that inlines this (the oracle's build contracts nothing either)
+#pragma clang fp contract(off)
+  const DevSys& S = Y.S;
+  const OrthoCell oc = ortho_cell(S);
+  const double rcut2 = S.rcut2, rdrewd = S.rdrewd, rdlrpot = S.rdlrpot;
+  const int imax = S.mxgrid - 2;
+  const size_t mx1 = static_cast<size_t>(S.mxgrid) + 1;
+  const double* __restrict__ erc = S.erc;
+  const double* __restrict__ vvv = S.vvv;
+  const int ntp = S.ntpatm;
+  const bool ldsrow = ntp <= kLdsTypes;
+  const int gcap = Y.gcap;
+  const int np = D.np;
+  const int nfwp = np * gcap;
+  const float rgcap = 1.0f / gcap;
+  const Rec* __restrict__ grec = Y.grec;
+  const int* __restrict__ grec_t = Y.grec_t;
+  double a2 = 0.0, av = 0.0;
+  // the terms of KU items whose separations are known
+  auto finish = [&](const PairSlot (&sl)[KU], const double (&chg)[KU], const int (&vk)[KU],
+                    const int (&pp)[KU]) {
+    double e0[KU], e1[KU], e2v[KU], w0[KU], w1[KU], w2[KU];
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      e0[u] = e1[u] = e2v[u] = w0[u] = w1[u] = w2[u] = 0.0;
+      // tables only for items some lane of the wave counts
+      if (__builtin_amdgcn_ballot_w64(sl[u].in) != 0) {
+        const double* te = erc + sl[u].le;
+        const double* tv =
+            vvv + static_cast<size_t>(vk[u] >= 0 && sl[u].in ? vk[u] : 0) * mx1 + sl[u].lv;
+        e0[u] = gload(te);
+        e1[u] = gload(te + 1);
+        e2v[u] = gload(te + 2);
+        w0[u] = gload(tv);
+        w1[u] = gload(tv + 1);
+        w2[u] = gload(tv + 2);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      double b2 = 0.0, bv = 0.0;
+      if (sl[u].in && fabs(chg[u]) > 1e-10)
+        b2 = b2 + interp3v(e0[u], e1[u], e2v[u], sl[u].pe) * chg[u];
+      if (sl[u].in && vk[u] >= 0) bv = bv + interp3v(w0[u], w1[u], w2[u], sl[u].pv);
+      if (D.psg[pp[u]] < 0) {
+        a2 -= b2;
+        av -= bv;
+      } else {
+        a2 += b2;
+        av += bv;
+      }
+    }
+  };
+  // framework candidates: item i is slot i mod gcap of position i / gcap
+  for (int i = t; i < nfwp; i += KU * nt) {
+    int pp[KU], slot[KU];
+    bool ok[KU];
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const int idx = i + u * nt;
+      ok[u] = idx < nfwp;
+      const int ic = ok[u] ? idx : i;
+      const int p = div_rc(ic, gcap, rgcap);
+      pp[u] = p;
+      slot[u] = D.plo[p] + (ic - p * gcap);
+    }
+    Dbl2 xy[KU], zq[KU];
+    int ty[KU];
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      xy[u] = Dbl2{0.0, 0.0};
+      zq[u] = Dbl2{0.0, 0.0};
+      ty[u] = 0;
+      // the wave's first lane holds its smallest item: past the end there,
+      // past the end on every lane
+      if (__builtin_amdgcn_readfirstlane(i + u * nt) < nfwp) {
+        xy[u] = gload2(&grec[slot[u]].x);
+        zq[u] = gload2(&grec[slot[u]].z);
+        ty[u] = gload(grec_t + slot[u]);
+      }
+    }
+    int vk[KU];
+#pragma unroll
+    for (int u = 0; u < KU; ++u) ty[u] = ty[u] < 0 ? 0 : (ty[u] >= ntp ? ntp - 1 : ty[u]);
+    pair_rows(S, D, ldsrow, pp, ty, vk);
+    PairSlot sl[KU];
+    double chg[KU];
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const int p = pp[u];
+      double dx = D.ppx[p] - xy[u].x, dy = D.ppy[p] - xy[u].y, dz = D.ppz[p] - zq[u].x;
+      image<TRI>(S, oc, dx, dy, dz);
+      const double rsq = dx * dx + dy * dy + dz * dz;
+      sl[u] = pair_slot(rsq, ok[u], rcut2, rdrewd, rdlrpot, imax);
+      chg[u] = D.pce[p] * zq[u].y;
+    }
+    finish(sl, chg, vk, pp);
+  }
+  // guest atoms: item i is atom (i - nfwp) mod ng of position (i - nfwp) / ng,
+  // the moving molecule's own atoms left out
+  const int ng = D.nguest;
+  const int total = nfwp + np * ng;
+  // this thread's first item past the candidates (the last candidate pass
+  // masked the ones it reached)
+  int i = t < nfwp ? t + ((nfwp - t + nt - 1) / nt) * nt : t;
+  if (i < total) {
+    const int ig = D.ig, ns = D.ns;
+    const int own0 = (D.kind != FMC_INSERT) ? D.gpre[ig] + D.imol * ns : -1;
+    const float rng = 1.0f / max(ng, 1);
+    for (; i < total; i += KU * nt) {
+      int pp[KU], gg[KU], la[KU];
+      bool ok[KU];
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int idx = i + u * nt;
+        const bool live = idx < total;
+        const int r = (live ? idx : i) - nfwp;
+        const int p = div_rc(r, ng, rng);
+        const int a = r - p * ng;
+        ok[u] = live && !(own0 >= 0 && a >= own0 && a < own0 + ns);
+        int g = 0;
+#pragma unroll
+        for (int h = 1; h < kMaxGuestTypes; ++h) g += (h < Z.ngt && D.gpre[h] <= a) ? 1 : 0;
+        pp[u] = p;
+        gg[u] = g;
+        la[u] = a - D.gpre[g];
+      }
+      double ax[KU], ay[KU], az[KU];
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        ax[u] = gload(C.gx[gg[u]] + la[u]);
+        ay[u] = gload(C.gy[gg[u]] + la[u]);
+        az[u] = gload(C.gz[gg[u]] + la[u]);
+      }
+      PairSlot sl[KU];
+      double chg[KU];
+      int ty[KU], vk[KU];
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int p = pp[u], g = gg[u];
+        const int nsg = Z.ns[g];
+        const int sa =
+            la[u] - div_rc(la[u], nsg, __builtin_amdgcn_rcpf(static_cast<float>(nsg))) * nsg;
+        double dx = D.ppx[p] - ax[u], dy = D.ppy[p] - ay[u], dz = D.ppz[p] - az[u];
+        image<TRI>(S, oc, dx, dy, dz);
+        const double rsq = dx * dx + dy * dy + dz * dz;
+        sl[u] = pair_slot(rsq, ok[u], rcut2, rdrewd, rdlrpot, imax);
+        chg[u] = D.pce[p] * Z.gq[g][sa];
+        ty[u] = Z.gtype[g][sa];
+      }
+      pair_rows(S, D, ldsrow, pp, ty, vk);
+      finish(sl, chg, vk, pp);
+    }
+  }
+  e2 = a2;
+  ev = av;
+}
+
+// The candidate records of a prepared move, pulled into L2 while the move
+// before it is decided, by the waves that build no tables for it (table
+// builders take waves 1..np, one position each): one dword per 128-byte
+// line of each position's cell list.  The words feed a sum that is stored
+// only under a condition no launch meets (nmoves < 0), which keeps the
+// loads; these waves would otherwise wait at the decision's barrier.
+template <int BT>
+__device__ __forceinline__ void prefetch_records(const BSys& Y, const SysQ& Z,
+                                                 const MoveDesc& Dn, int tid, long long nmoves,
+                                                 int* sink) {
+  const int np = Dn.np;
+  const int first = (np + 1) * kWave;   // the first thread of the first idle wave
+  if (tid < first) return;
+  const int lanes = BT - first;
+  const int nl = (Y.gcap * static_cast<int>(sizeof(Rec)) + 127) / 128;
+  const float rnl = 1.0f / nl;
+  const int total = np * nl;
+  const char* base = reinterpret_cast<const char*>(Y.grec);
+  constexpr int kPf = 8;
+  int acc = 0;
+  for (int j0 = tid - first; j0 < total; j0 += kPf * lanes) {
+    int v[kPf];
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) {
+      const int j = j0 + u * lanes;
+      const int jc = j < total ? j : j0;
+      const int p = div_rc(jc, nl, rnl);
+      const int l = jc - p * nl;
+      const int set = Dn.pset[p], s = Dn.psite[p];
+      const double x = set ? Dn.nx[s] : Dn.ox[s];
+      const double y = set ? Dn.ny[s] : Dn.oy[s];
+      const double z = set ? Dn.nz[s] : Dn.oz[s];
+      const size_t off =
+          static_cast<size_t>(cell_of(Z, x, y, z)) * Y.gcap * sizeof(Rec) + l * 128;
+      v[u] = gload(reinterpret_cast<const int*>(base + off));
+    }
+#pragma unroll
+    for (int u = 0; u < kPf; ++u) acc ^= v[u];
+  }
+  if (acc == 0x5eed1e55 && nmoves < 0) *sink = acc;
+}
+
+// ---------------------------------------------------------------------------
 // The persistent kernel
 // ---------------------------------------------------------------------------
 // STREAM: S(k), a_k and the k indices stay in global memory (L2-resident)
@@ -1128,15 +1474,17 @@ __global__ void __launch_bounds__(BT)
   __shared__ int csel, dsel, rsel, spec_hit;   // current buffers, adopted?
   __shared__ float cur_c, spec_c;       // duni state after this / the next proposal
   __shared__ int cur_ir, cur_jr, spec_ir, spec_jr;
-  __shared__ double pre_rande, pre_delrc;   // this move's draw and tail change
-  __shared__ double pre_thr;                // estep above which it is rejected
-  __shared__ int pre_fast;                  // pre_thr applies
+  // the decision's inputs of the move in descriptor i: its draw, tail
+  // change and the estep above which it is rejected (pre_fast: that applies)
+  __shared__ double pre_rande[2], pre_delrc[2], pre_thr[2];
+  __shared__ int pre_fast[2];
   __shared__ double red[4][BT / kWave];
   __shared__ int stop_flag;
 #ifdef FMC_STAGE_CLOCKS
   __shared__ unsigned long long clk[kClkSlots];
 #endif
   __shared__ int shift_lo, shift_hi;
+  __shared__ int pf_sink;   // written by no launch (keeps the record prefetch)
   __shared__ SysQ Z;   // the system's small constants, for the whole launch
   BChain& C = Cbuf[0];   // the launch's first state (the loop rebinds it)
 
@@ -1253,6 +1601,9 @@ __global__ void __launch_bounds__(BT)
 #endif
     // dsel is written only by thread 0's decision, after the reduction barrier
     const int dcur = dsel;
+    // csel and rsel likewise (the decision may flip them while other waves
+    // still read the prepared buffers)
+    const int cnext = csel ^ 1, rnext = rsel ^ 1;
     MoveDesc& D = Dbuf[dcur];
     const int kind = D.kind;
     const int ns = D.ns;
@@ -1270,8 +1621,6 @@ __global__ void __launch_bounds__(BT)
         clk_tables = __builtin_amdgcn_s_memtime();
 #endif
       }
-      const int ntp = Y.S.ntpatm;
-      const bool ldsrow = ntp <= kLdsTypes;
       // ---- reciprocal space (ewald1_guest, ewald.f:141-319) ----
       // dS(k) = sum_new q e^{ik.r} - sum_old q e^{ik.r}
       if (STREAM) {
@@ -1356,111 +1705,21 @@ __global__ void __launch_bounds__(BT)
       //      but the last wave.  It reads only what the set-up phases
       //      wrote, so no barrier separates it from the reciprocal part: a
       //      wave that finishes its k-vectors starts its pairs at once ----
-      // the decision's inputs, by the first lane of the wave before it (it
-      // finishes its real-space share ahead of the preparing wave): the
-      // acceptance draw (read without taking it), the tail change, the
-      // rejection bound
-      if (tid == RT - kWave) {
-        const float* u = rng_u[rsel];
-        float uni = u[cur_ir - 1] - u[cur_jr - 1];
-        if (uni < 0.0f) uni = uni + 1.0f;
-        float c = cur_c - C.cd;
-        if (c < 0.0f) c = c + C.cm;
-        uni = uni - c;
-        if (uni < 0.0f) uni = uni + 1.0f;
-        const double rande = uni;
-        pre_rande = rande;
-        pre_delrc = (kind == FMC_INSERT || kind == FMC_DELETE) ? tail_change(Z, C, D) : 0.0;
-        double thr = 0.0;
-        pre_fast = reject_bound(C, D, rande, &thr) ? 1 : 0;
-        pre_thr = thr;
+      // the decision's inputs of a fresh proposal, by the first lane of the
+      // wave before it (a prepared proposal had them computed while the
+      // move before it was decided)
+      if (fresh && tid == RT - kWave) {
+        const DecisionIn in = decision_inputs(Z, C, D, rng_u[rsel], cur_ir, cur_jr, cur_c);
+        pre_rande[dcur] = in.rande;
+        pre_delrc[dcur] = in.delrc;
+        pre_thr[dcur] = in.thr;
+        pre_fast[dcur] = in.fast;
       }
       if (tid < RT) {
-        const int gcap = Y.gcap;
-        const int nfwp = np * gcap;
-        const int ng = D.nguest;
-        const float rgcap = 1.0f / gcap, rng = 1.0f / max(ng, 1);
-        const int total = nfwp + np * ng;
-        const int own0 = (kind != FMC_INSERT) ? D.gpre[ig] + D.imol * ns : -1;
-        double a2 = 0, av = 0;
-        // kU items per thread per pass: their index loads, then their
-        // coordinate loads, issue together (one memory latency per pass)
-        constexpr int kU = KU;
-        for (int base = tid; base < total; base += kU * RT) {
-          int pp[kU], aa[kU];
-#pragma unroll
-          for (int u = 0; u < kU; ++u) {
-            const int idx = base + u * RT;
-            pp[u] = -1;
-            aa[u] = 0;
-            if (idx < nfwp) {
-              const int p = div_rc(idx, gcap, rgcap);
-              pp[u] = p;
-              aa[u] = D.plo[p] + (idx - p * gcap);   // record slot
-            } else if (idx < total) {
-              const int r = idx - nfwp;
-              const int p = div_rc(r, ng, rng);
-              const int a = r - p * ng;
-              if (!(own0 >= 0 && a >= own0 && a < own0 + ns)) {
-                pp[u] = p;
-                aa[u] = -1 - a;   // guest atom a
-              }
-            }
-          }
-          double ax[kU], ay[kU], az[kU], aq[kU];
-          int at[kU];
-#pragma unroll
-          for (int u = 0; u < kU; ++u) {
-            ax[u] = ay[u] = az[u] = aq[u] = 0.0;
-            at[u] = 0;
-            if (pp[u] < 0) continue;
-            if (aa[u] >= 0) {
-              const double* rr = &Y.grec[aa[u]].x;
-              ax[u] = gload(rr);
-              ay[u] = gload(rr + 1);
-              az[u] = gload(rr + 2);
-              aq[u] = gload(rr + 3);
-              at[u] = gload(Y.grec_t + aa[u]);
-            } else {
-              const int a = -1 - aa[u];
-              int g = 0;
-              while (g + 1 < Z.ngt && D.gpre[g + 1] <= a) ++g;
-              const int la = a - D.gpre[g];
-              const int sa =
-                  la - div_rc(la, Z.ns[g], __builtin_amdgcn_rcpf(static_cast<float>(Z.ns[g]))) *
-                           Z.ns[g];
-              ax[u] = gload(C.gx[g] + la);
-              ay[u] = gload(C.gy[g] + la);
-              az[u] = gload(C.gz[g] + la);
-              aq[u] = Z.gq[g][sa];
-              at[u] = Z.gtype[g][sa];
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < kU; ++u) {
-            if (pp[u] < 0) continue;
-            const int p = pp[u];
-            // FP64 in both precisions: the nearest pairs' terms reach tens
-            // of kcal/mol and cancel against each other and against the
-            // reciprocal part, so an FP32 rounding of each would dominate
-            // the error of the move's energy change, and on this path FP64
-            // pair arithmetic costs about what FP32 does
-            double b2 = 0, bv = 0;
-            const int vk =
-                ldsrow ? D.ptab[p][at[u]] : gload(Y.S.pairtab + D.pts[p] * ntp + at[u]);
-            pair_fp64_vk(Y.S, D.pce[p], vk, D.ppx[p] - ax[u], D.ppy[p] - ay[u],
-                         D.ppz[p] - az[u], aq[u], b2, bv);
-            if (D.psg[p] < 0) {
-              a2 -= b2;
-              av -= bv;
-            } else {
-              a2 += b2;
-              av += bv;
-            }
-          }
-        }
-        e2 = a2;
-        ev = av;
+        if (Y.S.imcon == 3)
+          real_space<KU, true>(Y, Z, C, D, tid, RT, e2, ev);
+        else
+          real_space<KU, false>(Y, Z, C, D, tid, RT, e2, ev);
       }
     }
     // block reductions (fixed tree): DPP within each wave, then the waves'
@@ -1481,9 +1740,22 @@ __global__ void __launch_bounds__(BT)
     // it adopts the prepared proposal, and rebuilds them otherwise
     if (kind != 0 && tid >= kWave) {
       MoveDesc& Dn = Dbuf[dcur ^ 1];
-      if (Dn.kind != 0)
+      if (Dn.kind != 0) {
+        // the prepared move's decision inputs, on the last wave's first
+        // lane: the chain state, duni table and duni state the preparing
+        // wave left (the decision reads those and writes other fields)
+        if (tid == BT - kWave) {
+          const DecisionIn in =
+              decision_inputs(Z, Cbuf[cnext], Dn, rng_u[rnext], spec_ir, spec_jr, spec_c);
+          pre_rande[dcur ^ 1] = in.rande;
+          pre_delrc[dcur ^ 1] = in.delrc;
+          pre_thr[dcur ^ 1] = in.thr;
+          pre_fast[dcur ^ 1] = in.fast;
+        }
         build_tables<T, MIXED>(Z, Y, Dn, tid / kWave - 1, BT / kWave - 1, tid % kWave, tab,
                                tabp, xy_l, zt, nlm);
+        prefetch_records<BT>(Y, Z, Dn, tid, nmoves, &pf_sink);
+      }
     }
     if (tid == 0) {
       double a = 0.0, b = 0.0, c = 0.0;
@@ -1505,22 +1777,23 @@ __global__ void __launch_bounds__(BT)
       if (kind != 0 && !stop_flag && C.steps + 1 < C.stop_at && mv + 1 < nmoves) {
         double estep, e1x;
         bool rej;
-        if (pre_fast) {
+        const double delrc = pre_delrc[dcur], rande = pre_rande[dcur];
+        if (pre_fast[dcur]) {
           // estep and the logged e1 as decide forms them; the bound first
           e1x = a;
           if (kind == FMC_INSERT) {
             e1x = e1x + Z.engsic[ig];
-            estep = (e1x + b + Z.e3[ig] + c + pre_delrc) / C.engunit;
+            estep = (e1x + b + Z.e3[ig] + c + delrc) / C.engunit;
           } else if (kind == FMC_DELETE) {
             e1x = e1x - Z.engsic[ig];
-            estep = (e1x - b - Z.e3[ig] - c + pre_delrc) / C.engunit;
+            estep = (e1x - b - Z.e3[ig] - c + delrc) / C.engunit;
           } else {
             estep = (e1x + b + c) / C.engunit;
           }
-          rej = estep > pre_thr ||
-                !decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x);
+          rej = estep > pre_thr[dcur] ||
+                !decision(Z, C, D, a, b, c, delrc, rande, estep, e1x);
         } else {
-          rej = !decision(Z, C, D, a, b, c, pre_delrc, pre_rande, estep, e1x);
+          rej = !decision(Z, C, D, a, b, c, delrc, rande, estep, e1x);
         }
         if (rej && !(kind == FMC_DISPLACE && estep < 0.0)) {
           spec_hit = 1;

@@ -2,8 +2,8 @@
 
 Runs the 256-chain workload of the benchmark (32 synthetic frameworks x 8
 CO2 pressures, mixed precision, the FP64 recompute every 1000 moves of each
-chain, budgeted launches) at the fixed 512x12 tile (k-group width 4, four
-real-space items per pass) and prints where a move's cycles go: the mean
+chain, budgeted launches) at the fixed 512x12 tile (k-group width 4 and two
+real-space items per pass unless --kgroup/--kunroll say otherwise) and prints where a move's cycles go: the mean
 s_memtime cycles per move of each stage, and per wave the end of its
 parallel-stage work and of its decision-stage work.
 
@@ -30,12 +30,12 @@ PROPOSE, TABLES, PARALLEL, DECISION, COMMIT, MOVES, FRESH, DISPLACE = range(8)
 WAVE_PARALLEL, WAVE_DECISION, WAVE_RECIP, SLOTS = 8, 16, 24, 32
 
 
-def build(recompute_every=1000):
+def build(recompute_every=1000, kgroup=4, kunroll=2):
     from . import synth
     from .batch import Batch
     ws = synth.c3()
     b = Batch(len(ws), capi.PREC_MIXED)
-    b.set_tile_shape(512, 12, 4, 4)
+    b.set_tile_shape(512, 12, kgroup, kunroll)
     ctxs = {}
     for w in ws:
         key = id(w.fw)
@@ -78,13 +78,15 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--budget-ms", type=float, default=15.0)
+    ap.add_argument("--kgroup", type=int, default=4, help="k-group width of the 512x12 tile")
+    ap.add_argument("--kunroll", type=int, default=2, help="real-space items per pass")
     args = ap.parse_args(argv)
     if not args.plain:
         capi.LIB_PATH = os.path.join(os.path.dirname(capi.LIB_PATH), "libfastmc_amd_clocks.so")
         if not os.path.exists(capi.LIB_PATH):
             raise SystemExit(f"{capi.LIB_PATH} missing: make -C fastmc-1.4.0_amd clocks")
     L = capi.lib()
-    b, n = build()
+    b, n = build(kgroup=args.kgroup, kunroll=args.kunroll)
     budget = args.budget_ms * 1e3
     for _ in range(args.warmup):
         b.launch_budget(10 ** 12, budget)
