@@ -586,6 +586,7 @@ int build_system(fmc_batch* b, const fmc_ctx* ctx, uint64_t key,
     Y.q.ns[g] = t.ns;
     for (int i = 0; i < t.ns; ++i) {
       Y.q.gq[g][i] = t.q[i];
+      Y.q.gce[g][i] = t.q[i] / s.epsq * kR4pie0;
       Y.q.gtype[g][i] = t.type[i] - 1;
       Y.q.gmass[g][i] = t.mass[i];
       Y.q.tx[g][i] = t.x[i];
@@ -594,6 +595,11 @@ int build_system(fmc_batch* b, const fmc_ctx* ctx, uint64_t key,
     }
     Y.q.e3[g] = t.e3;
     Y.q.engsic[g] = t.engsic;
+    if (s.ntpatm <= kLdsTypes)
+      for (int i = 0; i < t.ns; ++i)
+        for (int tj = 0; tj < s.ntpatm; ++tj)
+          Y.q.sptab[g][i][tj] =
+              static_cast<short>(s.pairtab[static_cast<size_t>(t.type[i] - 1) * s.ntpatm + tj]);
   }
   // tail as a quadratic form in the molecule counts
   {

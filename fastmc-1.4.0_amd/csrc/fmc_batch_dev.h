@@ -73,6 +73,9 @@ struct SysQ {
   double engsic[kMaxGuestTypes];
   // tail(n) = t0 + sum_g tb[g] n_g + sum_gh tA[g][h] n_g n_h
   double t0, tb[kMaxGuestTypes], tA[kMaxGuestTypes][kMaxGuestTypes];
+  double gce[kMaxGuestTypes][kMaxSites];   // q / epsq * r4pie0 per site
+  // pairtab row of each guest site's type (ntpatm <= kLdsTypes)
+  short sptab[kMaxGuestTypes][kMaxSites][kLdsTypes];
 };
 
 struct alignas(16) Rec {
@@ -349,7 +352,8 @@ __device__ __forceinline__ int cell_of(const SysQ& Z, double x, double y,
 // tile has its own copy, read by fmc_stage_clocks_<threads>_<kpt>.
 constexpr int kClkPropose = 0, kClkTables = 1, kClkParallel = 2, kClkDecision = 3,
               kClkCommit = 4, kClkMoves = 5, kClkFresh = 6, kClkDisplace = 7,
-              kClkWaveParallel = 8, kClkWaveDecision = 16, kClkWaveRecip = 24, kClkSlots = 32;
+              kClkWaveParallel = 8, kClkWaveDecision = 16, kClkWaveRecip = 24,
+              kClkBuild = 32, kClkSlots = 36;
 constexpr int kClkGroups = 4096;
 #ifdef FMC_STAGE_CLOCKS
 static __device__ unsigned long long g_stage_clocks[kClkGroups][kClkSlots];
@@ -991,15 +995,14 @@ __device__ __forceinline__ void recompute_finish(const SysQ& Z, BChain& C, doubl
   C.n_recompute++;
 }
 
-// The per-move tables of a trial's positions, one position per wave (waves
-// w < nw of the caller's range take positions p = w, w + nw, ...): its
-// phase factors e^{i 2 pi l f} for every dimension and l (ewald.f:97-137,
-// one sincospi per lane, into the wave's own rows of tab), then, read back
-// within the wave, its q X[l] Y[m] row and signed Z[n] row for the
-// reciprocal sum, its pair-table row, and its coordinates, charge factors
-// and candidate-list start for real space.  Positions are independent, so
-// no work-group barrier is needed inside; the caller's next barrier
-// publishes the tables.
+// The per-move tables of a trial's positions, built in two phases that a
+// work-group barrier separates, each spread over the threads t < nt of the
+// caller's range with every entry written by exactly one thread:
+//   table_phases: the phase factors e^{i 2 pi l f} of every position, every
+//     dimension and l (ewald.f:97-137), one sincospi per entry, into tab;
+//   table_rows:   from them, each position's q X[l] Y[m] row and signed Z[n]
+//     row for the reciprocal sum, its pair-table row, and its coordinates,
+//     charge factors and candidate-list start for real space.
 //
 // A displacement in DELTA form (mixed precision) sums, per site, the change
 // of q e^{ik.r} rather than the new term minus the old one:
@@ -1009,120 +1012,148 @@ __device__ __forceinline__ void recompute_finish(const SysQ& Z, BChain& C, doubl
 // FP64 phase factors in FP64 and only then rounded to T, so their rounding
 // error is relative to the change of the term, not to the term: the FP32
 // difference of two nearly equal phase factors would lose the digits that
-// the framework's large |S(k)| amplifies in 2 a_k S.dS.  Each position's
-// wave computes the partner position's phase factors it needs into tabp.
-template <typename T, bool DELTA>
-__device__ __forceinline__ void build_tables(const SysQ& Z, const BSys& Y, MoveDesc& D,
-                                             int wave, int nw, int lane,
+// the framework's large |S(k)| amplifies in 2 a_k S.dS.  The partner
+// position's phase factors each row needs go to tabp.
+// Positions follow propose's layout: the nold old sites first (a delete or
+// a displacement), then the new ones, so a position's set and site follow
+// from its index.
+__device__ __forceinline__ int pos_nold(const MoveDesc& D) {
+  return D.kind != FMC_INSERT ? D.ns : 0;
+}
+
+template <bool DELTA>
+__device__ __forceinline__ void table_phases(const SysQ& Z, const MoveDesc& D, int t, int nt,
                                              Cplx<double> (*tab)[3][kBTab],
-                                             Cplx<double> (*tabp)[3][kBTab], Cplx<T>* xy_l,
-                                             Cplx<T>* zt, int nlm) {
-  const int np = D.np, ig = D.ig, kind = D.kind;
+                                             Cplx<double> (*tabp)[3][kBTab]) {
+  const int np = D.np, ns = D.ns, nold = pos_nold(D);
+  const bool delta = DELTA && D.kind == FMC_DISPLACE;
+  const int k0 = Z.kmax[0] + 1, k1 = Z.kmax[1] + 1, k2 = Z.kmax[2] + 1;
+  const int nown = k0 + k1 + k2;
+  const float rnown = 1.0f / nown, rk2 = 1.0f / k2, rk01 = 1.0f / (k0 + k1);
+  // items: every position's own entries (kmax+1 per dimension, packed),
+  // then in delta form the partner entries the rows need: the new Z for
+  // each old position, then the old X and Y for each new one
+  const int nown_all = np * nown;
+  const int total = nown_all + (delta ? ns * nown : 0);
+  for (int i = t; i < total; i += nt) {
+    int p, d, l;
+    bool own = true;
+    if (i < nown_all) {
+      p = div_rc(i, nown, rnown);
+      const int j = i - p * nown;
+      d = j < k0 ? 0 : (j < k0 + k1 ? 1 : 2);
+      l = j - (d == 0 ? 0 : (d == 1 ? k0 : k0 + k1));
+    } else {
+      own = false;
+      int ii = i - nown_all;
+      if (ii < ns * k2) {
+        p = div_rc(ii, k2, rk2);   // old site p
+        d = 2;
+        l = ii - p * k2;
+      } else {
+        ii -= ns * k2;
+        const int sn = div_rc(ii, k0 + k1, rk01);
+        p = ns + sn;                // new site sn
+        const int jj = ii - sn * (k0 + k1);
+        d = jj < k0 ? 0 : 1;
+        l = jj - (d == 1 ? k0 : 0);
+      }
+    }
+    const bool set = p >= nold;
+    const int s = set ? p - nold : p;
+    const bool newpos = own ? set : !set;
+    const double px = newpos ? D.nx[s] : D.ox[s];
+    const double py = newpos ? D.ny[s] : D.oy[s];
+    const double pz = newpos ? D.nz[s] : D.oz[s];
+    double f = Z.rcell[d] * px + Z.rcell[3 + d] * py + Z.rcell[6 + d] * pz;
+    f = f - floor(f);
+    double ph = l * f;
+    ph = ph - rint(ph);
+    double sv, cv;
+    sincospi(2.0 * ph, &sv, &cv);
+    if (!DELTA || own)
+      tab[p][d][l] = {cv, sv};
+    else
+      tabp[p][d][l] = {cv, sv};
+  }
+}
+
+template <typename T, bool DELTA>
+__device__ __forceinline__ void table_rows(const SysQ& Z, const BSys& Y, MoveDesc& D, int t,
+                                           int nt, const Cplx<double> (*tab)[3][kBTab],
+                                           const Cplx<double> (*tabp)[3][kBTab], Cplx<T>* xy_l,
+                                           Cplx<T>* zt, int nlm) {
+  const int np = D.np, ig = D.ig, kind = D.kind, nold = pos_nold(D);
   const int ntp = Y.S.ntpatm;
   const int w2 = 2 * Z.kmax[1] + 1, k3 = Z.kmax[2], nz = 2 * k3 + 1;
-  const float rw2 = 1.0f / w2;
+  const float rw2 = 1.0f / w2, rnlm = 1.0f / nlm, rnz = 1.0f / nz;
   const bool delta = DELTA && kind == FMC_DISPLACE;
-  for (int p = wave; p < np; p += nw) {
-    const int set = D.pset[p], s = D.psite[p];
+  // per position: coordinates, charge factors, site type, sign, cell
+  for (int p = t; p < np; p += nt) {
+    const bool set = p >= nold;
+    const int s = set ? p - nold : p;
     const double x = set ? D.nx[s] : D.ox[s];
     const double y = set ? D.ny[s] : D.oy[s];
     const double z = set ? D.nz[s] : D.oz[s];
     const double q = Z.gq[ig][s];
+    D.plo[p] = cell_of(Z, x, y, z) * Y.gcap;
+    D.ppx[p] = x;
+    D.ppy[p] = y;
+    D.ppz[p] = z;
+    D.pce[p] = Z.gce[ig][s];
+    D.pq[p] = (set || delta) ? q : -q;
+    D.pts[p] = Z.gtype[ig][s];
+    D.psg[p] = (kind == FMC_DISPLACE && !set) ? -1 : 1;
+  }
+  // pair-table rows
+  if (ntp <= kLdsTypes)
+    for (int i = t; i < np * kLdsTypes; i += nt) {
+      const int p = i / kLdsTypes, tj = i - p * kLdsTypes;
+      if (tj < ntp) D.ptab[p][tj] = Z.sptab[ig][p >= nold ? p - nold : p][tj];
+    }
+  // X[l] * Y[m] for every (l, m) (m < 0 conjugates Y); in delta form the
+  // new position's row is X_n Y_n - X_o Y_o
+  for (int i = t; i < np * nlm; i += nt) {
+    const int p = div_rc(i, nlm, rnlm), r = i - p * nlm;
+    const bool set = p >= nold;
+    const double q = Z.gq[ig][set ? p - nold : p];
     const double pq = (set || delta) ? q : -q;
-    const int ts = Z.gtype[ig][s];
-    // one sincospi per lane: first the position's own phase factors (kmax+1
-    // per dimension, packed), then, in delta form, the partner position's
-    // that the rows below need (the new Z for the old position, the old X
-    // and Y for the new one); both fit one pass of the wave while the
-    // entries number at most 64
-    const int k0 = Z.kmax[0] + 1, k1 = Z.kmax[1] + 1, k2 = Z.kmax[2] + 1;
-    const int nown = k0 + k1 + k2;
-    const int d0 = set ? 0 : 2;   // the partner's first dimension
-    const int npart = delta ? (set ? k0 + k1 : k2) : 0;
-    for (int j = lane; j < nown + npart; j += kWave) {
-      const bool own = j < nown;
-      int d, l;
-      if (own) {
-        d = j < k0 ? 0 : (j < k0 + k1 ? 1 : 2);
-        l = j - (d == 0 ? 0 : (d == 1 ? k0 : k0 + k1));
-      } else {
-        const int jj = j - nown;
-        d = d0 + (d0 == 0 && jj >= k0 ? 1 : 0);
-        l = jj - (d == 1 ? k0 : 0);
-      }
-      const double px = own ? x : (set ? D.ox[s] : D.nx[s]);
-      const double py = own ? y : (set ? D.oy[s] : D.ny[s]);
-      const double pz = own ? z : (set ? D.oz[s] : D.nz[s]);
-      double f = Z.rcell[d] * px + Z.rcell[3 + d] * py + Z.rcell[6 + d] * pz;
-      f = f - floor(f);
-      double ph = l * f;
-      ph = ph - rint(ph);
-      double sv, cv;
-      sincospi(2.0 * ph, &sv, &cv);
-      if (!DELTA || own)
-        tab[p][d][l] = {cv, sv};
-      else
-        tabp[p][d][l] = {cv, sv};
+    const int l = div_rc(r, w2, rw2), m = r - l * w2 - Z.kmax[1];
+    const int am = m < 0 ? -m : m;
+    const Cplx<double> a = tab[p][0][l];
+    const Cplx<double> b = tab[p][1][am];
+    double ox, oy;
+    if (m >= 0) {
+      ox = a.x * b.x - a.y * b.y;
+      oy = a.y * b.x + b.y * a.x;
+    } else {
+      ox = a.x * b.x + a.y * b.y;
+      oy = a.y * b.x - b.y * a.x;
     }
-    if (lane == 0) {
-      D.plo[p] = cell_of(Z, x, y, z) * Y.gcap;
-      D.ppx[p] = x;
-      D.ppy[p] = y;
-      D.ppz[p] = z;
-      D.pce[p] = q / Z.epsq * kR4pie0;
-      D.pq[p] = pq;
-      D.pts[p] = ts;
-      D.psg[p] = (kind == FMC_DISPLACE && set == 0) ? -1 : 1;
-    }
-    if (ntp <= kLdsTypes)
-      for (int tj = lane; tj < ntp; tj += kWave)
-        D.ptab[p][tj] = static_cast<short>(Y.S.pairtab[ts * ntp + tj]);
-    // the phase factors of this position, written by this wave, read back
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    // X[l] * Y[m] for every (l, m) (m < 0 conjugates Y); in delta form the
-    // new position's row is X_n Y_n - X_o Y_o
-    Cplx<T>* xr = xy_l + p * nlm;
-    const bool dxy = delta && set;
-    for (int r = lane; r < nlm; r += kWave) {
-      const int l = div_rc(r, w2, rw2), m = r - l * w2 - Z.kmax[1];
-      const int am = m < 0 ? -m : m;
-      const Cplx<double> a = tab[p][0][l];
-      const Cplx<double> b = tab[p][1][am];
-      double ox, oy;
+    if (delta && set) {
+      const Cplx<double> c = tabp[p][0][l];
+      const Cplx<double> e = tabp[p][1][am];
       if (m >= 0) {
-        ox = a.x * b.x - a.y * b.y;
-        oy = a.y * b.x + b.y * a.x;
+        ox -= c.x * e.x - c.y * e.y;
+        oy -= c.y * e.x + e.y * c.x;
       } else {
-        ox = a.x * b.x + a.y * b.y;
-        oy = a.y * b.x - b.y * a.x;
+        ox -= c.x * e.x + c.y * e.y;
+        oy -= c.y * e.x - e.y * c.x;
       }
-      if (dxy) {
-        const Cplx<double> c = tabp[p][0][l];
-        const Cplx<double> e = tabp[p][1][am];
-        if (m >= 0) {
-          ox -= c.x * e.x - c.y * e.y;
-          oy -= c.y * e.x + e.y * c.x;
-        } else {
-          ox -= c.x * e.x + c.y * e.y;
-          oy -= c.y * e.x - e.y * c.x;
-        }
-      }
-      xr[r] = {static_cast<T>(pq * ox), static_cast<T>(pq * oy)};
     }
-    // signed Z[n]; in delta form the old position's row is Z_n - Z_o
-    const bool dz = delta && !set;
-    for (int i = lane; i < nz; i += kWave) {
-      const int n = i - k3;
-      const int an = n < 0 ? -n : n;
-      Cplx<double> c = tab[p][2][an];
-      if (dz) {
-        const Cplx<double> e = tabp[p][2][an];
-        c = {e.x - c.x, e.y - c.y};
-      }
-      zt[p * kZRow + i] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
+    xy_l[p * nlm + r] = {static_cast<T>(pq * ox), static_cast<T>(pq * oy)};
+  }
+  // signed Z[n]; in delta form the old position's row is Z_n - Z_o
+  for (int i = t; i < np * nz; i += nt) {
+    const int p = div_rc(i, nz, rnz), k = i - p * nz;
+    const int n = k - k3;
+    const int an = n < 0 ? -n : n;
+    Cplx<double> c = tab[p][2][an];
+    if (delta && p < nold) {
+      const Cplx<double> e = tabp[p][2][an];
+      c = {e.x - c.x, e.y - c.y};
     }
+    zt[p * kZRow + k] = {static_cast<T>(c.x), static_cast<T>(n < 0 ? -c.y : c.y)};
   }
 }
 
@@ -1394,30 +1425,26 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
 }
 
 // The candidate records of a prepared move, pulled into L2 while the move
-// before it is decided, by the waves that build no tables for it (table
-// builders take waves 1..np, one position each): one dword per 128-byte
-// line of each position's cell list.  The words feed a sum that is stored
-// only under a condition no launch meets (nmoves < 0), which keeps the
-// loads; these waves would otherwise wait at the decision's barrier.
-template <int BT>
+// before it is decided, by threads t < nt of a wave that builds no tables
+// for it: one dword per 128-byte line of each position's cell list.  The
+// words feed a sum that is stored only under a condition no launch meets
+// (nmoves < 0), which keeps the loads; the wave would otherwise wait at
+// the decision's barrier.
 __device__ __forceinline__ void prefetch_records(const BSys& Y, const SysQ& Z,
-                                                 const MoveDesc& Dn, int tid, long long nmoves,
-                                                 int* sink) {
+                                                 const MoveDesc& Dn, int t, int nt,
+                                                 long long nmoves, int* sink) {
   const int np = Dn.np;
-  const int first = (np + 1) * kWave;   // the first thread of the first idle wave
-  if (tid < first) return;
-  const int lanes = BT - first;
   const int nl = (Y.gcap * static_cast<int>(sizeof(Rec)) + 127) / 128;
   const float rnl = 1.0f / nl;
   const int total = np * nl;
   const char* base = reinterpret_cast<const char*>(Y.grec);
   constexpr int kPf = 8;
   int acc = 0;
-  for (int j0 = tid - first; j0 < total; j0 += kPf * lanes) {
+  for (int j0 = t; j0 < total; j0 += kPf * nt) {
     int v[kPf];
 #pragma unroll
     for (int u = 0; u < kPf; ++u) {
-      const int j = j0 + u * lanes;
+      const int j = j0 + u * nt;
       const int jc = j < total ? j : j0;
       const int p = div_rc(jc, nl, rnl);
       const int l = jc - p * nl;
@@ -1614,8 +1641,9 @@ __global__ void __launch_bounds__(BT)
       // the positions' tables: built by every wave for a fresh proposal;
       // a prepared one had them built while the previous move was decided
       if (fresh) {
-        build_tables<T, MIXED>(Z, Y, D, tid / kWave, BT / kWave, tid % kWave, tab, tabp, xy_l,
-                               zt, nlm);
+        table_phases<MIXED>(Z, D, tid, BT, tab, tabp);
+        __syncthreads();
+        table_rows<T, MIXED>(Z, Y, D, tid, BT, tab, tabp, xy_l, zt, nlm);
         __syncthreads();
 #ifdef FMC_STAGE_CLOCKS
         clk_tables = __builtin_amdgcn_s_memtime();
@@ -1735,15 +1763,28 @@ __global__ void __launch_bounds__(BT)
     }
     __syncthreads();
     FMC_CLOCK(clk_reduced);
-    // while thread 0 decides, waves 1.. build the tables of the prepared
-    // next move (this move's are read no more); the next move uses them if
-    // it adopts the prepared proposal, and rebuilds them otherwise
-    if (kind != 0 && tid >= kWave) {
-      MoveDesc& Dn = Dbuf[dcur ^ 1];
-      if (Dn.kind != 0) {
-        // the prepared move's decision inputs, on the last wave's first
-        // lane: the chain state, duni table and duni state the preparing
-        // wave left (the decision reads those and writes other fields)
+    // the tables of the prepared next move (this move's are read no more),
+    // in two phases: the phase factors on waves 1.., a barrier, then the
+    // rows on waves 1 to the one before the last, while thread 0 decides
+    // and the last wave computes the prepared move's decision inputs and
+    // prefetches its candidate records.  The next move uses them if it
+    // adopts the prepared proposal, and rebuilds them otherwise.
+    MoveDesc& Dn = Dbuf[dcur ^ 1];
+    const bool prep = kind != 0 && Dn.kind != 0;
+    if (prep && tid >= kWave)
+      table_phases<MIXED>(Z, Dn, tid - kWave, BT - kWave, tab, tabp);
+    __syncthreads();
+    FMC_CLOCK(clk_phases);
+    if (prep && tid >= kWave) {
+      if (tid < BT - kWave) {
+        table_rows<T, MIXED>(Z, Y, Dn, tid - kWave, BT - 2 * kWave, tab, tabp, xy_l, zt, nlm);
+#ifdef FMC_STAGE_CLOCKS
+        if (tid == kWave) clk[kClkBuild + 1] += __builtin_amdgcn_s_memtime() - clk_phases;
+#endif
+      } else {
+        // the prepared move's decision inputs, on the first lane: the chain
+        // state, duni table and duni state the preparing wave left (the
+        // decision reads those and writes other fields)
         if (tid == BT - kWave) {
           const DecisionIn in =
               decision_inputs(Z, Cbuf[cnext], Dn, rng_u[rnext], spec_ir, spec_jr, spec_c);
@@ -1752,11 +1793,15 @@ __global__ void __launch_bounds__(BT)
           pre_thr[dcur ^ 1] = in.thr;
           pre_fast[dcur ^ 1] = in.fast;
         }
-        build_tables<T, MIXED>(Z, Y, Dn, tid / kWave - 1, BT / kWave - 1, tid % kWave, tab,
-                               tabp, xy_l, zt, nlm);
-        prefetch_records<BT>(Y, Z, Dn, tid, nmoves, &pf_sink);
+        prefetch_records(Y, Z, Dn, tid - (BT - kWave), kWave, nmoves, &pf_sink);
+#ifdef FMC_STAGE_CLOCKS
+        if (tid == BT - kWave) clk[kClkBuild + 2] += __builtin_amdgcn_s_memtime() - clk_phases;
+#endif
       }
     }
+#ifdef FMC_STAGE_CLOCKS
+    if (tid == 0) clk[kClkBuild] += clk_phases - clk_reduced;
+#endif
     if (tid == 0) {
       double a = 0.0, b = 0.0, c = 0.0;
       for (int w = 0; w < BT / kWave; ++w) {
@@ -1822,6 +1867,9 @@ __global__ void __launch_bounds__(BT)
       }
     }
     FMC_CLOCK(clk_decided);
+#ifdef FMC_STAGE_CLOCKS
+    if (tid == 0) clk[kClkBuild + 3] += clk_decided - clk_phases;
+#endif
     __syncthreads();
     FMC_CLOCK(clk_after_decision);
     {

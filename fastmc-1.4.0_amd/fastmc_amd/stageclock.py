@@ -27,7 +27,7 @@ from . import capi
 
 # slot layout of the kernel's stage clocks (csrc/fmc_batch_dev.h, kClk*)
 PROPOSE, TABLES, PARALLEL, DECISION, COMMIT, MOVES, FRESH, DISPLACE = range(8)
-WAVE_PARALLEL, WAVE_DECISION, WAVE_RECIP, SLOTS = 8, 16, 24, 32
+WAVE_PARALLEL, WAVE_DECISION, WAVE_RECIP, BUILD, SLOTS = 8, 16, 24, 32, 36
 
 
 def build(recompute_every=1000, kgroup=4, kunroll=2):
@@ -69,6 +69,13 @@ def table(P):
         out.append(f"| {w} | {P[:, WAVE_PARALLEL + w].sum() / moves:.0f} | "
                    f"{P[:, WAVE_RECIP + w].sum() / moves:.0f} | "
                    f"{P[:, WAVE_DECISION + w].sum() / moves:.0f} |")
+    # the decision stage in its two phases
+    out += ["", "| decision stage, mean cycles per move | |", "|---|---|"]
+    for i, name in enumerate(("phase factors of the prepared move (waves 1-7), to the barrier",
+                              "after it: the rows on wave 1",
+                              "after it: the prepared decision inputs and prefetch (wave 7)",
+                              "after it: thread 0's decision")):
+        out.append(f"| {name} | {P[:, BUILD + i].sum() / moves:.0f} |")
     return out
 
 
