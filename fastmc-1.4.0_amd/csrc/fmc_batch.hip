@@ -350,7 +350,7 @@ struct fmc_batch {
   int precision = FMC_PREC_FP64;
   // tile shape: block size, register tile (0: streaming form), k-group
   // width and real-space items per pass
-  int bt = 0, kpt = 0, kg = 4, ku = 4;
+  int bt = 0, kpt = 0, kg = 4, ku = 2;
   bool tuned = false;    // false: autotune at the next launch
   bool force_stream = false;
   int tune_probes = 0;   // shapes the last autotune timed
@@ -1384,14 +1384,15 @@ extern "C" int fmc_batch_set_tile(fmc_batch* b, int threads) {
   if (threads == 0) {
     std::lock_guard<std::mutex> g(b->mu);
     b->bt = b->kpt = 0;
-    b->kg = b->ku = 4;
+    b->kg = 4;
+    b->ku = 2;
     b->tuned = false;
     b->force_stream = false;
     b->fixed_kpt = false;
     return FMC_OK;
   }
   const bool stream = (threads & FMC_TILE_STREAM) != 0;
-  return fmc_batch_set_tile_shape(b, threads & ~FMC_TILE_STREAM, stream ? 0 : -1, 4, 4);
+  return fmc_batch_set_tile_shape(b, threads & ~FMC_TILE_STREAM, stream ? 0 : -1, 4, 2);
 }
 
 extern "C" int fmc_batch_get_tile(const fmc_batch* b, int* threads) {
@@ -1406,7 +1407,7 @@ extern "C" int fmc_batch_set_tile_shape(fmc_batch* b, int threads, int kpt, int 
   if (!b) return FMC_ERR_ARG;
   if (threads != 256 && threads != 512) return FMC_ERR_UNSUPPORTED;
   if (kgroup != 2 && kgroup != 4 && kgroup != 8) return FMC_ERR_UNSUPPORTED;
-  if (kunroll != 2 && kunroll != 4) return FMC_ERR_UNSUPPORTED;
+  if (kunroll != 2 && kunroll != 1) return FMC_ERR_UNSUPPORTED;
   if (kpt > 0) {
     bool have = false;
     for (const auto& c : all_cfgs())

@@ -22,13 +22,13 @@ namespace fmc {
 #define FMC_ONE(KG_, KU_, MIXED_)                                                      \
   {FMC_TILE_BT, 0, KG_, KU_, MIXED_, batch_kernel<FMC_TILE_BT, 1, MIXED_, true, KG_, KU_>}
 #define FMC_KU(KU_) FMC_ONE(4, KU_, false), FMC_ONE(4, KU_, true)
-extern const KCfg FMC_TILES[] = {FMC_KU(2), FMC_KU(4)};
+extern const KCfg FMC_TILES[] = {FMC_KU(2), FMC_KU(1)};
 #else
 #define FMC_ONE(KG_, KU_, MIXED_)                                                          \
   {FMC_TILE_BT, FMC_TILE_KPT, KG_, KU_, MIXED_,                                            \
    batch_kernel<FMC_TILE_BT, FMC_TILE_KPT, MIXED_, false, KG_, KU_>}
-#define FMC_KG(KG_) FMC_ONE(KG_, 2, false), FMC_ONE(KG_, 2, true), FMC_ONE(KG_, 4, false), \
-                    FMC_ONE(KG_, 4, true)
+#define FMC_KG(KG_) FMC_ONE(KG_, 2, false), FMC_ONE(KG_, 2, true), FMC_ONE(KG_, 1, false), \
+                    FMC_ONE(KG_, 1, true)
 #if FMC_TILE_KPT >= 8
 extern const KCfg FMC_TILES[] = {FMC_KG(2), FMC_KG(4), FMC_KG(8)};
 #else

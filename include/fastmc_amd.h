@@ -329,7 +329,7 @@ int fmc_batch_get_tile(const fmc_batch *b, int *threads);
  * register tile's k-vectors per thread (-1: the smallest that covers the
  * largest Nk while it fits, else the streaming form; 0: the streaming
  * form); kgroup, the k-vectors whose LDS reads issue together (2, 4, 8);
- * kunroll, the real-space items each thread loads per pass (2, 4).  Only
+ * kunroll, the real-space items each thread loads per pass (1, 2).  Only
  * threads changes a chain's arithmetic.  fmc_batch_set_tile(b, 0) lets the
  * autotuner time every compiled shape on real moves before the next launch
  * (the k-vector enumeration it tiles: ewald.f:141-213); get reports the
