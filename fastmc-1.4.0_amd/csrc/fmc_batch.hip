@@ -747,7 +747,7 @@ int launch(fmc_batch* b, long long n, int bt, int kpt, int kg, int ku,
   if ((r = lds_need(b, cfg, &lds))) return r;
   hipLaunchKernelGGL(cfg->fn, dim3(static_cast<unsigned>(b->active_ids.size())),
                      dim3(cfg->bt), lds, b->stream, b->d_sys, b->d_chains, b->d_ids, n,
-                     b->max_nk, budget_ticks);
+                     b->max_nk, b->max_pos, budget_ticks);
   BCHECK(hipGetLastError());
   return FMC_OK;
 }

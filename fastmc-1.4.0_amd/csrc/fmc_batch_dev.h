@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "fastmc_amd.h"
 #include "fmc_device.h"
@@ -375,19 +376,22 @@ __device__ __forceinline__ double fmat(double a, double b, double c) {
 }
 
 // dS(k) of the trial's positions: sum_p (q_p X_p[l] Y_p[m]) Z_p[n], with
-// q X Y tabulated per (l, m) and Z per signed n (conjugates folded in); four
-// multiply-adds per position, accumulated in T.  Both kernel forms call
-// this one routine (contraction off, fused multiply-adds written out), so
-// they round identically.
+// q X Y tabulated per (l, m) row, the positions of a row adjacent
+// (xy[lm * pcap + p]), and Z per position and signed n (conjugates folded
+// in, zt[p * kZRow + n]); four multiply-adds per position, accumulated in
+// T over the positions in order.  Both kernel forms call these routines
+// (contraction off, fused multiply-adds written out), so they round
+// identically.
 constexpr int kZRow = 2 * kBTab;   // Z table row: n = -16..16
+constexpr int kMaxPos = 2 * kMaxSites;
 template <typename T>
-__device__ __forceinline__ void trial_ds(int np, const Cplx<T>* xy, int nlm, int lm,
-                                         const Cplx<T>* zt, int zoff, T& dr, T& di) {
+__device__ __forceinline__ void trial_ds(int np, const Cplx<T>* xy, int xo, const Cplx<T>* zt,
+                                         int zo, T& dr, T& di) {
 #pragma clang fp contract(off)
   T ar = 0, ai = 0;
   for (int p = 0; p < np; ++p) {
-    const Cplx<T> a = xy[p * nlm + lm];
-    const Cplx<T> c = zt[p * kZRow + zoff];
+    const Cplx<T> a = xy[xo + p];
+    const Cplx<T> c = zt[p * kZRow + zo];
     ar = fmat(a.x, c.x, ar);
     ar = fmat(-a.y, c.y, ar);
     ai = fmat(a.y, c.x, ai);
@@ -398,53 +402,96 @@ __device__ __forceinline__ void trial_ds(int np, const Cplx<T>* xy, int nlm, int
 }
 
 // trial_ds for the k-vectors a thread owns, kpt of its KPT slots (kpt the
-// same on every thread), positions outer: the LDS reads of one position for
-// a group of four k issue together and wait once, rather than once per
-// (k, position).  Each k accumulates over the positions in the same order
-// as trial_ds, so the sums are the same.  A slot of a group beyond the
-// thread's k-vectors carries index 0 and its sum is never used.
-template <typename T, int KPT, int G>
-__device__ __forceinline__ void trial_ds_tile(int np, const Cplx<T>* xy, int nlm,
-                                              const Cplx<T>* zt, int zoff, int kpt,
-                                              const int (&kv)[KPT], T (&dr)[KPT],
-                                              T (&di)[KPT]) {
+// same on every thread), xo/zo their row offsets in the two tables.  The
+// positions are unrolled (a uniform exit after the last), so every read
+// takes its position's offset as an immediate; per position the LDS reads
+// of a group of G k-vectors issue together and wait once.  Each k sums
+// its positions in the same order as trial_ds.  A slot of a group beyond
+// the thread's k-vectors reads row 0 and its sum is never used.
+// An LDS table entry by byte address; the address is made opaque to the
+// optimiser (an empty asm that only passes it through a register), so a
+// constant added to it stays the read's immediate offset instead of being
+// folded into a scalar base and re-added per read.
+__device__ __forceinline__ unsigned lds_opaque(unsigned a) {
+  asm volatile("" : "+v"(a));
+  return a;
+}
+template <typename T>
+__device__ __forceinline__ Cplx<T> lds_cplx(unsigned addr) {
+  typedef T V2 __attribute__((ext_vector_type(2)));
+  const V2 v = *reinterpret_cast<const __attribute__((address_space(3))) V2*>(
+      static_cast<uintptr_t>(addr));
+  return {v.x, v.y};
+}
+
+// Position P's terms for the k-vectors of one thread: per group of G
+// k-vectors, every LDS read is issued before the first multiply-add waits
+// on one (the scheduler would otherwise interleave them pairwise under
+// register pressure).  A slot of a group beyond the thread's k-vectors reads
+// row 0 and its sum is never used.
+template <typename T, int KPT, int G, int P>
+__device__ __forceinline__ void trial_ds_position(const unsigned (&xa)[KPT],
+                                                  const unsigned (&za)[KPT], int kpt,
+                                                  T (&dr)[KPT], T (&di)[KPT]) {
 #pragma clang fp contract(off)
   constexpr int NG = (KPT + G - 1) / G;
+#pragma unroll
+  for (int g = 0; g < NG; ++g) {
+    if (g * G < kpt) {
+      Cplx<T> a[G], c[G];
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj) {
+        const int j = g * G + jj;
+        if (j >= KPT) break;
+        a[jj] = lds_cplx<T>(xa[j] + P * sizeof(Cplx<T>));
+        c[jj] = lds_cplx<T>(za[j] + P * kZRow * sizeof(Cplx<T>));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int jj = 0; jj < G; ++jj) {
+        const int j = g * G + jj;
+        if (j >= KPT) break;
+        dr[j] = fmat(a[jj].x, c[jj].x, dr[j]);
+        dr[j] = fmat(-a[jj].y, c[jj].y, dr[j]);
+        di[j] = fmat(a[jj].y, c[jj].x, di[j]);
+        di[j] = fmat(a[jj].x, c[jj].y, di[j]);
+      }
+    }
+  }
+}
+
+template <typename T, int KPT, int G, int... P>
+__device__ __forceinline__ void trial_ds_positions(int np, const unsigned (&xa)[KPT],
+                                                   const unsigned (&za)[KPT], int kpt,
+                                                   T (&dr)[KPT], T (&di)[KPT],
+                                                   std::integer_sequence<int, P...>) {
+  // positions 0, 1, ... in order, stopping after the last (np is uniform)
+  (void)((P < np ? (trial_ds_position<T, KPT, G, P>(xa, za, kpt, dr, di), true) : false) &&
+         ...);
+}
+
+// trial_ds for the k-vectors a thread owns, kpt of its KPT slots (kpt the
+// same on every thread), xo/zo their row offsets in the two tables.  The
+// positions are unrolled, so every read takes its position's offset as an
+// immediate from the k-vector's row address; each k sums its positions in
+// the same order as trial_ds.
+template <typename T, int KPT, int G>
+__device__ __forceinline__ void trial_ds_tile(int np, const Cplx<T>* xy, const Cplx<T>* zt,
+                                              int kpt, const int (&xo)[KPT],
+                                              const int (&zo)[KPT], T (&dr)[KPT],
+                                              T (&di)[KPT]) {
+  unsigned xa[KPT], za[KPT];
+  const unsigned xb = static_cast<unsigned>(reinterpret_cast<uintptr_t>(xy));
+  const unsigned zb = static_cast<unsigned>(reinterpret_cast<uintptr_t>(zt));
 #pragma unroll
   for (int j = 0; j < KPT; ++j) {
     dr[j] = 0;
     di[j] = 0;
+    xa[j] = lds_opaque(xb + static_cast<unsigned>(xo[j]) * sizeof(Cplx<T>));
+    za[j] = lds_opaque(zb + static_cast<unsigned>(zo[j]) * sizeof(Cplx<T>));
   }
-  for (int p = 0; p < np; ++p) {
-    const Cplx<T>* xp = xy + p * nlm;
-    const Cplx<T>* zp = zt + p * kZRow + zoff;
-#pragma unroll
-    for (int g = 0; g < NG; ++g) {
-      if (g * G < kpt) {
-        Cplx<T> a[G], c[G];
-#pragma unroll
-        for (int jj = 0; jj < G; ++jj) {
-          const int j = g * G + jj;
-          if (j >= KPT) break;
-          a[jj] = xp[kv[j] & 0xffff];
-          c[jj] = zp[static_cast<signed char>((kv[j] >> 16) & 0xff)];
-        }
-        // every read of the group is issued before the first multiply-add
-        // waits on one (the scheduler would otherwise interleave them
-        // pairwise under register pressure)
-        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-        for (int jj = 0; jj < G; ++jj) {
-          const int j = g * G + jj;
-          if (j >= KPT) break;
-          dr[j] = fmat(a[jj].x, c[jj].x, dr[j]);
-          dr[j] = fmat(-a[jj].y, c[jj].y, dr[j]);
-          di[j] = fmat(a[jj].y, c[jj].x, di[j]);
-          di[j] = fmat(a[jj].x, c[jj].y, di[j]);
-        }
-      }
-    }
-  }
+  trial_ds_positions<T, KPT, G>(np, xa, za, kpt, dr, di,
+                                std::make_integer_sequence<int, kMaxPos>{});
 }
 
 // a_k (|S + dS|^2 - |S|^2) = a_k (2 S.dS + |dS|^2): no cancellation of |S|^2
@@ -1082,7 +1129,7 @@ template <typename T, bool DELTA>
 __device__ __forceinline__ void table_rows(const SysQ& Z, const BSys& Y, MoveDesc& D, int t,
                                            int nt, const Cplx<double> (*tab)[3][kBTab],
                                            const Cplx<double> (*tabp)[3][kBTab], Cplx<T>* xy_l,
-                                           Cplx<T>* zt, int nlm) {
+                                           Cplx<T>* zt, int nlm, int pcap) {
   const int np = D.np, ig = D.ig, kind = D.kind, nold = pos_nold(D);
   const int ntp = Y.S.ntpatm;
   const int w2 = 2 * Z.kmax[1] + 1, k3 = Z.kmax[2], nz = 2 * k3 + 1;
@@ -1112,7 +1159,9 @@ __device__ __forceinline__ void table_rows(const SysQ& Z, const BSys& Y, MoveDes
       if (tj < ntp) D.ptab[p][tj] = Z.sptab[ig][p >= nold ? p - nold : p][tj];
     }
   // X[l] * Y[m] for every (l, m) (m < 0 conjugates Y); in delta form the
-  // new position's row is X_n Y_n - X_o Y_o
+  // new position's row is X_n Y_n - X_o Y_o.  Item i is row i mod nlm of
+  // position i / nlm: neighbouring threads read the same or neighbouring
+  // phase factors.
   for (int i = t; i < np * nlm; i += nt) {
     const int p = div_rc(i, nlm, rnlm), r = i - p * nlm;
     const bool set = p >= nold;
@@ -1141,7 +1190,7 @@ __device__ __forceinline__ void table_rows(const SysQ& Z, const BSys& Y, MoveDes
         oy -= c.y * e.x - e.y * c.x;
       }
     }
-    xy_l[p * nlm + r] = {static_cast<T>(pq * ox), static_cast<T>(pq * oy)};
+    xy_l[r * pcap + p] = {static_cast<T>(pq * ox), static_cast<T>(pq * oy)};
   }
   // signed Z[n]; in delta form the old position's row is Z_n - Z_o
   for (int i = t; i < np * nz; i += nt) {
@@ -1165,7 +1214,10 @@ __device__ __forceinline__ void table_rows(const SysQ& Z, const BSys& Y, MoveDes
 // Minimum image (min_image's operations, utility_pack.f:1138-1238): a
 // cubic or orthorhombic cell (imcon 1-2) from cell lengths and inverses
 // held in registers, a parallelepiped (imcon 3) through fractional
-// coordinates.
+// coordinates.  Rounding to even (one instruction) replaces Fortran's nint
+// (half away from zero, several): the two differ only where the scaled
+// separation is exactly +-1/2, where both images lie at the same distance
+// to within one rounding.
 struct OrthoCell {
   double a0, a1, a2, i0, i1, i2;
 };
@@ -1184,14 +1236,14 @@ __device__ __forceinline__ void image(const DevSys& S, const OrthoCell& o, doubl
     const double sx = S.rcell[0] * x + S.rcell[3] * y + S.rcell[6] * z;
     const double sy = S.rcell[1] * x + S.rcell[4] * y + S.rcell[7] * z;
     const double sz = S.rcell[2] * x + S.rcell[5] * y + S.rcell[8] * z;
-    const double ax = sx - round(sx), ay = sy - round(sy), az = sz - round(sz);
+    const double ax = sx - rint(sx), ay = sy - rint(sy), az = sz - rint(sz);
     x = S.cell[0] * ax + S.cell[3] * ay + S.cell[6] * az;
     y = S.cell[1] * ax + S.cell[4] * ay + S.cell[7] * az;
     z = S.cell[2] * ax + S.cell[5] * ay + S.cell[8] * az;
   } else {
-    x -= o.a0 * round(o.i0 * x);
-    y -= o.a1 * round(o.i1 * y);
-    z -= o.a2 * round(o.i2 * z);
+    x -= o.a0 * rint(o.i0 * x);
+    y -= o.a1 * rint(o.i1 * y);
+    z -= o.a2 * rint(o.i2 * z);
   }
 }
 
@@ -1211,7 +1263,7 @@ struct PairSlot {
 };
 
 __device__ __forceinline__ PairSlot pair_slot(double rsq, bool ok, double rcut2, double rdrewd,
-                                              double rdlrpot, int imax) {
+                                              double rdlrpot, int imax, bool same_grid) {
 #pragma clang fp contract(off)
   PairSlot s;
   s.in = ok && rcut2 > rsq;
@@ -1221,6 +1273,11 @@ __device__ __forceinline__ PairSlot pair_slot(double rsq, bool ok, double rcut2,
   l = l > imax ? imax : l;
   s.pe = ue - static_cast<double>(l);
   s.le = s.in ? l : 0;
+  if (same_grid) {   // the usual case, rvdw = rcut: one grid for both tables
+    s.pv = s.pe;
+    s.lv = s.le;
+    return s;
+  }
   const double uv = r * rdlrpot;
   l = static_cast<int>(uv);
   l = l > imax ? imax : l;
@@ -1271,6 +1328,7 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
   const DevSys& S = Y.S;
   const OrthoCell oc = ortho_cell(S);
   const double rcut2 = S.rcut2, rdrewd = S.rdrewd, rdlrpot = S.rdlrpot;
+  const bool same_grid = rdrewd == rdlrpot;
   const int imax = S.mxgrid - 2;
   const size_t mx1 = static_cast<size_t>(S.mxgrid) + 1;
   const double* __restrict__ erc = S.erc;
@@ -1359,7 +1417,7 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
       double dx = D.ppx[p] - xy[u].x, dy = D.ppy[p] - xy[u].y, dz = D.ppz[p] - zq[u].x;
       image<TRI>(S, oc, dx, dy, dz);
       const double rsq = dx * dx + dy * dy + dz * dz;
-      sl[u] = pair_slot(rsq, ok[u], rcut2, rdrewd, rdlrpot, imax);
+      sl[u] = pair_slot(rsq, ok[u], rcut2, rdrewd, rdlrpot, imax, same_grid);
       chg[u] = D.pce[p] * zq[u].y;
     }
     finish(sl, chg, vk, pp);
@@ -1412,7 +1470,7 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
         double dx = D.ppx[p] - ax[u], dy = D.ppy[p] - ay[u], dz = D.ppz[p] - az[u];
         image<TRI>(S, oc, dx, dy, dz);
         const double rsq = dx * dx + dy * dy + dz * dz;
-        sl[u] = pair_slot(rsq, ok[u], rcut2, rdrewd, rdlrpot, imax);
+        sl[u] = pair_slot(rsq, ok[u], rcut2, rdrewd, rdlrpot, imax, same_grid);
         chg[u] = D.pce[p] * Z.gq[g][sa];
         ty[u] = Z.gtype[g][sa];
       }
@@ -1481,7 +1539,7 @@ __device__ __forceinline__ void prefetch_records(const BSys& Y, const SysQ& Z,
 template <int BT, int KPT, bool MIXED, bool STREAM, int KG, int KU>
 __global__ void __launch_bounds__(BT)
     batch_kernel(const BSys* __restrict__ systems, BChain* __restrict__ chains,
-                 const int* __restrict__ ids, long long nmoves, int nkcap,
+                 const int* __restrict__ ids, long long nmoves, int nkcap, int pcap,
                  long long budget_ticks) {
   using T = typename std::conditional<MIXED, float, double>::type;
   __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
@@ -1539,8 +1597,8 @@ __global__ void __launch_bounds__(BT)
   extern __shared__ double dyn_lds[];
   double* ak_l = dyn_lds;
   int* kv_l = reinterpret_cast<int*>(dyn_lds + (STREAM ? 0 : nkcap));
-  // X[l]*Y[m] for every (l, m) of the half space, per trial position:
-  // [p][l*(2*kmax2+1) + m + kmax2]
+  // X[l]*Y[m] for every (l, m) of the half space, the trial positions of a
+  // row adjacent: [l*(2*kmax2+1) + m + kmax2][p], pcap positions per row
   const int nlm = (Z.kmax[0] + 1) * (2 * Z.kmax[1] + 1);
   Cplx<T>* xy_l =
       reinterpret_cast<Cplx<T>*>(STREAM ? reinterpret_cast<int*>(dyn_lds)
@@ -1643,7 +1701,7 @@ __global__ void __launch_bounds__(BT)
       if (fresh) {
         table_phases<MIXED>(Z, D, tid, BT, tab, tabp);
         __syncthreads();
-        table_rows<T, MIXED>(Z, Y, D, tid, BT, tab, tabp, xy_l, zt, nlm);
+        table_rows<T, MIXED>(Z, Y, D, tid, BT, tab, tabp, xy_l, zt, nlm, pcap);
         __syncthreads();
 #ifdef FMC_STAGE_CLOCKS
         clk_tables = __builtin_amdgcn_s_memtime();
@@ -1657,7 +1715,7 @@ __global__ void __launch_bounds__(BT)
           const int v = Y.kv[k];
           const int lm = ul(v) * w2 + um(v) + Z.kmax[1];
           T dr32, di32;
-          trial_ds<T>(np, xy_l, nlm, lm, zt, un(v) + Z.kmax[2], dr32, di32);
+          trial_ds<T>(np, xy_l, lm * pcap, zt, un(v) + Z.kmax[2], dr32, di32);
           const double dr = dr32, di = di32;
           const double s0 = gload(C.sre + k), s1 = gload(C.sim + k);
           gstore(C.dsre + k, dr);
@@ -1667,14 +1725,17 @@ __global__ void __launch_bounds__(BT)
       } else {
         // fully unrolled (predicated, no early exit) so that sre/sim/dre/dim
         // keep static register indices
-        int kvj[KPT];
+        // each k's row offsets in the q X Y table and the Z rows
+        int xo[KPT], zo[KPT];
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
           const int k = tid + j * BT;
-          kvj[j] = (j < kpt && k < nk) ? kv_l[k] : 0;
+          const int v = (j < kpt && k < nk) ? kv_l[k] : 0;
+          xo[j] = (v & 0xffff) * pcap;
+          zo[j] = static_cast<signed char>((v >> 16) & 0xff) + Z.kmax[2];
         }
-        trial_ds_tile<T, KPT, KG>(np, xy_l, nlm, zt, Z.kmax[2], __builtin_amdgcn_readfirstlane(kpt),
-                              kvj, dre, dim);
+        trial_ds_tile<T, KPT, KG>(np, xy_l, zt, __builtin_amdgcn_readfirstlane(kpt), xo, zo, dre,
+                                  dim);
 #pragma unroll
         for (int j = 0; j < KPT; ++j) {
           const int k = tid + j * BT;
@@ -1777,7 +1838,8 @@ __global__ void __launch_bounds__(BT)
     FMC_CLOCK(clk_phases);
     if (prep && tid >= kWave) {
       if (tid < BT - kWave) {
-        table_rows<T, MIXED>(Z, Y, Dn, tid - kWave, BT - 2 * kWave, tab, tabp, xy_l, zt, nlm);
+        table_rows<T, MIXED>(Z, Y, Dn, tid - kWave, BT - 2 * kWave, tab, tabp, xy_l, zt, nlm,
+                             pcap);
 #ifdef FMC_STAGE_CLOCKS
         if (tid == kWave) clk[kClkBuild + 1] += __builtin_amdgcn_s_memtime() - clk_phases;
 #endif
@@ -2032,7 +2094,7 @@ __global__ void __launch_bounds__(BT)
 }
 
 // One compiled tile shape of the persistent kernel.
-using BatchKernelFn = void (*)(const BSys*, BChain*, const int*, long long, int, long long);
+using BatchKernelFn = void (*)(const BSys*, BChain*, const int*, long long, int, int, long long);
 struct KCfg {
   int bt, kpt;   // kpt 0: the streaming form
   int kg, ku;
