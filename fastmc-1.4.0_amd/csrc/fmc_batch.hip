@@ -394,6 +394,7 @@ struct fmc_batch {
   int max_nk = 0;
   int max_nlm = 0;   // (kmax1+1)(2 kmax2+1) over the systems
   int max_pos = 0;   // trial positions: 2 x the largest guest
+  int max_grid = 0;  // erfc table entries (mxgrid + 1) over the systems
   std::vector<int> active_ids;
 };
 
@@ -681,12 +682,13 @@ void invalidate_graph(fmc_batch* b) {
 // LDS sizing follows the systems in use
 void refresh_limits(fmc_batch* b) {
   invalidate_graph(b);
-  b->max_nk = b->max_nlm = b->max_pos = 0;
+  b->max_nk = b->max_nlm = b->max_pos = b->max_grid = 0;
   for (const auto& sh : b->systems) {
     if (!sh || sh->refs <= 0) continue;
     b->max_nk = std::max(b->max_nk, sh->dev.nk);
     b->max_nlm = std::max(b->max_nlm, sh->nlm);
     b->max_pos = std::max(b->max_pos, sh->npos);
+    b->max_grid = std::max(b->max_grid, sh->dev.S.mxgrid + 1);
   }
 }
 
@@ -716,13 +718,14 @@ int sync_tables(fmc_batch* b) {
   return FMC_OK;
 }
 
-// dynamic LDS of a form: (a_k, packed k index unless streaming) + X.Y
-// tables; FMC_ERR_UNSUPPORTED when it and the kernel's static LDS exceed
-// the device's per-work-group limit
+// dynamic LDS of a form: the erfc table, (a_k, packed k index unless
+// streaming) + X.Y tables; FMC_ERR_UNSUPPORTED when it and the kernel's
+// static LDS exceed the device's per-work-group limit
 int lds_need(fmc_batch* b, const KCfg* cfg, size_t* out) {
   const size_t elt = b->precision == FMC_PREC_MIXED ? 8 : 16;
   const int nkcap = b->max_nk;
-  size_t lds = static_cast<size_t>(b->max_pos) * b->max_nlm * elt;
+  size_t lds = static_cast<size_t>(b->max_pos) * b->max_nlm * elt +
+               static_cast<size_t>(b->max_grid) * sizeof(double);
   if (cfg->kpt > 0)
     lds += static_cast<size_t>(nkcap) * sizeof(double) +
            static_cast<size_t>((nkcap + 3) & ~3) * sizeof(int);
@@ -747,7 +750,7 @@ int launch(fmc_batch* b, long long n, int bt, int kpt, int kg, int ku,
   if ((r = lds_need(b, cfg, &lds))) return r;
   hipLaunchKernelGGL(cfg->fn, dim3(static_cast<unsigned>(b->active_ids.size())),
                      dim3(cfg->bt), lds, b->stream, b->d_sys, b->d_chains, b->d_ids, n,
-                     b->max_nk, b->max_pos, budget_ticks);
+                     b->max_nk, b->max_pos, b->max_grid, budget_ticks);
   BCHECK(hipGetLastError());
   return FMC_OK;
 }

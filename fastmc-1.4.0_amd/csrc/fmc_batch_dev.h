@@ -1320,8 +1320,8 @@ __device__ __forceinline__ double interp3v(double v0, double v1, double v2, doub
 // those of pair_fp64_vk per item, whatever KU is.
 template <int KU, bool TRI>
 __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const BChain& C,
-                                           const MoveDesc& D, int t, int nt, double& e2,
-                                           double& ev) {
+                                           const MoveDesc& D, const double* erc_l, int t,
+                                           int nt, double& e2, double& ev) {
   // no contraction: an item's terms round the same way in every kernel
   // shape that inlines this (the oracle's build contracts nothing either)
 #pragma clang fp contract(off)
@@ -1331,7 +1331,7 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
   const bool same_grid = rdrewd == rdlrpot;
   const int imax = S.mxgrid - 2;
   const size_t mx1 = static_cast<size_t>(S.mxgrid) + 1;
-  const double* __restrict__ erc = S.erc;
+  const double* __restrict__ erc = erc_l;   // the erfc table, copied to LDS
   const double* __restrict__ vvv = S.vvv;
   const int ntp = S.ntpatm;
   const bool ldsrow = ntp <= kLdsTypes;
@@ -1354,9 +1354,9 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
         const double* te = erc + sl[u].le;
         const double* tv =
             vvv + static_cast<size_t>(vk[u] >= 0 && sl[u].in ? vk[u] : 0) * mx1 + sl[u].lv;
-        e0[u] = gload(te);
-        e1[u] = gload(te + 1);
-        e2v[u] = gload(te + 2);
+        e0[u] = te[0];
+        e1[u] = te[1];
+        e2v[u] = te[2];
         w0[u] = gload(tv);
         w1[u] = gload(tv + 1);
         w2[u] = gload(tv + 2);
@@ -1539,7 +1539,7 @@ __device__ __forceinline__ void prefetch_records(const BSys& Y, const SysQ& Z,
 template <int BT, int KPT, bool MIXED, bool STREAM, int KG, int KU>
 __global__ void __launch_bounds__(BT)
     batch_kernel(const BSys* __restrict__ systems, BChain* __restrict__ chains,
-                 const int* __restrict__ ids, long long nmoves, int nkcap, int pcap,
+                 const int* __restrict__ ids, long long nmoves, int nkcap, int pcap, int ercap,
                  long long budget_ticks) {
   using T = typename std::conditional<MIXED, float, double>::type;
   __shared__ Cplx<double> tab[2 * kMaxSites][3][kBTab];   // per position p
@@ -1595,17 +1595,20 @@ __global__ void __launch_bounds__(BT)
   // dynamic LDS: a_k [nkcap], packed k index [nkcap] (register form), then
   // the XY tables
   extern __shared__ double dyn_lds[];
-  double* ak_l = dyn_lds;
-  int* kv_l = reinterpret_cast<int*>(dyn_lds + (STREAM ? 0 : nkcap));
+  // the system's erfc table (ercap entries reserved), read by real space
+  double* erc_l = dyn_lds;
+  double* ak_l = dyn_lds + ercap;
+  int* kv_l = reinterpret_cast<int*>(ak_l + (STREAM ? 0 : nkcap));
   // X[l]*Y[m] for every (l, m) of the half space, the trial positions of a
   // row adjacent: [l*(2*kmax2+1) + m + kmax2][p], pcap positions per row
   const int nlm = (Z.kmax[0] + 1) * (2 * Z.kmax[1] + 1);
   Cplx<T>* xy_l =
-      reinterpret_cast<Cplx<T>*>(STREAM ? reinterpret_cast<int*>(dyn_lds)
+      reinterpret_cast<Cplx<T>*>(STREAM ? reinterpret_cast<int*>(ak_l)
                                         : kv_l + ((nkcap + 3) & ~3));
   // this thread owns k = tid + j*BT, j < kpt = ceil(nk/BT); the index
   // (l, m) is precomputed into kv's upper bits
   const int kpt = (nk + BT - 1) / BT;
+  for (int i = tid; i <= Y.S.mxgrid; i += BT) erc_l[i] = Y.S.erc[i];
   if (!STREAM) {
     for (int k = tid; k < nk; k += BT) {
       ak_l[k] = Y.ak[k];
@@ -1806,9 +1809,9 @@ __global__ void __launch_bounds__(BT)
       }
       if (tid < RT) {
         if (Y.S.imcon == 3)
-          real_space<KU, true>(Y, Z, C, D, tid, RT, e2, ev);
+          real_space<KU, true>(Y, Z, C, D, erc_l, tid, RT, e2, ev);
         else
-          real_space<KU, false>(Y, Z, C, D, tid, RT, e2, ev);
+          real_space<KU, false>(Y, Z, C, D, erc_l, tid, RT, e2, ev);
       }
     }
     // block reductions (fixed tree): DPP within each wave, then the waves'
@@ -2094,7 +2097,8 @@ __global__ void __launch_bounds__(BT)
 }
 
 // One compiled tile shape of the persistent kernel.
-using BatchKernelFn = void (*)(const BSys*, BChain*, const int*, long long, int, int, long long);
+using BatchKernelFn = void (*)(const BSys*, BChain*, const int*, long long, int, int, int,
+                               long long);
 struct KCfg {
   int bt, kpt;   // kpt 0: the streaming form
   int kg, ku;
