@@ -1546,6 +1546,7 @@ extern "C" int fmc_batch_stats(fmc_batch* b, int id, fmc_chain_stats* st) {
   if (!b->chains[id]) return FMC_ERR_ARG;
   BChain C;
   BCHECK(hipMemcpy(&C, b->d_chains + id, sizeof(BChain), hipMemcpyDeviceToHost));
+  fold_samples(C);   // the production samples still pending, as the kernel adds them
   std::memset(st, 0, sizeof(*st));
   st->steps = C.steps;
   st->prod_steps = C.prod_steps;

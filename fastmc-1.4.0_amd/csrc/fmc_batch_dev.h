@@ -137,6 +137,7 @@ struct BChain {
   int tran_count[kMaxGuestTypes], accept_tran[kMaxGuestTypes];
   int rota_count, accept_rota;
   long long steps, prod_steps;
+  long long pend;   // production samples of the current state not yet in the sums
   long long accepted[6], attempted[6];
   double sum_n[kMaxGuestTypes], sum_n2[kMaxGuestTypes], sum_e[kMaxGuestTypes];
   double sum_en[kMaxGuestTypes], sum_e2[kMaxGuestTypes];
@@ -354,7 +355,7 @@ __device__ __forceinline__ int cell_of(const SysQ& Z, double x, double y,
 constexpr int kClkPropose = 0, kClkTables = 1, kClkParallel = 2, kClkDecision = 3,
               kClkCommit = 4, kClkMoves = 5, kClkFresh = 6, kClkDisplace = 7,
               kClkWaveParallel = 8, kClkWaveDecision = 16, kClkWaveRecip = 24,
-              kClkBuild = 32, kClkSlots = 36;
+              kClkBuild = 32, kClkPrep = 36, kClkSlots = 40;
 constexpr int kClkGroups = 4096;
 #ifdef FMC_STAGE_CLOCKS
 static __device__ unsigned long long g_stage_clocks[kClkGroups][kClkSlots];
@@ -730,15 +731,55 @@ __host__ __device__ inline long long win_target(int k, int L) {
                           : static_cast<long long>(kWin) * L + 1;
 }
 
+// The production sums take every production move's sample (N_g, E_g)
+// (gcmc.f:1814-1848).  The sample changes only when a move is accepted, so
+// a move counts its sample in pend and the repeats are added in one step,
+// r N, r N^2, r E, r E N, r E^2, just before an accepted move changes the
+// state.  A reader of the sums in between (a statistics window, the host)
+// takes the sums with the pending repeats added the same way without
+// storing them, so the sums' bits follow from the trajectory alone, not
+// from where launches split it or which windows are kept.  The N sums are
+// integers and stay exact; an E sum differs from the sample-by-sample sum
+// by rounding only.
+struct SampleSums {
+  double n, n2, e, en, e2;
+};
+
+__host__ __device__ inline SampleSums sums_with_pending(const BChain& C, int g) {
+#pragma clang fp contract(off)
+  const double r = static_cast<double>(C.pend);
+  const double n = C.nmol[g];
+  const double e = C.energy[g];
+  if (C.pend == 0) return {C.sum_n[g], C.sum_n2[g], C.sum_e[g], C.sum_en[g], C.sum_e2[g]};
+  return {C.sum_n[g] + r * n, C.sum_n2[g] + r * (n * n), C.sum_e[g] + r * e,
+          C.sum_en[g] + r * (e * n), C.sum_e2[g] + r * (e * e)};
+}
+
+__host__ __device__ inline void fold_samples(BChain& C) {
+  if (C.pend == 0) return;
+  for (int g = 0; g < C.ntpguest; ++g) {
+    const SampleSums t = sums_with_pending(C, g);
+    C.sum_n[g] = t.n;
+    C.sum_n2[g] = t.n2;
+    C.sum_e[g] = t.e;
+    C.sum_en[g] = t.en;
+    C.sum_e2[g] = t.e2;
+  }
+  C.pend = 0;
+}
+
 static __device__ void window_snapshot(BChain& C, double* snaps) {
   while (C.win_snap < kWinSnap && C.prod_steps == win_target(C.win_snap, C.win_len)) {
     double* w = snaps + C.win_snap * kWinQ * kMaxGuestTypes;
     for (int g = 0; g < kMaxGuestTypes; ++g) {
-      gstore(w + 0 * kMaxGuestTypes + g, C.sum_n[g]);
-      gstore(w + 1 * kMaxGuestTypes + g, C.sum_e[g]);
-      gstore(w + 2 * kMaxGuestTypes + g, C.sum_en[g]);
-      gstore(w + 3 * kMaxGuestTypes + g, C.sum_n2[g]);
-      gstore(w + 4 * kMaxGuestTypes + g, C.sum_e2[g]);
+      const SampleSums t = g < C.ntpguest ? sums_with_pending(C, g)
+                                          : SampleSums{C.sum_n[g], C.sum_n2[g], C.sum_e[g],
+                                                       C.sum_en[g], C.sum_e2[g]};
+      gstore(w + 0 * kMaxGuestTypes + g, t.n);
+      gstore(w + 1 * kMaxGuestTypes + g, t.e);
+      gstore(w + 2 * kMaxGuestTypes + g, t.en);
+      gstore(w + 3 * kMaxGuestTypes + g, t.n2);
+      gstore(w + 4 * kMaxGuestTypes + g, t.e2);
     }
     C.win_snap++;
   }
@@ -753,12 +794,15 @@ __device__ __forceinline__ void selectivity_window(BChain& C) {
   const double av = static_cast<double>(C.sel_count);
   const double nw = static_cast<double>(C.sel_nwind);
   const double weight = 1.0;
+  // the N sums with the pending samples (exact integers)
+  double sn[kMaxGuestTypes];
+  for (int g = 0; g < C.ntpguest; ++g) sn[g] = sums_with_pending(C, g).n;
   for (int i = 1; i < C.ntpguest; ++i) {
-    const double aN = (C.sum_n[i] - C.sel_base[i]) / nw;
+    const double aN = (sn[i] - C.sel_base[i]) / nw;
     if (aN == 0.0) continue;
     for (int j = 0; j < i; ++j) {
       const int p = i * (i - 1) / 2 + j;
-      const double sij = ((C.sum_n[j] - C.sel_base[j]) / nw) / aN;
+      const double sij = ((sn[j] - C.sel_base[j]) / nw) / aN;
       const double delta = weight * (sij - C.sel_mean[p]);
       C.sel_mean[p] = C.sel_mean[p] + delta / av;
       C.sel_sec[p] = C.sel_sec[p] + weight * (sij * sij - C.sel_sec[p]) / (av - 1.0 + weight);
@@ -766,23 +810,16 @@ __device__ __forceinline__ void selectivity_window(BChain& C) {
       C.sel_var[p] = (w * C.sel_sec[p] - (w * C.sel_mean[p] * C.sel_mean[p])) / w;
     }
   }
-  for (int g = 0; g < C.ntpguest; ++g) C.sel_base[g] = C.sum_n[g];
+  for (int g = 0; g < C.ntpguest; ++g) C.sel_base[g] = sn[g];
 }
 
 // Production statistics of one move (gcmc.f:1814-1848), after C.steps has
-// counted it; true when it is a production sample.
+// counted it; true when it is a production sample.  The sample itself is
+// counted in pend (fold_samples).
 __device__ __forceinline__ bool production_sample(BChain& C) {
   if (C.steps <= C.eq_steps) return false;
   C.prod_steps++;
-  for (int g = 0; g < C.ntpguest; ++g) {
-    const double n = C.nmol[g];
-    const double e = C.energy[g];
-    C.sum_n[g] += n;
-    C.sum_n2[g] += n * n;
-    C.sum_e[g] += e;
-    C.sum_en[g] += e * n;
-    C.sum_e2[g] += e * e;
-  }
+  C.pend++;
   if (C.ntpguest > 1 && C.prod_steps % C.sel_nwind == 0) selectivity_window(C);
   return true;
 }
@@ -923,6 +960,7 @@ static __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
     acc = accept_test(estep, rande, C.statvolm, C.press[ig], nmols, C.temp,
                       C.beta, D.kind);
     if (acc) {
+      fold_samples(C);   // the samples so far are of the state before the move
       C.elrc = C.elrc + delrc;
       C.nmol[ig] = nmols + (D.kind == FMC_INSERT ? 1 : -1);
     }
@@ -936,6 +974,7 @@ static __device__ void decide(const SysQ& Z, BChain& C, Rng& R, MoveDesc& D,
                         C.beta, FMC_DISPLACE);
     }
     if (acc) {
+      fold_samples(C);
       if (D.sub == 4) C.accept_tran[ig]++;
       else if (D.sub == 5) C.accept_rota++;
       else C.accept_disp[ig]++;
@@ -1758,14 +1797,35 @@ __global__ void __launch_bounds__(BT)
       constexpr int RT = BT - kWave;   // threads on real space
       if (tid >= RT) {
         const int lane = tid - RT;
+#ifdef FMC_STAGE_CLOCKS
+        const unsigned long long t_p0 = __builtin_amdgcn_s_memtime();
+#endif
         BChain& Cn = Cbuf[csel ^ 1];
         const long long* src = reinterpret_cast<const long long*>(&C);
         long long* dst = reinterpret_cast<long long*>(&Cn);
         static_assert(sizeof(BChain) % 8 == 0, "BChain copies in 8-byte words");
-        for (int i = lane; i < static_cast<int>(sizeof(BChain) / 8); i += kWave) dst[i] = src[i];
+        {
+          // every read of the copy issued before its writes
+          constexpr int kW = static_cast<int>(sizeof(BChain) / 8);
+          constexpr int kPer = (kW + kWave - 1) / kWave;
+          long long w[kPer];
+#pragma unroll
+          for (int u = 0; u < kPer; ++u) {
+            const int i = lane + u * kWave;
+            w[u] = i < kW ? src[i] : 0;
+          }
+#pragma unroll
+          for (int u = 0; u < kPer; ++u) {
+            const int i = lane + u * kWave;
+            if (i < kW) dst[i] = w[u];
+          }
+        }
         for (int i = lane; i < 97; i += kWave) rng_u[rsel ^ 1][i] = rng_u[rsel][i];
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
+#ifdef FMC_STAGE_CLOCKS
+        if (lane == 0) clk[kClkPrep] += __builtin_amdgcn_s_memtime() - t_p0;
+#endif
         {
           // every lane holds the same duni state
           Rng Rs;
@@ -1784,7 +1844,13 @@ __global__ void __launch_bounds__(BT)
             Cn.steps++;
             (void)production_sample(Cn);
           }
+#ifdef FMC_STAGE_CLOCKS
+          if (lane == 0) clk[kClkPrep + 1] += __builtin_amdgcn_s_memtime() - t_p0;
+#endif
           propose(Z, Cn, Rs, Dbuf[dsel ^ 1], lane);
+#ifdef FMC_STAGE_CLOCKS
+          if (lane == 0) clk[kClkPrep + 2] += __builtin_amdgcn_s_memtime() - t_p0;
+#endif
           if (lane == 0) {
             spec_c = Rs.c;
             spec_ir = Rs.ir;
@@ -1798,8 +1864,8 @@ __global__ void __launch_bounds__(BT)
       //      wrote, so no barrier separates it from the reciprocal part: a
       //      wave that finishes its k-vectors starts its pairs at once ----
       // the decision's inputs of a fresh proposal, by the first lane of the
-      // wave before it (a prepared proposal had them computed while the
-      // move before it was decided)
+      // wave before the last (a prepared proposal had them computed while
+      // the move before it was decided)
       if (fresh && tid == RT - kWave) {
         const DecisionIn in = decision_inputs(Z, C, D, rng_u[rsel], cur_ir, cur_jr, cur_c);
         pre_rande[dcur] = in.rande;

@@ -27,7 +27,7 @@ from . import capi
 
 # slot layout of the kernel's stage clocks (csrc/fmc_batch_dev.h, kClk*)
 PROPOSE, TABLES, PARALLEL, DECISION, COMMIT, MOVES, FRESH, DISPLACE = range(8)
-WAVE_PARALLEL, WAVE_DECISION, WAVE_RECIP, BUILD, SLOTS = 8, 16, 24, 32, 36
+WAVE_PARALLEL, WAVE_DECISION, WAVE_RECIP, BUILD, PREP, SLOTS = 8, 16, 24, 32, 36, 40
 
 
 def build(recompute_every=1000, kgroup=4, kunroll=2):
@@ -76,6 +76,13 @@ def table(P):
                               "after it: the prepared decision inputs and prefetch (wave 7)",
                               "after it: thread 0's decision")):
         out.append(f"| {name} | {P[:, BUILD + i].sum() / moves:.0f} |")
+    # the prepared rejection on the last wave, cycles from its start
+    prepared = max(P[:, PREP + 2].astype(bool).sum() and moves, 1.0)
+    out += ["", "| the last wave's prepared rejection, mean cycles per move from its start to the end of | |",
+            "|---|---|"]
+    for i, name in enumerate(("the copy of the chain state and duni table",
+                              "the booking of the move as rejected", "the next proposal")):
+        out.append(f"| {name} | {P[:, PREP + i].sum() / prepared:.0f} |")
     return out
 
 
