@@ -1174,8 +1174,10 @@ __device__ __forceinline__ void table_rows(const SysQ& Z, const BSys& Y, MoveDes
   const int w2 = 2 * Z.kmax[1] + 1, k3 = Z.kmax[2], nz = 2 * k3 + 1;
   const float rw2 = 1.0f / w2, rnlm = 1.0f / nlm, rnz = 1.0f / nz;
   const bool delta = DELTA && kind == FMC_DISPLACE;
-  // per position: coordinates, charge factors, site type, sign, cell
-  for (int p = t; p < np; p += nt) {
+  // per position: coordinates, charge factors, site type, sign, cell, on
+  // the range's last threads (the items below go round-robin, so those
+  // have the fewest)
+  for (int p = nt - 1 - t; p >= 0 && p < np; p += nt) {
     const bool set = p >= nold;
     const int s = set ? p - nold : p;
     const double x = set ? D.nx[s] : D.ox[s];
