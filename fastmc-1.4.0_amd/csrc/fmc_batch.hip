@@ -725,10 +725,10 @@ int lds_need(fmc_batch* b, const KCfg* cfg, size_t* out) {
   const size_t elt = b->precision == FMC_PREC_MIXED ? 8 : 16;
   const int nkcap = b->max_nk;
   size_t lds = static_cast<size_t>(b->max_pos) * b->max_nlm * elt +
-               static_cast<size_t>(b->max_grid) * sizeof(double);
+               static_cast<size_t>(dyn_erc_slots(b->max_grid)) * sizeof(double);
   if (cfg->kpt > 0)
-    lds += static_cast<size_t>(nkcap) * sizeof(double) +
-           static_cast<size_t>((nkcap + 3) & ~3) * sizeof(int);
+    lds += static_cast<size_t>(dyn_ak_slots(nkcap)) * sizeof(double) +
+           static_cast<size_t>(dyn_kv_slots(nkcap)) * sizeof(int);
   int dev = 0, lmax = 0;
   BCHECK(hipGetDevice(&dev));
   BCHECK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
@@ -831,10 +831,11 @@ int snapshot(fmc_batch* b, Snapshot* s, bool restore) {
 // tile that covers the largest Nk or the streaming form, every k-group
 // width (the k-vectors whose LDS reads issue together) and real-space items
 // per pass.  Each shape first runs a short launch (loading its code), then
-// a timed one, GPU time from HIP events on the batch stream; the three
-// fastest are timed again with twice the moves and the fastest of those is
-// kept.  The chains are restored to their state before tuning, so the
-// tuned run starts where it would have.  Only the block size changes a
+// a timed one of 64 moves, GPU time from HIP events on the batch stream;
+// the three fastest are timed again over 256 moves, long enough that a
+// launch's start-up (LDS tables, chain state) does not decide it, and the
+// fastest of those is kept.  The chains are restored to their state before
+// tuning, so the tuned run starts where it would have.  Only the block size changes a
 // chain's arithmetic (the reductions over threads); the chains record the
 // shape their launches used (fmc_chain_stats, checkpoints).
 int autotune(fmc_batch* b) {
@@ -868,11 +869,11 @@ int autotune(fmc_batch* b) {
     return FMC_OK;
   };
   for (auto& sh : cand)
-    if ((r = time_shape(sh, 32))) return r;
+    if ((r = time_shape(sh, 64))) return r;
   std::sort(cand.begin(), cand.end(), [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
   const size_t top = std::min<size_t>(3, cand.size());
   for (size_t i = 0; i < top; ++i)
-    if ((r = time_shape(cand[i], 64))) return r;
+    if ((r = time_shape(cand[i], 256))) return r;
   std::sort(cand.begin(), cand.begin() + top,
             [](const Shape& x, const Shape& y) { return x.ms < y.ms; });
   if ((r = snapshot(b, &snap, true))) return r;

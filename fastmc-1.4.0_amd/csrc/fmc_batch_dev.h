@@ -166,6 +166,15 @@ __host__ __device__ constexpr int pack_tile(int bt, int kpt, int kg, int ku) {
   return (bt / 64) | (kpt << 6) | (kg << 12) | (ku << 16);
 }
 
+// Dynamic LDS of the batch kernel, from a 16-byte aligned base: the erfc
+// table, a_k and the packed k index (register form only), then the X.Y
+// tables. Every region is padded to 16 bytes so the FP64 X.Y entries
+// (complex doubles, read as 128-bit loads) stay aligned whatever the table
+// sizes; an 8-byte aligned 128-bit LDS read is split by the hardware.
+__host__ __device__ constexpr int dyn_erc_slots(int ercap) { return (ercap + 1) & ~1; }
+__host__ __device__ constexpr int dyn_ak_slots(int nkcap) { return (nkcap + 1) & ~1; }
+__host__ __device__ constexpr int dyn_kv_slots(int nkcap) { return (nkcap + 3) & ~3; }
+
 // ---- RNG on one lane (gcmc.f:2786-2799) ----
 struct Rng {
   float* u;
@@ -1633,19 +1642,19 @@ __global__ void __launch_bounds__(BT)
   __syncthreads();
   const int nk = Y.nk;
   if (tid == 0) C.tile_code = pack_tile(BT, STREAM ? 0 : KPT, KG, KU);
-  // dynamic LDS: a_k [nkcap], packed k index [nkcap] (register form), then
-  // the XY tables
-  extern __shared__ double dyn_lds[];
+  // dynamic LDS (dyn_erc_slots): the erfc table, a_k [nkcap] and the
+  // packed k index [nkcap] (register form), then the XY tables
+  extern __shared__ __attribute__((aligned(16))) double dyn_lds[];
   // the system's erfc table (ercap entries reserved), read by real space
   double* erc_l = dyn_lds;
-  double* ak_l = dyn_lds + ercap;
-  int* kv_l = reinterpret_cast<int*>(ak_l + (STREAM ? 0 : nkcap));
+  double* ak_l = dyn_lds + dyn_erc_slots(ercap);
+  int* kv_l = reinterpret_cast<int*>(ak_l + (STREAM ? 0 : dyn_ak_slots(nkcap)));
   // X[l]*Y[m] for every (l, m) of the half space, the trial positions of a
   // row adjacent: [l*(2*kmax2+1) + m + kmax2][p], pcap positions per row
   const int nlm = (Z.kmax[0] + 1) * (2 * Z.kmax[1] + 1);
   Cplx<T>* xy_l =
       reinterpret_cast<Cplx<T>*>(STREAM ? reinterpret_cast<int*>(ak_l)
-                                        : kv_l + ((nkcap + 3) & ~3));
+                                        : kv_l + dyn_kv_slots(nkcap));
   // this thread owns k = tid + j*BT, j < kpt = ceil(nk/BT); the index
   // (l, m) is precomputed into kv's upper bits
   const int kpt = (nk + BT - 1) / BT;
