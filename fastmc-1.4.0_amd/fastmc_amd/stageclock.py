@@ -2,9 +2,10 @@
 
 Runs the 256-chain workload of the benchmark (32 synthetic frameworks x 8
 CO2 pressures, mixed precision, the FP64 recompute every 1000 moves of each
-chain, budgeted launches) at the fixed 512x12 tile (k-group width 4 and two
-real-space items per pass unless --kgroup/--kunroll say otherwise) and prints where a move's cycles go: the mean
-s_memtime cycles per move of each stage, and per wave the end of its
+chain, budgeted launches) at the fixed 512x12 tile (k-group width 8 and
+one real-space item per pass unless --kgroup/--kunroll say otherwise) and
+prints where a move's cycles go: the mean s_memtime cycles per move of each
+stage, and per wave the end of its
 parallel-stage work and of its decision-stage work.
 
 The clocks come from the profiling build of the library, in which the
@@ -30,7 +31,7 @@ PROPOSE, TABLES, PARALLEL, DECISION, COMMIT, MOVES, FRESH, DISPLACE = range(8)
 WAVE_PARALLEL, WAVE_DECISION, WAVE_RECIP, BUILD, PREP, SLOTS = 8, 16, 24, 32, 36, 40
 
 
-def build(recompute_every=1000, kgroup=4, kunroll=2):
+def build(recompute_every=1000, kgroup=8, kunroll=1):
     from . import synth
     from .batch import Batch
     ws = synth.c3()
@@ -92,8 +93,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--budget-ms", type=float, default=15.0)
-    ap.add_argument("--kgroup", type=int, default=4, help="k-group width of the 512x12 tile")
-    ap.add_argument("--kunroll", type=int, default=2, help="real-space items per pass")
+    ap.add_argument("--kgroup", type=int, default=8, help="k-group width of the 512x12 tile")
+    ap.add_argument("--kunroll", type=int, default=1, help="real-space items per pass")
     args = ap.parse_args(argv)
     if not args.plain:
         capi.LIB_PATH = os.path.join(os.path.dirname(capi.LIB_PATH), "libfastmc_amd_clocks.so")
