@@ -1395,29 +1395,29 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
   // the terms of KU items whose separations are known
   auto finish = [&](const PairSlot (&sl)[KU], const double (&chg)[KU], const int (&vk)[KU],
                     const int (&pp)[KU]) {
+    // every item reads its table entries, a masked one entry 0 of the
+    // first table (valid, the same address on every masked lane), so no
+    // value needs a default
     double e0[KU], e1[KU], e2v[KU], w0[KU], w1[KU], w2[KU];
 #pragma unroll
     for (int u = 0; u < KU; ++u) {
-      e0[u] = e1[u] = e2v[u] = w0[u] = w1[u] = w2[u] = 0.0;
-      // tables only for items some lane of the wave counts
-      if (__builtin_amdgcn_ballot_w64(sl[u].in) != 0) {
-        const double* te = erc + sl[u].le;
-        const double* tv =
-            vvv + static_cast<size_t>(vk[u] >= 0 && sl[u].in ? vk[u] : 0) * mx1 + sl[u].lv;
-        e0[u] = te[0];
-        e1[u] = te[1];
-        e2v[u] = te[2];
-        w0[u] = gload(tv);
-        w1[u] = gload(tv + 1);
-        w2[u] = gload(tv + 2);
-      }
+      const double* te = erc + sl[u].le;
+      const double* tv =
+          vvv + static_cast<size_t>(vk[u] >= 0 && sl[u].in ? vk[u] : 0) * mx1 + sl[u].lv;
+      e0[u] = te[0];
+      e1[u] = te[1];
+      e2v[u] = te[2];
+      w0[u] = gload(tv);
+      w1[u] = gload(tv + 1);
+      w2[u] = gload(tv + 2);
     }
 #pragma unroll
     for (int u = 0; u < KU; ++u) {
+      // an item's terms (a masked item's are zero)
       double b2 = 0.0, bv = 0.0;
       if (sl[u].in && fabs(chg[u]) > 1e-10)
-        b2 = b2 + interp3v(e0[u], e1[u], e2v[u], sl[u].pe) * chg[u];
-      if (sl[u].in && vk[u] >= 0) bv = bv + interp3v(w0[u], w1[u], w2[u], sl[u].pv);
+        b2 = interp3v(e0[u], e1[u], e2v[u], sl[u].pe) * chg[u];
+      if (sl[u].in && vk[u] >= 0) bv = interp3v(w0[u], w1[u], w2[u], sl[u].pv);
       if (D.psg[pp[u]] < 0) {
         a2 -= b2;
         av -= bv;
@@ -1444,15 +1444,17 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
     int ty[KU];
 #pragma unroll
     for (int u = 0; u < KU; ++u) {
-      xy[u] = Dbl2{0.0, 0.0};
-      zq[u] = Dbl2{0.0, 0.0};
-      ty[u] = 0;
-      // the wave's first lane holds its smallest item: past the end there,
-      // past the end on every lane
-      if (__builtin_amdgcn_readfirstlane(i + u * nt) < nfwp) {
+      // the wave's first active lane holds its smallest item: item 0 is
+      // live there (the loop runs on live lanes only), and a later item
+      // past the end there is past the end on every lane
+      if (u == 0 || __builtin_amdgcn_readfirstlane(i + u * nt) < nfwp) {
         xy[u] = gload2(&grec[slot[u]].x);
         zq[u] = gload2(&grec[slot[u]].z);
         ty[u] = gload(grec_t + slot[u]);
+      } else {
+        xy[u] = Dbl2{0.0, 0.0};
+        zq[u] = Dbl2{0.0, 0.0};
+        ty[u] = 0;
       }
     }
     int vk[KU];
