@@ -206,7 +206,8 @@ def test_kvec_count_brute_force(oracle):
                     if k @ k <= rcp2:
                         cnt += 1
         assert L.orc_kvec_count(_dp(cell), _ip(kmax)) == cnt
-    # SURVEY figures for the workloads
+    # the k-vector counts of the C1 (25.8 A, kmax 8) and C2 (51.6 A, kmax 12)
+    # workloads
     assert L.orc_kvec_count(_dp(np.array([25.8, 0, 0, 0, 25.8, 0, 0, 0, 25.8])),
                             _ip(np.array([8, 8, 8], dtype=np.int32))) == 1258
     assert L.orc_kvec_count(_dp(np.array([51.6, 0, 0, 0, 51.6, 0, 0, 0, 51.6])),
