@@ -238,11 +238,117 @@ using namespace fmc;
 
 namespace {
 
+// Device blocks that chains and systems leaving a batch released, kept for
+// the next admissions.  hipFree waits for the device, so a serving loop
+// that frees a chain's buffers between slices stalls on it; a block goes
+// back here instead and is handed out again to a request of at least half
+// its size.  Per device; at most kCacheBytes are held (blocks past that
+// are freed), and the cache is emptied when the last batch is destroyed.
+// A block is released only when no launch can reach it any more (its
+// chain left the active set after its stream drained), and every buffer
+// the batch allocates is written in full before it is read, so a reused
+// block's old contents are never seen.
+class DevCache {
+ public:
+  static constexpr size_t kCacheBytes = size_t(16) << 30;
+
+  // a block of at least bytes on the current device (*cap: its size)
+  hipError_t take(size_t bytes, void** p, size_t* cap, int* dev) {
+    hipError_t e = hipGetDevice(dev);
+    if (e != hipSuccess) return e;
+    const size_t want = size_class(bytes);
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      auto it = free_.lower_bound({*dev, want});
+      if (it != free_.end() && it->first.first == *dev && it->first.second <= 2 * want) {
+        *p = it->second;
+        *cap = it->first.second;
+        held_ -= *cap;
+        free_.erase(it);
+        return hipSuccess;
+      }
+    }
+    e = hipMalloc(p, want);
+    if (e == hipErrorOutOfMemory) {   // cached blocks may be what is missing
+      (void)hipGetLastError();
+      drain();
+      e = hipMalloc(p, want);
+    }
+    *cap = e == hipSuccess ? want : 0;
+    if (e != hipSuccess) *p = nullptr;
+    return e;
+  }
+
+  void give(int dev, void* p, size_t cap) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      if (held_ + cap <= kCacheBytes) {
+        free_.emplace(std::make_pair(dev, cap), p);
+        held_ += cap;
+        return;
+      }
+    }
+    (void)hipFree(p);
+  }
+
+  void drain() {
+    std::multimap<std::pair<int, size_t>, void*> all;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      all.swap(free_);
+      held_ = 0;
+    }
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto& kv : all) {
+      (void)hipSetDevice(kv.first.first);
+      (void)hipFree(kv.second);
+    }
+    (void)hipSetDevice(cur);
+  }
+
+  // batches alive: the last one to go empties the cache
+  void batch_created() {
+    std::lock_guard<std::mutex> g(mu_);
+    ++batches_;
+  }
+  void batch_destroyed() {
+    bool last;
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      last = --batches_ == 0;
+    }
+    if (last) drain();
+  }
+
+ private:
+  // powers of two up to 2 MiB, then whole 2 MiB steps
+  static size_t size_class(size_t bytes) {
+    constexpr size_t kStep = size_t(2) << 20;
+    if (bytes >= kStep) return (bytes + kStep - 1) / kStep * kStep;
+    size_t c = 256;
+    while (c < bytes) c <<= 1;
+    return c;
+  }
+  std::mutex mu_;
+  std::multimap<std::pair<int, size_t>, void*> free_;
+  size_t held_ = 0;
+  int batches_ = 0;
+};
+
+// never destroyed: blocks may still be released while the process exits
+DevCache& dev_cache() {
+  static DevCache* c = new DevCache();
+  return *c;
+}
+
 struct DevBuf {
   void* p = nullptr;
-  size_t n = 0;
+  size_t n = 0;     // bytes asked for
+  size_t cap = 0;   // bytes of the block
+  int dev = 0;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) dev_cache().give(dev, p, cap);
   }
 };
 
@@ -251,7 +357,7 @@ int dput(std::vector<std::unique_ptr<DevBuf>>& keep, T** out, const T* h,
          size_t n) {
   auto b = std::make_unique<DevBuf>();
   const size_t bytes = std::max<size_t>(1, n) * sizeof(T);
-  BCHECK(hipMalloc(&b->p, bytes));
+  BCHECK(dev_cache().take(bytes, &b->p, &b->cap, &b->dev));
   b->n = bytes;
   if (h && n) BCHECK(hipMemcpy(b->p, h, n * sizeof(T), hipMemcpyHostToDevice));
   *out = static_cast<T*>(b->p);
@@ -350,7 +456,7 @@ struct fmc_batch {
   int precision = FMC_PREC_FP64;
   // tile shape: block size, register tile (0: streaming form), k-group
   // width and real-space items per pass
-  int bt = 0, kpt = 0, kg = 4, ku = 2;
+  int bt = 0, kpt = 0, kg = 4, ku = 1;
   bool tuned = false;    // false: autotune at the next launch
   bool force_stream = false;
   int tune_probes = 0;   // shapes the last autotune timed
@@ -928,6 +1034,7 @@ extern "C" int fmc_batch_create(fmc_batch** out, int max_chains, int precision) 
   BCHECK(hipMalloc(&b->d_dev, max_chains * sizeof(double)));
   BCHECK(hipMalloc(&b->d_dev1, sizeof(double)));
   BCHECK(hipStreamCreateWithFlags(&b->setup, hipStreamNonBlocking));
+  dev_cache().batch_created();
   *out = b.release();
   return FMC_OK;
 }
@@ -955,6 +1062,7 @@ extern "C" int fmc_batch_destroy(fmc_batch* b) {
   if (b->tev1) (void)hipEventDestroy(b->tev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
+  dev_cache().batch_destroyed();
   return FMC_OK;
 }
 
@@ -1389,14 +1497,16 @@ extern "C" int fmc_batch_set_tile(fmc_batch* b, int threads) {
     std::lock_guard<std::mutex> g(b->mu);
     b->bt = b->kpt = 0;
     b->kg = 4;
-    b->ku = 2;
+    b->ku = 1;
     b->tuned = false;
     b->force_stream = false;
     b->fixed_kpt = false;
     return FMC_OK;
   }
   const bool stream = (threads & FMC_TILE_STREAM) != 0;
-  return fmc_batch_set_tile_shape(b, threads & ~FMC_TILE_STREAM, stream ? 0 : -1, 4, 2);
+  // k-group width 4 is compiled for every register tile; one real-space
+  // item per pass is the faster of the two
+  return fmc_batch_set_tile_shape(b, threads & ~FMC_TILE_STREAM, stream ? 0 : -1, 4, 1);
 }
 
 extern "C" int fmc_batch_get_tile(const fmc_batch* b, int* threads) {

@@ -286,7 +286,7 @@ class Batch:
         check(self.L.fmc_batch_get_tile(self.h, C.byref(v)), "fmc_batch_get_tile")
         return v.value
 
-    def set_tile_shape(self, threads: int, kpt: int = -1, kgroup: int = 4, kunroll: int = 2):
+    def set_tile_shape(self, threads: int, kpt: int = -1, kgroup: int = 4, kunroll: int = 1):
         """The whole tile shape (fmc_batch_set_tile_shape)."""
         check(self.L.fmc_batch_set_tile_shape(self.h, threads, kpt, kgroup, kunroll),
               "fmc_batch_set_tile_shape")

@@ -320,8 +320,9 @@ int fmc_batch_timing(fmc_batch *b, int reset, double *ms, long long *launches);
 /* tile shape: threads per chain work-group (256 or 512); 0 =
  * autotuned.  S(k) is register-resident when the largest Nk fits the tile,
  * else the streaming form runs (S(k) in HBM/L2, any Nk); threads |
- * FMC_TILE_STREAM forces the streaming form (256 or 512).  Results do not
- * depend on the form. */
+ * FMC_TILE_STREAM forces the streaming form (256 or 512), with k-group
+ * width 4 and one real-space item per pass.  Results do not depend on the
+ * form. */
 #define FMC_TILE_STREAM 0x10000
 int fmc_batch_set_tile(fmc_batch *b, int threads);
 int fmc_batch_get_tile(const fmc_batch *b, int *threads);

@@ -773,3 +773,42 @@ def test_device_proposals_are_rodrigues_rotations(fmc, oracle):
         b.close()
     print(f"\nchecked {checked_ins} insertions and {checked_disp} displacements")
     assert checked_ins >= 2 and checked_disp >= 20
+
+
+def test_reused_device_blocks_give_the_same_trajectory(fmc):
+    """A chain admitted after another system's chain left the batch takes
+    that chain's released device blocks (the batch keeps them for reuse
+    instead of freeing them) and still moves exactly as in a batch of its
+    own: every buffer is written in full before it is read."""
+    from fastmc_amd.batch import Batch
+    from fastmc_amd.capi import PREC_MIXED
+    A = _porous(seed=4, guests=("co2",), press=(20.0,)).system_arrays()
+    X = _porous(seed=5, guests=("co2",), press=(30.0,)).system_arrays()
+    n = 600
+
+    def add(b, arrays, press, seed):
+        return b.add_chain(fmc.Context.from_arrays(arrays), 298.0, [press], seed,
+                           delr=arrays["delr"])
+
+    ref_b = Batch(1, PREC_MIXED)
+    ref_b.set_tile(512)
+    cid = add(ref_b, A, 20.0e5, (3, 1, 4, 1))
+    ref_b.enable_log(cid, n)
+    ref_b.run(n)
+    ref = ref_b.get_log(cid, n)
+    b = Batch(2, PREC_MIXED)
+    b.set_tile(512)
+    for round_ in range(2):
+        # a chain of another framework fills its blocks with its own state,
+        # then leaves: its chain and system blocks go back for reuse
+        x = add(b, X, 30.0e5, (9, 8, 7, 6 + round_))
+        b.run(400)
+        assert b.stats(x)["steps"] == 400
+        b.remove_chain(x)
+        cid = add(b, A, 20.0e5, (3, 1, 4, 1))
+        b.enable_log(cid, n)
+        b.run(n)
+        assert np.array_equal(b.get_log(cid, n), ref), f"round {round_}"
+        b.remove_chain(cid)
+    b.close()
+    ref_b.close()
