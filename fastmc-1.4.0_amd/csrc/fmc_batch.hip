@@ -827,21 +827,28 @@ int sync_tables(fmc_batch* b) {
 // dynamic LDS of a form: the erfc table, (a_k, packed k index unless
 // streaming) + X.Y tables; FMC_ERR_UNSUPPORTED when it and the kernel's
 // static LDS exceed the device's per-work-group limit
-int lds_need(fmc_batch* b, const KCfg* cfg, size_t* out) {
+// The erfc table goes to LDS when it fits beside the rest (*ercap: its
+// entries, 0 when real space reads it from global memory instead, which
+// gives the same values): a system whose tables only just fit keeps the
+// register form rather than the whole batch falling back to streaming.
+int lds_need(fmc_batch* b, const KCfg* cfg, size_t* out, int* ercap = nullptr) {
   const size_t elt = b->precision == FMC_PREC_MIXED ? 8 : 16;
   const int nkcap = b->max_nk;
-  size_t lds = static_cast<size_t>(b->max_pos) * b->max_nlm * elt +
-               static_cast<size_t>(dyn_erc_slots(b->max_grid)) * sizeof(double);
+  size_t lds = static_cast<size_t>(b->max_pos) * b->max_nlm * elt;
   if (cfg->kpt > 0)
     lds += static_cast<size_t>(dyn_ak_slots(nkcap)) * sizeof(double) +
            static_cast<size_t>(dyn_kv_slots(nkcap)) * sizeof(int);
+  const size_t erc = static_cast<size_t>(dyn_erc_slots(b->max_grid)) * sizeof(double);
   int dev = 0, lmax = 0;
   BCHECK(hipGetDevice(&dev));
   BCHECK(hipDeviceGetAttribute(&lmax, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
   hipFuncAttributes fa;
   BCHECK(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(cfg->fn)));
-  if (lds + fa.sharedSizeBytes > static_cast<size_t>(lmax)) return FMC_ERR_UNSUPPORTED;
-  *out = lds;
+  const size_t cap = static_cast<size_t>(lmax);
+  if (lds + fa.sharedSizeBytes > cap) return FMC_ERR_UNSUPPORTED;
+  const bool with_erc = lds + erc + fa.sharedSizeBytes <= cap;
+  *out = lds + (with_erc ? erc : 0);
+  if (ercap) *ercap = with_erc ? b->max_grid : 0;
   return FMC_OK;
 }
 
@@ -852,11 +859,11 @@ int launch(fmc_batch* b, long long n, int bt, int kpt, int kg, int ku,
   if (b->active_ids.empty()) return FMC_OK;
   if (kpt > 0 && bt * kpt < b->max_nk) return FMC_ERR_UNSUPPORTED;
   size_t lds = 0;
-  int r;
-  if ((r = lds_need(b, cfg, &lds))) return r;
+  int r, ercap = 0;
+  if ((r = lds_need(b, cfg, &lds, &ercap))) return r;
   hipLaunchKernelGGL(cfg->fn, dim3(static_cast<unsigned>(b->active_ids.size())),
                      dim3(cfg->bt), lds, b->stream, b->d_sys, b->d_chains, b->d_ids, n,
-                     b->max_nk, b->max_pos, b->max_grid, budget_ticks);
+                     b->max_nk, b->max_pos, ercap, budget_ticks);
   BCHECK(hipGetLastError());
   return FMC_OK;
 }

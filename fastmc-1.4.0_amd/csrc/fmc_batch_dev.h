@@ -1370,7 +1370,8 @@ __device__ __forceinline__ double interp3v(double v0, double v1, double v2, doub
 // those of pair_fp64_vk per item, whatever KU is.
 template <int KU, bool TRI>
 __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const BChain& C,
-                                           const MoveDesc& D, const double* erc_l, int t,
+                                           const MoveDesc& D, const double* erc_l,
+                                           bool erc_in_lds, int t,
                                            int nt, double& e2, double& ev) {
   // no contraction: an item's terms round the same way in every kernel
   // shape that inlines this (the oracle's build contracts nothing either)
@@ -1381,7 +1382,10 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
   const bool same_grid = rdrewd == rdlrpot;
   const int imax = S.mxgrid - 2;
   const size_t mx1 = static_cast<size_t>(S.mxgrid) + 1;
-  const double* __restrict__ erc = erc_l;   // the erfc table, copied to LDS
+  // the erfc table: its LDS copy, or when that did not fit the table
+  // itself (the same values)
+  const double* __restrict__ erc = erc_l;
+  const double* __restrict__ erc_g = S.erc;
   const double* __restrict__ vvv = S.vvv;
   const int ntp = S.ntpatm;
   const bool ldsrow = ntp <= kLdsTypes;
@@ -1401,12 +1405,19 @@ __device__ __forceinline__ void real_space(const BSys& Y, const SysQ& Z, const B
     double e0[KU], e1[KU], e2v[KU], w0[KU], w1[KU], w2[KU];
 #pragma unroll
     for (int u = 0; u < KU; ++u) {
-      const double* te = erc + sl[u].le;
       const double* tv =
           vvv + static_cast<size_t>(vk[u] >= 0 && sl[u].in ? vk[u] : 0) * mx1 + sl[u].lv;
-      e0[u] = te[0];
-      e1[u] = te[1];
-      e2v[u] = te[2];
+      if (erc_in_lds) {
+        const double* te = erc + sl[u].le;
+        e0[u] = te[0];
+        e1[u] = te[1];
+        e2v[u] = te[2];
+      } else {
+        const double* te = erc_g + sl[u].le;
+        e0[u] = gload(te);
+        e1[u] = gload(te + 1);
+        e2v[u] = gload(te + 2);
+      }
       w0[u] = gload(tv);
       w1[u] = gload(tv + 1);
       w2[u] = gload(tv + 2);
@@ -1647,7 +1658,8 @@ __global__ void __launch_bounds__(BT)
   // dynamic LDS (dyn_erc_slots): the erfc table, a_k [nkcap] and the
   // packed k index [nkcap] (register form), then the XY tables
   extern __shared__ __attribute__((aligned(16))) double dyn_lds[];
-  // the system's erfc table (ercap entries reserved), read by real space
+  // the system's erfc table (ercap entries reserved; 0 when it did not fit
+  // and real space reads the table itself), read by real space
   double* erc_l = dyn_lds;
   double* ak_l = dyn_lds + dyn_erc_slots(ercap);
   int* kv_l = reinterpret_cast<int*>(ak_l + (STREAM ? 0 : dyn_ak_slots(nkcap)));
@@ -1660,7 +1672,8 @@ __global__ void __launch_bounds__(BT)
   // this thread owns k = tid + j*BT, j < kpt = ceil(nk/BT); the index
   // (l, m) is precomputed into kv's upper bits
   const int kpt = (nk + BT - 1) / BT;
-  for (int i = tid; i <= Y.S.mxgrid; i += BT) erc_l[i] = Y.S.erc[i];
+  if (ercap > 0)
+    for (int i = tid; i <= Y.S.mxgrid; i += BT) erc_l[i] = Y.S.erc[i];
   if (!STREAM) {
     for (int k = tid; k < nk; k += BT) {
       ak_l[k] = Y.ak[k];
@@ -1888,9 +1901,9 @@ __global__ void __launch_bounds__(BT)
       }
       if (tid < RT) {
         if (Y.S.imcon == 3)
-          real_space<KU, true>(Y, Z, C, D, erc_l, tid, RT, e2, ev);
+          real_space<KU, true>(Y, Z, C, D, erc_l, ercap > 0, tid, RT, e2, ev);
         else
-          real_space<KU, false>(Y, Z, C, D, erc_l, tid, RT, e2, ev);
+          real_space<KU, false>(Y, Z, C, D, erc_l, ercap > 0, tid, RT, e2, ev);
       }
     }
     // block reductions (fixed tree): DPP within each wave, then the waves'
