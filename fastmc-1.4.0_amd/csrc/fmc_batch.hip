@@ -195,28 +195,45 @@ __global__ void __launch_bounds__(kRecBT) recompute_kernel(RecomputeParams P) {
 }
 
 // S(k) of n atoms (FP64, direct phases): the framework part used by resync
+// The framework's S(k): a block takes 16 k-vectors, its threads 16 slices
+// of the atoms each (atom a in slice a mod 16), and the slices' partial
+// sums are added in slice order, so the result is the same on every run.
+constexpr int kSfacK = 16, kSfacSlices = 256 / kSfacK;
 __global__ void __launch_bounds__(256) sfac_kernel(int nk, const int* kv,
                                                    const double* rcell, int n,
                                                    const double* x, const double* y,
                                                    const double* z, const double* q,
                                                    double* re, double* im) {
-  const int k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= nk) return;
-  const int v = kv[k];
-  const int l = ul(v), m = um(v), nn = un(v);
+  __shared__ double part[2][kSfacSlices][kSfacK];
+  const int kk = threadIdx.x % kSfacK, sl = threadIdx.x / kSfacK;
+  const int k = blockIdx.x * kSfacK + kk;
   double sx = 0.0, sy = 0.0;
-  for (int a = 0; a < n; ++a) {
-    double ph = l * (rcell[0] * x[a] + rcell[3] * y[a] + rcell[6] * z[a]) +
-                m * (rcell[1] * x[a] + rcell[4] * y[a] + rcell[7] * z[a]) +
-                nn * (rcell[2] * x[a] + rcell[5] * y[a] + rcell[8] * z[a]);
-    ph -= rint(ph);
-    double sv, cv;
-    sincospi(2.0 * ph, &sv, &cv);
-    sx += q[a] * cv;
-    sy += q[a] * sv;
+  if (k < nk) {
+    const int v = kv[k];
+    const int l = ul(v), m = um(v), nn = un(v);
+    for (int a = sl; a < n; a += kSfacSlices) {
+      double ph = l * (rcell[0] * x[a] + rcell[3] * y[a] + rcell[6] * z[a]) +
+                  m * (rcell[1] * x[a] + rcell[4] * y[a] + rcell[7] * z[a]) +
+                  nn * (rcell[2] * x[a] + rcell[5] * y[a] + rcell[8] * z[a]);
+      ph -= rint(ph);
+      double sv, cv;
+      sincospi(2.0 * ph, &sv, &cv);
+      sx += q[a] * cv;
+      sy += q[a] * sv;
+    }
   }
-  re[k] = sx;
-  im[k] = sy;
+  part[0][sl][kk] = sx;
+  part[1][sl][kk] = sy;
+  __syncthreads();
+  if (sl == 0 && k < nk) {
+    double tx = 0.0, ty = 0.0;
+    for (int s2 = 0; s2 < kSfacSlices; ++s2) {
+      tx += part[0][s2][kk];
+      ty += part[1][s2][kk];
+    }
+    re[k] = tx;
+    im[k] = ty;
+  }
 }
 
 }  // namespace fmc
@@ -627,7 +644,7 @@ int build_system(fmc_batch* b, const fmc_ctx* ctx, uint64_t key,
       (r = dput(K, &sim, static_cast<double*>(nullptr), nk)) ||
       (r = dput(K, &rc, s.rcell, 9)))
     return r;
-  hipLaunchKernelGGL(sfac_kernel, dim3((nk + 255) / 256), dim3(256), 0, b->setup,
+  hipLaunchKernelGGL(sfac_kernel, dim3((nk + kSfacK - 1) / kSfacK), dim3(256), 0, b->setup,
                      nk, kv, rc, nfw, fx, fy, fz, fq, sre, sim);
   BCHECK(hipGetLastError());
   Y.sfw_re = sre;
