@@ -647,7 +647,10 @@ def test_budget_launches_do_not_change_trajectories(fmc):
         a.launch_budget(1300, 2000.0)
         a.sync()
     steps = [a.stats(cid)["steps"] for cid in range(3)]
-    assert min(steps) > 0 and len(set(steps)) > 1   # chains moved at their own speed
+    # every launch ended on its time budget, not on its move cap, so each
+    # chain's moves were split where its own clock said (three chains of
+    # nearly equal speed may well stop at the same counts)
+    assert min(steps) > 0 and max(steps) < 3 * 1300, steps
     for cid in range(3):
         # the same moves for this chain in launches of 111
         bb, _ = _two_batches(fmc, n_chains=3, recompute_every=250)
