@@ -92,36 +92,35 @@ __global__ void __launch_bounds__(256) grid_count_kernel(GridParams P) {
 // ordered fill: candidates of a cell in increasing atom index, then NaN
 // records up to the capacity
 __global__ void __launch_bounds__(256) grid_fill_kernel(GridParams P) {
-  __shared__ int scan[256];
-  __shared__ int base;
+  // per chunk of 256 atoms, each wave's count of candidates (two buffers:
+  // a chunk's counts are read before the barrier that lets the next chunk
+  // but one write them again, so one barrier per chunk suffices)
+  __shared__ int wcnt[2][4];
   const int c = blockIdx.x;
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
   double cx, cy, cz;
   grid_center(P, c, cx, cy, cz);
   Rec* rec = P.rec + static_cast<size_t>(c) * P.cap;
   int* rt = P.rtype + static_cast<size_t>(c) * P.cap;
-  if (threadIdx.x == 0) base = 0;
-  __syncthreads();
-  for (int a0 = 0; a0 < P.S.nfw; a0 += 256) {
+  int base = 0;   // candidates before this chunk (the same on every thread)
+  int buf = 0;
+  for (int a0 = 0; a0 < P.S.nfw; a0 += 256, buf ^= 1) {
     const int a = a0 + threadIdx.x;
-    int f = 0;
-    if (a < P.S.nfw)
-      f = within_reach(P, P.S.fx[a] - cx, P.S.fy[a] - cy, P.S.fz[a] - cz) ? 1 : 0;
-    scan[threadIdx.x] = f;
+    bool f = false;
+    if (a < P.S.nfw) f = within_reach(P, P.S.fx[a] - cx, P.S.fy[a] - cy, P.S.fz[a] - cz);
+    // ascending atom order: the wave's candidates below this lane, then
+    // the waves before this one
+    const unsigned long long bal = __ballot(f);
+    if (lane == 0) wcnt[buf][w] = __popcll(bal);
     __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      const int v = threadIdx.x >= o ? scan[threadIdx.x - o] : 0;
-      __syncthreads();
-      scan[threadIdx.x] += v;
-      __syncthreads();
-    }
-    const int slot = base + scan[threadIdx.x] - 1;
+    int off = base;
+    for (int i = 0; i < w; ++i) off += wcnt[buf][i];
+    const int slot = off + __popcll(bal & ((1ull << lane) - 1ull));
     if (f && slot < P.cap) {
       rec[slot] = {P.S.fx[a], P.S.fy[a], P.S.fz[a], P.S.fq[a]};
       rt[slot] = P.S.ftype[a];
     }
-    __syncthreads();
-    if (threadIdx.x == 255) base += scan[255];
-    __syncthreads();
+    base += wcnt[buf][0] + wcnt[buf][1] + wcnt[buf][2] + wcnt[buf][3];
   }
   const double nan = __builtin_nan("");
   for (int i = base + threadIdx.x; i < P.cap; i += 256) {
